@@ -1,0 +1,270 @@
+/*
+ * recview_hip.h -- C ABI of librecview_hip.so, the MI355X filtered
+ * back-projection engine that drops into RecView's reconstruction path.
+ *
+ * Part 1 replaces cudaReconst.h + reconstinfo.h of the reference: the same
+ * entry-point names, argument meanings and error behaviour, with CUDA types
+ * replaced by HIP-neutral ones (no CUDA or HIP header is needed to include
+ * this file).  Part 2 is the native batched API (no counterpart in the
+ * reference) used for multi-slice, mixed-precision and zoomed work.
+ *
+ * Every Part-1 declaration cites the reference declaration it replaces.
+ */
+#ifndef RECVIEW_HIP_H
+#define RECVIEW_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+#ifndef __cplusplus
+#include <stdbool.h>
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- types -- */
+
+/* cudaStream_t (reconstinfo.h:73) -> opaque handle (a hipStream_t inside). */
+typedef struct rv_stream_opaque* rv_stream_t;
+/* cufftHandle (reconstinfo.h:72) -> opaque; the engine has no FFT plan object. */
+typedef void* rv_fft_plan_t;
+/* float2 (reconstinfo.h:60) */
+typedef struct { float x, y; } rv_float2;
+/* cudaError_t (cudaReconst.h:58-59) -> int, 0 = success */
+typedef int rv_error_t;
+typedef int RV_BOOL;               /* BOOL of reconstinfo.h:41 */
+
+/* Device-query error flags, cudaReconst.h:8-13 (same values). */
+#define CUDA_ERROR_DEVICEINFO_MASK 0x0000ffff
+#define CUDA_ERROR_INSUFFICIENT_DRIVER 0x00010000
+#define CUDA_ERROR_INSUFFICIENT_COMPUTE_CAPABILITY 0x00020000
+#define CUDA_ERROR_VIRTUAL_DEVICE_DETECTED 0x00040000
+#define CUDA_ERROR_DEVICE_GETPROPERTY 0x00080000
+#define CUDA_ERROR_DEVICE_GETCOUNT 0x00100000
+
+/* constants.h:43-45 */
+#define RECONST_INFO_IDLE 0
+#define RECONST_INFO_BUSY 1
+#define RECONST_INFO_ERROR 2
+/* DlgQueue.h:23-26, StdAfx.h:29 */
+#define RQFLAGS_USEONLYEVENFRAMES 16
+#define RQFLAGS_USEONLYODDFRAMES 32
+#define RQFLAGS_SINOGRAMKEPT 128
+#define CGAZODOC_BINC_SAMPLE 1
+
+/* RECONST_INFO, reconstinfo.h:21-101: same field names and order, so that the
+ * caller's assignments compile unchanged.  Device buffers are owned by the
+ * library, grow only, and persist across calls until CudaReconstMemFree /
+ * CudaReconstResourceFree.  The engine keeps the whole filtered sinogram
+ * (d_px) instead of a 16x int LUT: d_igp stays NULL on the native path and is
+ * only allocated by the CudaReconstHostFFT / kernel-wrapper compatibility
+ * path.  d_strip holds all selected int16 rows of the slice. */
+struct RECONST_INFO {
+    unsigned char iStatus;
+    int ixdim;
+    int iInterpolation;
+    double center;
+    int iSinoCenter;
+    int iLenSinogr;
+    int* iReconst;
+    int iStartSino;
+    int iStepSino;
+    unsigned int* pDoc;
+    int* nSinogr;
+    char dataName[61];
+    char* bInc;
+    float* fdeg;
+    float* fexp;
+    int iMultiplex;
+    int iOffset;
+    int maxSinogrLen;
+    short** iSinogr;
+    float* fFilter;
+    unsigned int dReconFlags;
+    float fTiltAngle;
+    int drStart;
+    int drEnd;
+    double drX;
+    double drY;
+    RV_BOOL drOmit;
+    unsigned int threadID;
+    uintptr_t hThread;
+    unsigned int max_d_ifp;
+    unsigned int max_d_igp;
+    int* d_ifp;
+    int* d_igp;
+    float* d_filt;
+    unsigned int max_d_filt;
+    short* d_strip;
+    unsigned int max_d_strip;
+    rv_float2* d_p;
+    unsigned int max_d_p;
+    float* d_px;
+    unsigned int max_d_px;
+    rv_fft_plan_t fftplan;
+    unsigned int ifftdim;
+    rv_stream_t stream1, stream2;
+    uint64_t i64result;
+    uint64_t i64sum;
+    short** ppRef;
+    short** ppQry;
+    unsigned int uiMaxRef;
+    unsigned int uiMaxQry;
+    void* pcxyz1;
+    void* pcxyz2;
+    int iLsqBin;
+    int iLsqBxref;
+    int* piDrift;
+    int iFlag;
+};
+typedef struct RECONST_INFO RECONST_INFO;
+
+/* ------------------------------------------------- Part 1: host routines -- */
+
+/* cudaReconst.h:16 (impl cudaReconstHost.cpp:629-813): reconstructs one slice
+ * (int16 rows ri->iSinogr selected by bInc/fdeg) into the int32 image
+ * ri->iReconst, with the reference's LUT16 integer semantics.  Sets
+ * ri->iStatus = RECONST_INFO_ERROR on allocation failure; never throws. */
+void CudaReconstHost(RECONST_INFO* ri, int idev, bool bReport, bool bEnStream);
+/* cudaReconst.h:17 (cudaReconstHost.cpp:142-203) */
+void CudaReconstMemFree(RECONST_INFO* ri);
+/* cudaReconst.h:18 (cudaReconstHost.cpp:205-214) */
+void CudaReconstResourceFree(RECONST_INFO* ri, bool bCudaDeviceReset);
+/* cudaReconst.h:19 (cudaReconstHost.cpp:221-291): the CUDAFFT-build variant:
+ * zero padding, float zoom interpolation, LUT written per projection. */
+void CudaReconstHostFFT(RECONST_INFO* ri, int idev, bool bReport);
+/* cudaReconst.h:20-26 (cudaReconst.cu:449-538) */
+int GetCudaDeviceCount(int iMinComputeCapability);
+int GetCudaMaxThreadsPerBlock(int iDevice);
+int GetCudaWarpSize(int iDevice);
+int GetCudaDeviceName(int iDevice, char* pcName, int iszcName);
+int GetCudaDeviceComputingCapability(int iDevice, int* piMajor, int* piMinor);
+int GetCudaNumberOfCores(int iDevice, int* piCores, int* piProcessors);
+int GetCudaClockRate(int iDevice, int* piClockRate, int* piMemRate);
+/* cudaReconst.h:28: sinogram generation is not provided by this engine; the
+ * call logs error 28901 and sets ri->iStatus = RECONST_INFO_ERROR. */
+void CudaSinogramHost(RECONST_INFO* ri, int idev, bool bReport);
+
+/* ------------------------------------------ Part 1: kernel wrappers -- */
+
+/* cudaReconst.h:31-32 (cudaReconst.cu:158-192): filters one int16 strip
+ * (CUDAFFT semantics) and writes its 16N-entry int LUT to d_igp.  d_p
+ * (>= ndim rv_float2) is scratch; fftplan is ignored (may be NULL). */
+void CudaDeconv(int ixdim, int iIntpDim, int ndim, float center, float* d_filt,
+                short* d_strip, int* d_igp, rv_float2* d_p, rv_fft_plan_t* fftplan);
+/* cudaReconst.h:33-34 (cudaReconst.cu:195-228): adds one projection's LUT
+ * d_igp[16*ixdimp] into the int32 image d_ifp[ixdimp^2]. */
+void CudaBackProjStream(int ixdimp, float center, int iCenterOffset, int iIntpDim, double theta,
+                        int* d_ifp, int* d_igp, rv_stream_t stream);
+/* cudaReconst.h:35 (cudaReconst.cu:230-246): LUT build from a filtered row. */
+void CudaSinoPx2igpStream(int ixdimp, int* d_igp, float* d_px, rv_stream_t stream);
+
+/* cudaReconst.h:58-59 (cudaReconstHost.cpp:216-218): pinned host memory. */
+rv_error_t CUDA_MALLOC_HOST_INT(int** ptr, size_t size);
+rv_error_t CUDA_FREE_HOST(void* ptr);
+
+/* cudaReconst.cu:19: block size of the compatibility kernels, written by the
+ * application (gazo.cpp:549-550).  The tiled kernels use their own shapes. */
+extern int blocksize;
+
+/* cudaReconst.h:40: progress/abort callback imported from the application
+ * (general.cpp:2352).  Resolved as a weak symbol; a host without it (Python,
+ * tests) installs one with RecviewSetDlgCtrl.  Returning true aborts. */
+typedef bool (*rv_dlgctrl_fn)(RECONST_INFO* ri, int iProgStep, int iSino, int* pCurrStep);
+void RecviewSetDlgCtrl(rv_dlgctrl_fn fn);
+
+/* Error ring (replaces CError, cerror.cpp:35-43): codes 288xx / 300xx as the
+ * reference logs them.  Returns the number copied (most recent last). */
+int RecviewGetErrors(int* codes, int maxn);
+void RecviewClearErrors(void);
+/* Last HIP/library error message (thread-local). */
+const char* RecviewLastMessage(void);
+
+/* Options of the Part-1 host routines (no counterpart: the reference fixes
+ * them at compile time).  lut_formula 1 = cudaReconst.cu:237-239 (default),
+ * 0 = general.cpp:2131-2136.  index_rule 1 = trunc then range check
+ * (cudaReconst.cu:36-38, default), 0 = float mask (projx64.asm:261-266). */
+void RecviewSetCompatOptions(int lut_formula, int index_rule);
+
+/* ---------------------------------------------- Part 2: native batched API -- */
+
+#define RV_INTERP_LINEAR 0     /* production: true linear detector interpolation, FP32 */
+#define RV_INTERP_LUT16_F32 1  /* reference 1/16 sub-sample LUT, float values/accumulation */
+#define RV_INTERP_LUT16_I32 2  /* reference LUT16 with x3000 int16 input, x100 int LUT, int32 sum */
+
+#define RV_STORE_F32 0         /* filtered sinogram kept in FP32 */
+#define RV_STORE_F16 1         /* FP16 with per-row power-of-two scale */
+#define RV_STORE_BF16 2        /* BF16 with per-row power-of-two scale */
+
+#define RV_SINO_I16 0          /* input rows int16 (x LOG_SCALE) */
+#define RV_SINO_F32 1          /* input rows float line integrals */
+
+#define RV_PAD_REPLICATE 0     /* host path: edge replicate, placement n/2-1 */
+#define RV_PAD_ZERO 1          /* CUDAFFT path: zeros, placement n/2 */
+
+typedef struct {
+    int W;               /* detector width (after binning / stitching) */
+    int P;               /* number of projections (rows) */
+    int iInterpolation;  /* 2 = none, 3/4/5 = zoom x2/x4/x8 of the whole FOV (reference) */
+    double center;       /* RECONST_INFO.center semantics: (c_user + 0.5) / bin */
+    float tilt;          /* degrees added to each angle */
+    const float* fdeg;   /* host array [P] of angles in degrees */
+    int filter;          /* 0 Hann 1 Hamming 2 Ramp 3 Parzen 4 Butterworth 5 Shepp-Logan */
+    double cutoff;       /* filter cutoff (0.5 default) */
+    double order;        /* Butterworth order */
+    int sino_dtype;      /* RV_SINO_I16 / RV_SINO_F32 */
+    int pad_mode;        /* RV_PAD_REPLICATE / RV_PAD_ZERO */
+    /* Output grid.  out_n = 0: the reference grid (N = W*zoom square).
+     * Otherwise an out_n x out_n grid of pitch 1/pix_zoom detector samples
+     * whose pixel (0,0) centre sits at (x0, y0) of the reference z=1 grid. */
+    int out_n;
+    double pix_zoom;
+    double x0, y0;
+} RvSliceSpec;
+
+typedef struct {
+    int interp;          /* RV_INTERP_* */
+    int store;           /* RV_STORE_* (LINEAR only) */
+    int lut_formula;     /* LUT16 modes: 0 CPU formula, 1 CUDA formula */
+    int index_rule;      /* LUT16 modes: 0 float mask (floor), 1 trunc */
+    int persistent;      /* 1: persistent work-queue back-projection */
+    int tile;            /* 0 = autotuned, else tile config id */
+    int angles_per_chunk;/* 0 = default */
+} RvOptions;
+
+typedef struct rv_plan_opaque* rv_plan_t;
+
+/* Builds a plan for nslices slices on device `device`: geometry, filter
+ * tables, angle tables, filtered-sinogram workspace and the tile work list.
+ * Returns 0 or a negative error (message in RecviewLastMessage). */
+int rvPlanCreate(rv_plan_t* plan, int device, const RvSliceSpec* specs, int nslices,
+                 const RvOptions* opt);
+/* Reconstructs every slice: d_sino[s] points to device rows [P][W] of the
+ * slice's sino_dtype, d_out[s] to a device image (float, or int32 in the
+ * LUT16_I32 mode) of out_n^2 (or N^2) elements.  Asynchronous on `stream`. */
+int rvPlanExecute(rv_plan_t plan, const void* const* d_sino, void* const* d_out, rv_stream_t stream);
+/* Captures rvPlanExecute with these pointers into a hipGraph once and
+ * replays it on every later call with the same pointers. */
+int rvPlanExecuteGraph(rv_plan_t plan, const void* const* d_sino, void* const* d_out, rv_stream_t stream);
+/* Stage-level access for tests and benchmarks. */
+int rvPlanFilter(rv_plan_t plan, const void* const* d_sino, rv_stream_t stream);
+int rvPlanBackproject(rv_plan_t plan, void* const* d_out, rv_stream_t stream);
+/* Device pointer of slice s's filtered rows (FP32 [P][N], or the FP16/BF16
+ * rows with their per-row scales), and their row length. */
+int rvPlanFilteredRows(rv_plan_t plan, int s, void** rows, float** scales, int* row_len);
+/* Uploads filtered FP32 rows [P][row_len] for slice s (BP-only testing). */
+int rvPlanSetFilteredRows(rv_plan_t plan, int s, const float* d_rows, rv_stream_t stream);
+/* Plan facts: image side, number of tiles, tile shape, workspace bytes. */
+int rvPlanInfo(rv_plan_t plan, int s, int* out_n, int* n_fft, int* tiles, int* tile_w, int* tile_h);
+void rvPlanDestroy(rv_plan_t plan);
+
+/* The filter design of gazoDoc.cpp:3747-3812 (plus 5 = Shepp-Logan, new). */
+void RecviewSetFilter(int ndim, int ifilter, double cutoff, double order, float* out);
+/* Geometry of cudaReconstHost.cpp:635-658: returns n (ndim) and writes N, zoom. */
+int RecviewGeometry(int ixdim, int iInterpolation, int* N, int* zoom);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

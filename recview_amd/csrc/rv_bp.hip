@@ -1,0 +1,378 @@
+// rv_bp.hip -- tiled back-projection for CDNA4 (gfx950).
+//
+// Replaces projAtomicKernel9 / CudaBackProjStream (cudaReconst.cu:31-39,
+// 195-228: one launch per projection, a global atomic read-modify-write of the
+// whole image per projection) with one launch per batch of slices:
+//
+//  * a workgroup (4 wave64) owns a TW x TH pixel tile and keeps its sums in
+//    VGPRs across all projections, so the image is written once, coalesced;
+//  * projections are processed in chunks: for every angle of the chunk the
+//    detector segment that the tile can see (<= SEG samples) is staged in LDS,
+//    together with the per-angle geometry (FP64 on the host, reduced here to a
+//    tile-relative FP32 origin so the FP32 coordinate never exceeds SEG);
+//  * within a wave, the 64 lanes cover an 8x8 pixel block, so the LDS reads of
+//    one instruction fall in a window of about 12 consecutive samples;
+//  * work items (slice, tile) come from one grid or, persistently, from an
+//    atomic work counter over a cost-sorted list (uneven slices in one launch).
+//
+// Modes:
+//  BP_LINEAR  : production.  LDS holds per-sample pairs (a_j, d_j) with
+//               d_j = p[k+1]-p[k] and a_j = p[k] - j*d_j (j = sample index
+//               within the segment), so value(u) = a_j + u*d_j for the
+//               segment-relative coordinate u: one FMA, no fract.
+//  BP_LUT_F32 : the reference's 16-sub-sample LUT (cudaReconst.cu:230-240)
+//               in float, float sums; index t = ix*fcos + (iy*fsin + foffset)
+//               in the reference's FP32 operation order.
+//  BP_LUT_I32 : the reference LUT in int32 (x BACKPROJ_SCALE), int32 sums --
+//               bit-exact with the reference integer path.
+// This file is compiled with -ffp-contract=off; FMAs are written explicitly.
+#include "rv_internal.h"
+#include <hip/hip_fp16.h>
+#include <hip/hip_bf16.h>
+
+namespace rv {
+
+template <int PX, int PY>
+struct TileCfg {
+    static constexpr int WX = 8 * PX, WY = 8 * PY;   // wave block
+    static constexpr int TW = 2 * WX, TH = 2 * WY;   // 2 x 2 waves
+    static constexpr int T = 256;
+    // detector footprint of a tile: (TW-1)|cos| + (TH-1)|sin| + 3 margin samples
+    static constexpr int SEG = ((int)(1.0 + 1.41421356 * (TW > TH ? TW : TH)) + 3 + 3) / 4 * 4;
+};
+
+template <int MODE>
+struct AccT { using type = float; };
+template <>
+struct AccT<BP_LUT_I32> { using type = int; };
+
+__device__ __forceinline__ float load_row_val(const BpSlice& S, int row, int k)
+{
+    const long long off = (long long)row * S.row_stride + k;
+    if (S.store == 0) return ((const float*)S.rows)[off];
+    const float sc = S.scales[row];
+    if (S.store == 1) return __half2float(((const __half*)S.rows)[off]) * sc;
+    return __bfloat162float(((const __hip_bfloat16*)S.rows)[off]) * sc;
+}
+
+template <int MODE, int PX, int PY>
+__global__ void __launch_bounds__(256) bp_tiled_kernel(BpLaunch L)
+{
+    using C = TileCfg<PX, PY>;
+    using Acc = typename AccT<MODE>::type;
+    constexpr int SEG = C::SEG;
+    constexpr int EPA = (MODE == BP_LINEAR) ? SEG : 16 * SEG;   // LDS entries per angle
+    const int chunk = L.chunk;
+
+    // LDS layout (pointer arithmetic only, so every access stays a ds_* op):
+    // tab[chunk] float4 | s0s[chunk] int | rws[chunk] int | pad | data
+    extern __shared__ float4 lds4[];
+    float4* tab = lds4;
+    int* s0s = (int*)(lds4 + chunk);
+    int* rws = s0s + chunk;
+    void* data = (void*)(lds4 + chunk + (2 * chunk + 3) / 4);
+    __shared__ int item_sh;
+
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int wx0 = (wave & 1) * C::WX, wy0 = (wave >> 1) * C::WY;
+    const int lx = lane & 7, ly = lane >> 3;
+
+    int iter = 0;
+    while (true) {
+        int item;
+        if (L.counter) {
+            if (tid == 0) item_sh = (int)atomicAdd(L.counter, 1u);
+            __syncthreads();
+            item = item_sh;
+            __syncthreads();
+        } else {
+            item = (iter == 0) ? (int)blockIdx.x : L.total_items;
+        }
+        iter++;
+        if (item >= L.total_items) break;
+        if (L.item_order) item = L.item_order[item];
+
+        // slice lookup: last slice with tile_base <= item
+        int lo = 0, hi = L.nslices - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (L.slices[mid].tile_base <= item) lo = mid; else hi = mid - 1;
+        }
+        const BpSlice S = L.slices[lo];
+        const int t = item - S.tile_base;
+        const int X0 = (t % S.tiles_x) * C::TW;
+        const int Y0 = (t / S.tiles_x) * C::TH;
+        const int N = S.row_len;
+
+        Acc acc[PY][PX];
+#pragma unroll
+        for (int j = 0; j < PY; j++)
+#pragma unroll
+            for (int i = 0; i < PX; i++) acc[j][i] = (Acc)0;
+
+        // per-lane pixel offsets inside the tile
+        float dxf[PX], dyf[PY];
+#pragma unroll
+        for (int i = 0; i < PX; i++) dxf[i] = (float)(wx0 + lx + 8 * i);
+#pragma unroll
+        for (int j = 0; j < PY; j++) dyf[j] = (float)(wy0 + ly + 8 * j);
+
+        for (int a0 = 0; a0 < S.nang; a0 += chunk) {
+            // ---- per-angle tile geometry
+            if (tid < chunk) {
+                const int ga = a0 + tid;
+                if (ga < S.nang) {
+                    const AngleEntry e = S.ang[ga];
+                    if (MODE == BP_LINEAR) {
+                        const double ut = e.u0 + (double)X0 * e.dux + (double)Y0 * e.duy;
+                        const double umin = ut + fmin(0.0, (C::TW - 1) * e.dux) + fmin(0.0, (C::TH - 1) * e.duy);
+                        const int s0 = (int)floor(umin) - 1;
+                        tab[tid] = make_float4((float)e.dux, (float)e.duy, (float)(ut - (double)s0), 0.f);
+                        s0s[tid] = s0;
+                    } else {
+                        const double fc = e.fcos, fs = e.fsin;
+                        const double tt = (double)X0 * fc + (double)Y0 * fs + (double)e.foffset;
+                        const double tmin = tt + fmin(0.0, (C::TW - 1) * fc) + fmin(0.0, (C::TH - 1) * fs);
+                        const int s0 = (int)floor(tmin / 16.0) - 1;
+                        tab[tid] = make_float4(e.fcos, e.fsin, e.foffset, 0.f);
+                        s0s[tid] = s0;
+                    }
+                    rws[tid] = e.row;
+                } else {
+                    tab[tid] = (MODE == BP_LINEAR) ? make_float4(0.f, 0.f, 1.f, 0.f)
+                                                   : make_float4(0.f, 0.f, 0.f, 0.f);
+                    s0s[tid] = (MODE == BP_LINEAR) ? -(1 << 29) : 0;
+                    rws[tid] = -1;
+                }
+            }
+            __syncthreads();
+
+            // ---- stage detector segments into LDS
+            if (MODE == BP_LINEAR) {
+                float2* pairs = (float2*)data;
+                for (int idx = tid; idx < chunk * SEG; idx += C::T) {
+                    const int a = idx / SEG, j = idx - a * SEG;
+                    const int row = rws[a];
+                    const int k = s0s[a] + j;
+                    float av = 0.f, dv = 0.f;
+                    if (row >= 0 && k >= 0 && k < N) {
+                        const float p0 = load_row_val(S, row, k);
+                        if (k < N - 1) {
+                            const float p1 = load_row_val(S, row, k + 1);
+                            dv = p1 - p0;
+                            av = fmaf(-(float)j, dv, p0);
+                        } else {
+                            av = p0;
+                        }
+                    }
+                    pairs[idx] = make_float2(av, dv);
+                }
+            } else {
+                Acc* lut = (Acc*)data;
+                for (int idx = tid; idx < chunk * SEG; idx += C::T) {
+                    const int a = idx / SEG, j = idx - a * SEG;
+                    const int row = rws[a];
+                    const int k = s0s[a] + j;
+                    Acc* dst = lut + (size_t)a * EPA + 16 * j;
+                    if (row >= 0 && k >= 0 && k < N) {
+                        const float q0 = load_row_val(S, row, k);
+                        const float q1 = (k < N - 1) ? load_row_val(S, row, k + 1) : 0.f;
+                        float p0, p1p0;
+                        if (MODE == BP_LUT_I32) {
+                            if (S.lut_formula == 0) {       // general.cpp:2131-2133
+                                p0 = q0 * 100.f;
+                                p1p0 = (k == N - 1) ? 0.0f : (q1 - q0) / 16.f * 100.f;
+                            } else {                        // cudaReconst.cu:237-238
+                                p0 = q0 * 100.f;
+                                p1p0 = (k == N - 1) ? 0.0f : (q1 * 100.f - p0) / 16.f;
+                            }
+                        } else {
+                            p0 = q0;
+                            p1p0 = (k == N - 1) ? 0.0f : (q1 - p0) / 16.f;
+                        }
+#pragma unroll
+                        for (int s = 0; s < 16; s++) {
+                            const float v = p0 + p1p0 * (float)s;
+                            dst[s] = (MODE == BP_LUT_I32) ? (Acc)(int)v : (Acc)v;
+                        }
+                    } else {
+#pragma unroll
+                        for (int s = 0; s < 16; s++) dst[s] = (Acc)0;
+                    }
+                }
+            }
+            __syncthreads();
+
+            // ---- accumulate
+            if (MODE == BP_LINEAR) {
+                const float2* pairs = (const float2*)data;
+                for (int a = 0; a < chunk; a++) {
+                    const float4 tb = tab[a];
+                    const float2* seg = pairs + a * SEG;
+#pragma unroll
+                    for (int j = 0; j < PY; j++) {
+                        const float ro = fmaf(dyf[j], tb.y, tb.z);
+#pragma unroll
+                        for (int i = 0; i < PX; i++) {
+                            // u >= 1 by construction of s0, so truncation is the floor
+                            const float u = fmaf(dxf[i], tb.x, ro);
+                            const float2 ad = seg[(int)u];
+                            acc[j][i] += fmaf(u, ad.y, ad.x);
+                        }
+                    }
+                }
+            } else {
+                const Acc* lut = (const Acc*)data;
+                for (int a = 0; a < chunk; a++) {
+                    const float4 tb = tab[a];
+                    const int base = 16 * s0s[a];
+                    const Acc* la = lut + (size_t)a * EPA;
+#pragma unroll
+                    for (int j = 0; j < PY; j++) {
+                        const float fy = (float)(Y0 + (int)dyf[j]);
+                        const float rt = fy * tb.y + tb.z;              // iy*fsin + foffset
+#pragma unroll
+                        for (int i = 0; i < PX; i++) {
+                            const float fx = (float)(X0 + (int)dxf[i]);
+                            const float tt = fx * tb.x + rt;            // ix*fcos + (...)
+                            const int idx = (L.rule == 0) ? (int)floorf(tt) : (int)tt;
+                            const Acc v = la[idx - base];
+                            if (MODE == BP_LUT_I32)
+                                acc[j][i] = (Acc)((unsigned int)acc[j][i] + (unsigned int)v);
+                            else
+                                acc[j][i] += v;
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+        }
+
+        // ---- write the tile
+#pragma unroll
+        for (int j = 0; j < PY; j++) {
+            const int iy = Y0 + (int)dyf[j];
+            if (iy >= S.out_n) continue;
+#pragma unroll
+            for (int i = 0; i < PX; i++) {
+                const int ix = X0 + (int)dxf[i];
+                if (ix >= S.out_n) continue;
+                const long long o = (long long)iy * S.out_n + ix;
+                if (MODE == BP_LUT_I32) {
+                    unsigned int* out = (unsigned int*)S.out;
+                    const unsigned int v = (unsigned int)acc[j][i];
+                    out[o] = S.accumulate ? out[o] + v : v;
+                } else {
+                    float* out = (float*)S.out;
+                    const float v = (float)acc[j][i] * S.out_scale;
+                    out[o] = S.accumulate ? out[o] + v : v;
+                }
+            }
+        }
+        if (!L.counter) break;
+    }
+}
+
+// Tile configurations: id -> (PX, PY).  0: 64x64, 1: 64x128 (W x H), 2: 32x32, 3: 128x64.
+int bp_tile_side(int tile)
+{
+    switch (tile) {
+    case 1: return 64;
+    case 2: return 32;
+    case 3: return 128;
+    default: return 64;
+    }
+}
+
+static int tile_h(int tile)
+{
+    switch (tile) {
+    case 1: return 128;
+    case 2: return 32;
+    case 3: return 64;
+    default: return 64;
+    }
+}
+
+static int tile_seg(int tile)
+{
+    switch (tile) {
+    case 1: return TileCfg<4, 8>::SEG;
+    case 2: return TileCfg<2, 2>::SEG;
+    case 3: return TileCfg<8, 4>::SEG;
+    default: return TileCfg<4, 4>::SEG;
+    }
+}
+
+int bp_default_chunk(int mode, int tile)
+{
+    if (mode == BP_LINEAR) return (tile == 2) ? 64 : 32;
+    return (tile == 2) ? 16 : 8;
+}
+
+size_t bp_lds_bytes(int mode, int tile, int chunk)
+{
+    const int seg = tile_seg(tile);
+    const size_t head = (size_t)chunk * 16 + (size_t)((2 * chunk + 3) / 4) * 16;
+    const size_t body = (mode == BP_LINEAR) ? (size_t)chunk * seg * 8 : (size_t)chunk * seg * 16 * 4;
+    return head + body;
+}
+
+template <int MODE>
+static const void* kernel_ptr(int tile)
+{
+    switch (tile) {
+    case 1: return (const void*)&bp_tiled_kernel<MODE, 4, 8>;
+    case 2: return (const void*)&bp_tiled_kernel<MODE, 2, 2>;
+    case 3: return (const void*)&bp_tiled_kernel<MODE, 8, 4>;
+    default: return (const void*)&bp_tiled_kernel<MODE, 4, 4>;
+    }
+}
+
+static const void* kernel_for(int mode, int tile)
+{
+    if (mode == BP_LINEAR) return kernel_ptr<BP_LINEAR>(tile);
+    if (mode == BP_LUT_F32) return kernel_ptr<BP_LUT_F32>(tile);
+    return kernel_ptr<BP_LUT_I32>(tile);
+}
+
+// Kernels that use more than 64 KiB of LDS must say so once per process.
+static void allow_lds(const void* f, size_t lds)
+{
+    if (lds > 65536) (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+}
+
+int bp_blocks_per_cu(int mode, int tile, int chunk)
+{
+    const void* f = kernel_for(mode, tile);
+    const size_t lds = bp_lds_bytes(mode, tile, chunk);
+    allow_lds(f, lds);
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, 256, lds) != hipSuccess || nb < 1) nb = 1;
+    return nb;
+}
+
+template <int MODE>
+static void launch_mode(const BpLaunch& L, int grid, hipStream_t stream)
+{
+    const size_t lds = bp_lds_bytes(MODE, L.tile, L.chunk);
+    allow_lds(kernel_ptr<MODE>(L.tile), lds);
+    switch (L.tile) {
+    case 1: hipLaunchKernelGGL((bp_tiled_kernel<MODE, 4, 8>), dim3(grid), dim3(256), lds, stream, L); break;
+    case 2: hipLaunchKernelGGL((bp_tiled_kernel<MODE, 2, 2>), dim3(grid), dim3(256), lds, stream, L); break;
+    case 3: hipLaunchKernelGGL((bp_tiled_kernel<MODE, 8, 4>), dim3(grid), dim3(256), lds, stream, L); break;
+    default: hipLaunchKernelGGL((bp_tiled_kernel<MODE, 4, 4>), dim3(grid), dim3(256), lds, stream, L); break;
+    }
+}
+
+int bp_tile_h(int tile) { return tile_h(tile); }
+
+void launch_bp(const BpLaunch& L, int grid, hipStream_t stream)
+{
+    if (L.mode == BP_LINEAR) launch_mode<BP_LINEAR>(L, grid, stream);
+    else if (L.mode == BP_LUT_F32) launch_mode<BP_LUT_F32>(L, grid, stream);
+    else launch_mode<BP_LUT_I32>(L, grid, stream);
+}
+
+}  // namespace rv

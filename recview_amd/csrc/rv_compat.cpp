@@ -1,0 +1,455 @@
+// rv_compat.cpp -- the cudaReconst.h entry points (Part 1 of recview_hip.h).
+//
+// CudaReconstHost keeps the control flow of cudaReconstHost.cpp:629-813
+// (bDeconv / kept-sinogram re-projection with iCenterOffset, frame selection,
+// progress/abort callback, grow-only buffers, error codes) but replaces the
+// per-projection CPU filter + two tiny kernels per projection with:
+//   one H2D of all selected int16 rows -> one LDS FFT filter launch over all
+//   rows -> one tiled LUT16 back-projection launch over all projections ->
+//   one D2H of the int32 image.
+#include "recview_hip.h"
+#include "rv_common.h"
+
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+// cudaReconst.cu:19
+int blocksize = 256;
+
+// Imported from the application when it defines it (general.cpp:2352).
+extern "C" bool DBProjDlgCtrl(RECONST_INFO* ri, int iProgStep, int iSino, int* pCurrStep)
+    __attribute__((weak));
+
+namespace {
+
+rv_dlgctrl_fn g_dlgctrl = nullptr;
+int g_lut_formula = 1;   // cudaReconst.cu:237-239
+int g_index_rule = 1;    // trunc then range check, cudaReconst.cu:36-38 with tex1Dfetch
+
+bool call_dlgctrl(RECONST_INFO* ri, int iProgStep, int iSino, int* pCurrStep)
+{
+    if (g_dlgctrl) return g_dlgctrl(ri, iProgStep, iSino, pCurrStep);
+    if (DBProjDlgCtrl) return DBProjDlgCtrl(ri, iProgStep, iSino, pCurrStep);
+    return false;
+}
+
+// Grow-only device buffer held in a RECONST_INFO field pair (cudaReconstHost.cpp:44-81).
+template <typename T>
+bool grow(T*& ptr, unsigned int& maxb, size_t bytes)
+{
+    if (bytes > 0xffffffffull) return false;      // the reference sizes are unsigned int
+    if (ptr && bytes > maxb) {
+        if (hipFree(ptr) != hipSuccess) return false;
+        ptr = nullptr;
+        maxb = 0;
+    }
+    if (!ptr) {
+        if (hipMalloc((void**)&ptr, bytes ? bytes : 4) != hipSuccess) { ptr = nullptr; return false; }
+        maxb = (unsigned int)bytes;
+    }
+    return true;
+}
+
+struct Selected {
+    int i, slot, sidx;
+};
+
+// Frame selection of cudaReconstHost.cpp:712-730 with the progress / abort
+// callback of :715-717.  Returns the projections to use, in order.
+std::vector<Selected> select_frames(RECONST_INFO* ri, bool bReport)
+{
+    std::vector<Selected> sel;
+    const int iProgStep = ri->iLenSinogr / 20;                     // PROGRESS_BAR_UNIT
+    int iCurrStep = 0;
+    for (int i = ri->iStartSino; i < ri->iLenSinogr - 1; i += ri->iStepSino) {
+        const int isino = (i - ri->iStartSino) / ri->iStepSino;
+        if (bReport && (isino % 20 == 0)) {
+            if (call_dlgctrl(ri, iProgStep, i, &iCurrStep)) break;
+        }
+        if (!(ri->bInc[i] & CGAZODOC_BINC_SAMPLE)) continue;
+        const int sidx = i * ri->iMultiplex + ri->iOffset;
+        if (sidx >= ri->maxSinogrLen) break;
+        if (ri->dReconFlags & (RQFLAGS_USEONLYEVENFRAMES | RQFLAGS_USEONLYODDFRAMES)) {
+            if (i & 1) {
+                if (ri->dReconFlags & RQFLAGS_USEONLYEVENFRAMES) continue;
+            } else {
+                if (ri->dReconFlags & RQFLAGS_USEONLYODDFRAMES) continue;
+            }
+        }
+        if (ri->nSinogr) (*(ri->nSinogr))++;
+        sel.push_back({i, isino, sidx});
+    }
+    return sel;
+}
+
+// Shared body of CudaReconstHost (cudafft = false) and CudaReconstHostFFT (true).
+void reconst_body(RECONST_INFO* ri, int idev, bool bReport, bool bEnStream, bool cudafft)
+{
+    if (!rv::hip_ok(hipSetDevice(idev), "hipSetDevice")) {
+        rv::log_code(28800);
+        return;
+    }
+    const rv::Geom g = rv::geometry(ri->ixdim, ri->iInterpolation, ri->center, true);
+    const int W = g.W, N = g.N, n = g.n;
+    if (W < 4 || g.ndimp < 3 || n > 16384) {
+        rv::set_message("unsupported detector width / zoom (FFT length must be 8..16384)");
+        rv::log_code(28808);
+        ri->iStatus = RECONST_INFO_ERROR;
+        return;
+    }
+    const size_t mem_ifp = sizeof(int) * (size_t)N * N;
+    const int igpdimy = ((ri->iLenSinogr - 1) - ri->iStartSino + ri->iStepSino - 1) / ri->iStepSino;
+    const bool bDeconv = ((ri->dReconFlags & RQFLAGS_SINOGRAMKEPT) == 0) || (ri->d_ifp == nullptr) ||
+                         (ri->max_d_ifp < mem_ifp);
+    if (bDeconv) {
+        // CudaReconstMemAlloc (cudaReconstHost.cpp:19-141): image, rows, filter.
+        const size_t rows = (size_t)(igpdimy > 0 ? igpdimy : 1);
+        if (!grow(ri->d_ifp, ri->max_d_ifp, mem_ifp) ||
+            !grow(ri->d_strip, ri->max_d_strip, sizeof(short) * rows * W) ||
+            !grow(ri->d_px, ri->max_d_px, sizeof(float) * rows * N) ||
+            !grow(ri->d_filt, ri->max_d_filt, sizeof(float) * (size_t)n)) {
+            rv::set_message("Out of device memory");
+            ri->iStatus = RECONST_INFO_ERROR;
+            rv::log_code(28801);
+            return;
+        }
+        if (!rv::hip_ok(hipMemcpy(ri->d_filt, ri->fFilter, sizeof(float) * n, hipMemcpyHostToDevice),
+                        "filter upload")) {
+            ri->iStatus = RECONST_INFO_ERROR;
+            rv::log_code(28801);
+            return;
+        }
+        ri->iSinoCenter = g.ic;                                       // :677
+    }
+    // Streams (cudaReconstHost.cpp:685-702): failure falls back to the null stream.
+    hipStream_t st = nullptr;
+    if (bEnStream && !cudafft) {
+        hipError_t e1 = hipSuccess;
+        if (ri->stream1 == nullptr) {
+            hipStream_t s = nullptr;
+            e1 = hipStreamCreate(&s);
+            if (e1 == hipSuccess) ri->stream1 = (rv_stream_t)s;
+        }
+        if (e1 != hipSuccess) {
+            ri->stream1 = nullptr;
+            rv::log_code(28807);
+        } else {
+            st = (hipStream_t)ri->stream1;
+        }
+    }
+
+    std::vector<Selected> sel = select_frames(ri, bReport);
+    const int nsel = (int)sel.size();
+
+    if (bDeconv && nsel > 0) {
+        // Gather the selected int16 rows into pinned memory, one H2D, one filter launch.
+        short* h_rows = nullptr;
+        if (hipHostMalloc((void**)&h_rows, sizeof(short) * (size_t)nsel * W, 0) != hipSuccess) {
+            ri->iStatus = RECONST_INFO_ERROR;
+            rv::log_code(28803);
+            return;
+        }
+        std::vector<int> slots(nsel);
+        for (int q = 0; q < nsel; q++) {
+            std::memcpy(h_rows + (size_t)q * W, ri->iSinogr[sel[q].sidx], sizeof(short) * W);
+            slots[q] = sel[q].slot;
+        }
+        int* d_slots = nullptr;
+        bool ok = rv::hip_ok(hipMalloc(&d_slots, sizeof(int) * nsel), "slot list") &&
+                  rv::hip_ok(hipMemcpyAsync(ri->d_strip, h_rows, sizeof(short) * (size_t)nsel * W,
+                                            hipMemcpyHostToDevice, st), "row upload") &&
+                  rv::hip_ok(hipMemcpy(d_slots, slots.data(), sizeof(int) * nsel,
+                                       hipMemcpyHostToDevice), "slot upload");
+        const float* tw = ok ? rv::device_twiddles(idev, g.ndimp) : nullptr;
+        if (ok && tw) {
+            rv::FilterJob J{};
+            J.sino = ri->d_strip;
+            J.in_stride = W;
+            J.row_in = nullptr;
+            J.row_out = d_slots;
+            J.nrows = nsel;
+            J.in_kind = 0;
+            J.W = W; J.z = g.z; J.ic = g.ic; J.n = n; J.np = g.ndimp; J.N = N;
+            J.place = cudafft ? n / 2 : n / 2 - 1;
+            J.pad_mode = cudafft ? 1 : 0;
+            J.crop = cudafft ? n / 2 - g.h : g.ihoffset;
+            J.filt = ri->d_filt;
+            J.tw = tw;
+            J.out = ri->d_px;
+            J.out_stride = N;
+            J.store = 0;
+            rv::launch_filter(J, st);
+            ok = rv::hip_ok(hipGetLastError(), "filter launch") &&
+                 rv::hip_ok(hipStreamSynchronize(st), "filter");
+        } else {
+            ok = false;
+        }
+        (void)hipFree(d_slots);
+        (void)hipHostFree(h_rows);
+        if (!ok) {
+            rv::log_code(28804);
+            ri->iStatus = RECONST_INFO_ERROR;
+            return;
+        }
+    }
+
+    // Back-projection of every selected projection in one launch.
+    if (nsel == 0) {
+        if (!rv::hip_ok(hipMemsetAsync(ri->d_ifp, 0, mem_ifp, st), "image clear")) rv::log_code(28806);
+    } else {
+        const int icoff = g.ic - ri->iSinoCenter;                          // :775
+        std::vector<rv::AngleEntry> tab(nsel);
+        for (int q = 0; q < nsel; q++) {
+            rv::AngleEntry& e = tab[q];
+            std::memset(&e, 0, sizeof(e));
+            rv::lut_params(ri->fdeg[sel[q].i], ri->fTiltAngle, g, icoff, &e.fcos, &e.fsin, &e.foffset);
+            e.row = sel[q].slot;
+        }
+        const int tile = 0;
+        const int tw = rv::bp_tile_side(tile), th = rv::bp_tile_h(tile);
+        rv::BpSlice S{};
+        S.rows = ri->d_px;
+        S.scales = nullptr;
+        S.row_stride = N;
+        S.row_len = N;
+        S.nang = nsel;
+        S.out = ri->d_ifp;
+        S.out_n = N;
+        S.tiles_x = (N + tw - 1) / tw;
+        S.tiles_y = (N + th - 1) / th;
+        S.tile_base = 0;
+        S.accumulate = 0;
+        S.lut_formula = cudafft ? 1 : g_lut_formula;
+        S.store = 0;
+        S.out_scale = 1.f;
+        // table + slice descriptor in the d_p scratch field (unused by this path)
+        const size_t tab_bytes = sizeof(rv::AngleEntry) * (size_t)nsel;
+        const size_t need = tab_bytes + sizeof(rv::BpSlice) + 64;
+        if (!grow(ri->d_p, ri->max_d_p, need)) {
+            ri->iStatus = RECONST_INFO_ERROR;
+            rv::log_code(28801);
+            return;
+        }
+        unsigned char* base = (unsigned char*)ri->d_p;
+        rv::AngleEntry* d_tab = (rv::AngleEntry*)base;
+        rv::BpSlice* d_slice = (rv::BpSlice*)(base + ((tab_bytes + 63) / 64) * 64);
+        S.ang = d_tab;
+        // synchronous copies: the host table and descriptor live on this stack frame
+        bool ok = rv::hip_ok(hipMemcpy(d_tab, tab.data(), tab_bytes, hipMemcpyHostToDevice),
+                             "angle table") &&
+                  rv::hip_ok(hipMemcpy(d_slice, &S, sizeof(S), hipMemcpyHostToDevice), "slice");
+        if (ok) {
+            rv::BpLaunch L{};
+            L.slices = d_slice;
+            L.nslices = 1;
+            L.total_items = S.tiles_x * S.tiles_y;
+            L.item_order = nullptr;
+            L.counter = nullptr;
+            L.mode = rv::BP_LUT_I32;
+            L.rule = cudafft ? 1 : g_index_rule;
+            L.tile = tile;
+            L.chunk = rv::bp_default_chunk(rv::BP_LUT_I32, tile);
+            rv::launch_bp(L, L.total_items, st);
+            ok = rv::hip_ok(hipGetLastError(), "back-projection launch");
+        }
+        if (!ok) {
+            rv::log_code(28804);
+            ri->iStatus = RECONST_INFO_ERROR;
+            return;
+        }
+    }
+    // D2H of the image (cudaReconstHost.cpp:810-811); the copy waits for the stream.
+    if (!rv::hip_ok(hipStreamSynchronize(st), "reconstruction") ||
+        !rv::hip_ok(hipMemcpy(ri->iReconst, ri->d_ifp, mem_ifp, hipMemcpyDeviceToHost), "image download"))
+        rv::log_code(28805);
+}
+
+}  // namespace
+
+extern "C" {
+
+void RecviewSetDlgCtrl(rv_dlgctrl_fn fn) { g_dlgctrl = fn; }
+
+void RecviewSetCompatOptions(int lut_formula, int index_rule)
+{
+    g_lut_formula = lut_formula ? 1 : 0;
+    g_index_rule = index_rule ? 1 : 0;
+}
+
+void CudaReconstHost(RECONST_INFO* ri, int idev, bool bReport, bool bEnStream)
+{
+    reconst_body(ri, idev, bReport, bEnStream, false);
+}
+
+void CudaReconstHostFFT(RECONST_INFO* ri, int idev, bool bReport)
+{
+    reconst_body(ri, idev, bReport, false, true);
+}
+
+void CudaReconstMemFree(RECONST_INFO* ri)
+{
+    (void)hipSetDevice(ri->iStartSino);                            // cudaReconstHost.cpp:143
+    if (ri->d_ifp && hipFree(ri->d_ifp) == hipSuccess) { ri->d_ifp = nullptr; ri->max_d_ifp = 0; }
+    if (ri->d_strip && hipFree(ri->d_strip) == hipSuccess) { ri->d_strip = nullptr; ri->max_d_strip = 0; }
+    if (ri->d_px && hipFree(ri->d_px) == hipSuccess) { ri->d_px = nullptr; ri->max_d_px = 0; }
+    if (ri->d_igp && hipFree(ri->d_igp) == hipSuccess) { ri->d_igp = nullptr; ri->max_d_igp = 0; }
+    if (ri->d_p && hipFree(ri->d_p) == hipSuccess) { ri->d_p = nullptr; ri->max_d_p = 0; }
+    if (ri->d_filt && hipFree(ri->d_filt) == hipSuccess) { ri->d_filt = nullptr; ri->max_d_filt = 0; }
+    ri->fftplan = nullptr;
+    ri->ifftdim = 0;
+}
+
+void CudaReconstResourceFree(RECONST_INFO* ri, bool bCudaDeviceReset)
+{
+    (void)hipSetDevice(ri->iStartSino);                            // cudaReconstHost.cpp:206-213
+    if (ri->stream1) { (void)hipStreamDestroy((hipStream_t)ri->stream1); ri->stream1 = nullptr; }
+    if (ri->stream2) { (void)hipStreamDestroy((hipStream_t)ri->stream2); ri->stream2 = nullptr; }
+    CudaReconstMemFree(ri);
+    if (bCudaDeviceReset) {
+        (void)hipDeviceSynchronize();
+        (void)hipDeviceReset();
+        rv::drop_device_caches(ri->iStartSino);
+    }
+}
+
+rv_error_t CUDA_MALLOC_HOST_INT(int** ptr, size_t size)
+{
+    return (rv_error_t)hipHostMalloc((void**)ptr, size, 0);
+}
+
+rv_error_t CUDA_FREE_HOST(void* ptr) { return (rv_error_t)hipHostFree(ptr); }
+
+void CudaSinogramHost(RECONST_INFO* ri, int, bool)
+{
+    rv::set_message("sinogram generation is not provided by this engine");
+    rv::log_code(28901);
+    if (ri) ri->iStatus = RECONST_INFO_ERROR;
+}
+
+void CudaDeconv(int ixdim, int iIntpDim, int ndim, float center, float* d_filt, short* d_strip,
+                int* d_igp, rv_float2* d_p, rv_fft_plan_t*)
+{
+    // cudaReconst.cu:158-192 for one strip: zero padding, float zoom, crop at
+    // n/2 - N/2, LUT with the p2igp formula.  d_p is scratch for the row.
+    int np = 0;
+    while ((1 << np) < ndim) np++;
+    if ((1 << np) != ndim || np < 3 || ndim > 16384) {
+        rv::log_code(28808);
+        return;
+    }
+    const int dev = rv::current_device();
+    const float* tw = rv::device_twiddles(dev, np);
+    if (!tw) { rv::log_code(28804); return; }
+    const int N = ixdim * iIntpDim;
+    rv::FilterJob J{};
+    J.sino = d_strip;
+    J.in_stride = ixdim;
+    J.nrows = 1;
+    J.in_kind = 0;
+    J.W = ixdim; J.z = iIntpDim; J.ic = (int)center; J.n = ndim; J.np = np; J.N = N;
+    J.place = ndim / 2;
+    J.pad_mode = 1;
+    J.crop = ndim / 2 - N / 2;
+    J.filt = d_filt;
+    J.tw = tw;
+    J.out = (float*)d_p;
+    J.out_stride = N;
+    rv::launch_filter(J, nullptr);
+    rv::launch_px2igp((const float*)d_p, d_igp, N, 1, 100.f, blocksize, nullptr);
+}
+
+void CudaBackProjStream(int ixdimp, float center, int iCenterOffset, int iIntpDim, double theta,
+                        int* d_ifp, int* d_igp, rv_stream_t stream)
+{
+    // cudaReconst.cu:199-222
+    const int ixdimh = ixdimp / 2;
+    const float fcos = (float)(std::cos(theta) * 16);
+    const float fsin = (float)(-std::sin(theta) * 16);
+    const float fcenter = (iIntpDim == 1) ? (ixdimh + center - (int)(center)) * 16
+                                          : (ixdimh + (center - (int)(center)) * iIntpDim) * 16;
+    const float foffset = fcenter - ixdimh * (fcos + fsin) + 16 * iCenterOffset;
+    rv::launch_bp_lut_global(d_ifp, d_igp, ixdimp, fcos, fsin, foffset, g_index_rule, blocksize,
+                             (hipStream_t)stream);
+}
+
+void CudaSinoPx2igpStream(int ixdimp, int* d_igp, float* d_px, rv_stream_t stream)
+{
+    rv::launch_px2igp(d_px, d_igp, ixdimp, 1, 100.f, blocksize, (hipStream_t)stream);
+}
+
+// ------------------------------------------------------------ device queries
+// cudaReconst.cu:449-538 with hipGetDeviceProperties.
+
+int GetCudaDeviceCount(int iMinComputeCapability)
+{
+    int count = 0;
+    const hipError_t e = hipGetDeviceCount(&count);
+    if (e == hipErrorNoDevice) return 0;
+    if (e == hipErrorInsufficientDriver) return CUDA_ERROR_INSUFFICIENT_DRIVER;
+    if (e != hipSuccess) return CUDA_ERROR_DEVICE_GETCOUNT;
+    for (int i = 0; i < count; i++) {
+        hipDeviceProp_t p;
+        if (hipGetDeviceProperties(&p, i) != hipSuccess) return i | CUDA_ERROR_DEVICE_GETPROPERTY;
+        if (p.major * 100 + p.minor * 10 < iMinComputeCapability)
+            return i | CUDA_ERROR_INSUFFICIENT_COMPUTE_CAPABILITY;
+    }
+    return count;
+}
+
+int GetCudaDeviceName(int iDevice, char* pcName, int iszcName)
+{
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, iDevice) != hipSuccess) return CUDA_ERROR_DEVICE_GETPROPERTY;
+    const int isz = (iszcName < 256) ? iszcName : 256;
+    for (int i = 0; i < isz; i++) {
+        pcName[i] = p.name[i];
+        if (p.name[i] == 0) break;
+    }
+    return 0;
+}
+
+int GetCudaDeviceComputingCapability(int iDevice, int* piMajor, int* piMinor)
+{
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, iDevice) != hipSuccess) return CUDA_ERROR_DEVICE_GETPROPERTY;
+    if (piMajor) *piMajor = p.major;
+    if (piMinor) *piMinor = p.minor;
+    return 0;
+}
+
+int GetCudaMaxThreadsPerBlock(int iDevice)
+{
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, iDevice) != hipSuccess) return CUDA_ERROR_DEVICE_GETPROPERTY;
+    return p.maxThreadsPerBlock;
+}
+
+int GetCudaWarpSize(int iDevice)
+{
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, iDevice) != hipSuccess) return CUDA_ERROR_DEVICE_GETPROPERTY;
+    return p.warpSize;
+}
+
+int GetCudaNumberOfCores(int iDevice, int* piCores, int* piProcessors)
+{
+    // CDNA compute unit: 4 SIMD x 16 lanes = 64 FP32 lanes per CU.
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, iDevice) != hipSuccess) return CUDA_ERROR_DEVICE_GETPROPERTY;
+    if (piCores) *piCores = 64;
+    if (piProcessors) *piProcessors = p.multiProcessorCount;
+    return 0;
+}
+
+int GetCudaClockRate(int iDevice, int* piClockRate, int* piMemRate)
+{
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, iDevice) != hipSuccess) return CUDA_ERROR_DEVICE_GETPROPERTY;
+    if (piClockRate) *piClockRate = p.clockRate;
+    if (piMemRate) *piMemRate = p.memoryClockRate;
+    return 0;
+}
+
+}  // extern "C"

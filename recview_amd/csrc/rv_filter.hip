@@ -1,0 +1,341 @@
+// rv_filter.hip -- sinogram ramp-filter stage as one LDS-resident FFT per row.
+//
+// Replaces the per-projection CPU filter of cudaReconstHost.cpp:731-756 (row
+// prep, CFft::FFT1Rev, multiply by fFilter, CFft::FFT1, crop) and the CUDAFFT
+// pipeline of cudaReconst.cu:123-192.  One workgroup owns one row: the padded
+// complex row (n <= 16384 points, 8n bytes) lives in LDS for the whole
+// transform, butterflies are register-staged so a single LDS buffer serves
+// every stage, and the filter multiply and the crop are fused into the last
+// stage of each transform.
+//
+// The butterfly network, twiddles and rounding order are those of CFft's
+// radix-2 Stockham passes (cfft.cpp:193-369): stage l reads x[b] and
+// x[b + n/2] for butterfly b = j*al + m and writes x[2*j*al + m] and
+// x[2*j*al + m + al].  Keeping the same network (this file is compiled with
+// -ffp-contract=off) makes the filtered rows bit-identical to the CPU
+// restatement, which is what lets the LUT16 integer path match the reference
+// bit for bit.
+#include "rv_internal.h"
+#include <hip/hip_fp16.h>
+#include <hip/hip_bf16.h>
+
+namespace rv {
+
+constexpr int kMaxBfly = 8;  // butterflies per thread and stage (n/2 <= 8 * blockDim)
+
+template <int IN_KIND>
+__device__ __forceinline__ float load_in(const void* base, long long off)
+{
+    if constexpr (IN_KIND == 0) return (float)((const int16_t*)base)[off];
+    else return ((const float*)base)[off];
+}
+
+// Value of padded position m (cudaReconstHost.cpp:734-751 for pad_mode 0,
+// cudaReconst.cu:124-134 after the memset of cu:174 for pad_mode 1).
+template <int IN_KIND>
+__device__ float prep_value(const FilterJob& J, const void* row, int m)
+{
+    const int z = J.z, W = J.W, ic = J.ic;
+    const int rel = m - J.place;
+    const int kk = (rel >= 0) ? rel / z : -((-rel + z - 1) / z);
+    const int j = rel - kk * z;
+    const int k = ic + kk;
+    if (J.pad_mode == 0) {
+        const int idx0 = (0 - ic) * z + J.place;
+        const int idx1 = (W - 1 - ic) * z + J.place + 1;
+        if (m < idx0) return load_in<IN_KIND>(row, 0);
+        if (m >= idx1) return load_in<IN_KIND>(row, W - 1);
+        if (j == 0) return (k >= 0 && k <= W - 1) ? load_in<IN_KIND>(row, k) : 0.f;
+        if (k >= 0 && k < W - 1 && m - j >= 0) {
+            if constexpr (IN_KIND == 0) {
+                const int s0 = ((const int16_t*)row)[k], s1 = ((const int16_t*)row)[k + 1];
+                return (float)(s0 * (z - j) / z + s1 * j / z);
+            } else {
+                const float s0 = ((const float*)row)[k], s1 = ((const float*)row)[k + 1];
+                return (s0 * (float)(z - j) + s1 * (float)j) / (float)z;
+            }
+        }
+        return 0.f;
+    } else {
+        if (j == 0) return (k >= 0 && k <= W - 1) ? load_in<IN_KIND>(row, k) : 0.f;
+        if (k >= 0 && k < W - 1) {
+            if constexpr (IN_KIND == 0) {
+                const int s0 = ((const int16_t*)row)[k], s1 = ((const int16_t*)row)[k + 1];
+                return (float)(s0 * (z - j) + s1 * j) / z;
+            } else {
+                const float s0 = ((const float*)row)[k], s1 = ((const float*)row)[k + 1];
+                return (s0 * (float)(z - j) + s1 * (float)j) / (float)z;
+            }
+        }
+        return 0.f;
+    }
+}
+
+__device__ __forceinline__ float2 cmul_tw(float2 b, float2 w)
+{
+    // scrr = br*wr - bi*wi ; scri = br*wi + bi*wr  (cfft.cpp:239-240)
+    return make_float2(b.x * w.x - b.y * w.y, b.x * w.y + b.y * w.x);
+}
+
+// Middle Stockham stage l (1 <= l <= np-2) in place, register staged.
+__device__ __forceinline__ void fft_mid_stage(float2* buf, const float2* tw, int half, int l, int T,
+                                              int tid)
+{
+    const int al = 1 << l, bt = half >> l;
+    float2 A[kMaxBfly], B[kMaxBfly];
+#pragma unroll
+    for (int i = 0; i < kMaxBfly; i++) {
+        const int b = tid + i * T;
+        if (b < half) { A[i] = buf[b]; B[i] = buf[b + half]; }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kMaxBfly; i++) {
+        const int b = tid + i * T;
+        if (b < half) {
+            const int j = b >> l, m = b & (al - 1);
+            const float2 s = cmul_tw(B[i], tw[m * bt]);
+            const int o = (j << (l + 1)) + m;
+            buf[o] = make_float2(A[i].x + s.x, A[i].y + s.y);
+            buf[o + al] = make_float2(A[i].x - s.x, A[i].y - s.y);
+        }
+    }
+    __syncthreads();
+}
+
+// First stage: x1[2j] = x[i] + x[i+n/2], x1[2j+1] = x[i] - x[i+n/2], with
+// i = n/2-1-j for FFTrev (cfft.cpp:199-211) and i = j for FFT (:284-294).
+template <bool REV>
+__device__ __forceinline__ void fft_first_stage(float2* buf, int half, int T, int tid)
+{
+    float2 A[kMaxBfly], B[kMaxBfly];
+#pragma unroll
+    for (int i = 0; i < kMaxBfly; i++) {
+        const int j = tid + i * T;
+        if (j < half) {
+            const int src = REV ? (half - 1 - j) : j;
+            A[i] = buf[src];
+            B[i] = buf[src + half];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kMaxBfly; i++) {
+        const int j = tid + i * T;
+        if (j < half) {
+            buf[2 * j] = make_float2(A[i].x + B[i].x, A[i].y + B[i].y);
+            buf[2 * j + 1] = make_float2(A[i].x - B[i].x, A[i].y - B[i].y);
+        }
+    }
+    __syncthreads();
+}
+
+template <int IN_KIND>
+__global__ void __launch_bounds__(1024) filter_rows_kernel(FilterJob J)
+{
+    extern __shared__ float2 buf[];
+    const int r = blockIdx.x;
+    const int rin = J.row_in ? J.row_in[r] : r;
+    const int rout = J.row_out ? J.row_out[r] : r;
+    const int n = J.n, half = n >> 1, np = J.np, T = blockDim.x, tid = threadIdx.x;
+    const float2* tw = (const float2*)J.tw;
+    const void* row = (IN_KIND == 0)
+        ? (const void*)((const int16_t*)J.sino + (long long)rin * J.in_stride)
+        : (const void*)((const float*)J.sino + (long long)rin * J.in_stride);
+
+    // Row preparation straight into LDS.
+    for (int m = tid; m < n; m += T) buf[m] = make_float2(prep_value<IN_KIND>(J, row, m), 0.f);
+    __syncthreads();
+
+    // ---- FFT1Rev (cfft.cpp:431-434 -> FFTrev with nmin < 0)
+    fft_first_stage<true>(buf, half, T, tid);
+    for (int l = 1; l < np - 1; l++) fft_mid_stage(buf, tw, half, l, T, tid);
+    {
+        // last stage (cfft.cpp:249-268) fused with p[k] *= fFilter[k]
+        // (cudaReconstHost.cpp:753)
+        float2 A[kMaxBfly], B[kMaxBfly];
+#pragma unroll
+        for (int i = 0; i < kMaxBfly; i++) {
+            const int k = tid + i * T;
+            if (k < half) { A[i] = buf[k]; B[i] = buf[half + k]; }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < kMaxBfly; i++) {
+            const int k = tid + i * T;
+            if (k < half) {
+                const float2 s = (k == 0) ? B[i] : cmul_tw(B[i], tw[k]);
+                const int ip = (k == 0) ? 0 : (n - k);
+                const int im = half - k;
+                const float hp = J.filt[ip], hm = J.filt[im];
+                const float pr = A[i].x + s.x, pi = A[i].y + s.y;
+                const float mr = A[i].x - s.x, mi = A[i].y - s.y;
+                buf[ip] = make_float2(pr * hp, pi * hp);
+                buf[im] = make_float2(mr * hm, mi * hm);
+            }
+        }
+        __syncthreads();
+    }
+
+    // ---- FFT1 (cfft.cpp:412-429 -> FFT with nmin < 0, then reverse and 1/n)
+    fft_first_stage<false>(buf, half, T, tid);
+    for (int l = 1; l < np - 1; l++) fft_mid_stage(buf, tw, half, l, T, tid);
+    {
+        float2 A[kMaxBfly], B[kMaxBfly];
+#pragma unroll
+        for (int i = 0; i < kMaxBfly; i++) {
+            const int k = tid + i * T;
+            if (k < half) { A[i] = buf[k]; B[i] = buf[half + k]; }
+        }
+        __syncthreads();
+        const float a = 1.f / (float)n;
+        float* outf = (float*)buf;                 // reuse LDS as the cropped real row
+        const bool lp = (J.store != 0);
+#pragma unroll
+        for (int i = 0; i < kMaxBfly; i++) {
+            const int k = tid + i * T;
+            if (k < half) {
+                const float2 s = cmul_tw(B[i], tw[k]);
+                // f[k] = x1[k] - scr, f[k + n/2] = x1[k] + scr (cfft.cpp:364-365);
+                // FFT1 then stores f[n-1-q] * (1/n) at q (cfft.cpp:421-425) and
+                // the caller keeps p[c + crop].re (cudaReconstHost.cpp:756).
+                const float v0 = (A[i].x - s.x) * a;   // f[k]
+                const float v1 = (A[i].x + s.x) * a;   // f[k + half]
+                const int c0 = n - 1 - k - J.crop;
+                const int c1 = n - 1 - (k + half) - J.crop;
+                if (!lp) {
+                    float* o = J.out + (long long)rout * J.out_stride;
+                    if (c0 >= 0 && c0 < J.N) o[c0] = v0;
+                    if (c1 >= 0 && c1 < J.N) o[c1] = v1;
+                } else {
+                    if (c0 >= 0 && c0 < J.N) outf[c0] = v0;
+                    if (c1 >= 0 && c1 < J.N) outf[c1] = v1;
+                }
+            }
+        }
+        if (lp) {
+            // Per-row power-of-two scale so that |v| / scale <= 2^14, then 16-bit store.
+            __shared__ float red[32];
+            __syncthreads();
+            float mx = 0.f;
+            for (int c = tid; c < J.N; c += T) mx = fmaxf(mx, fabsf(outf[c]));
+            for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
+            if ((tid & 63) == 0) red[tid >> 6] = mx;
+            __syncthreads();
+            if (tid == 0) {
+                float m2 = 0.f;
+                for (int w = 0; w < (T + 63) / 64; w++) m2 = fmaxf(m2, red[w]);
+                int e = 0;
+                if (m2 > 0.f) { frexpf(m2 / 16384.f, &e); }
+                red[0] = ldexpf(1.f, e);
+            }
+            __syncthreads();
+            const float sc = red[0];
+            const float inv = 1.f / sc;            // power of two: exact
+            if (tid == 0) J.out_scale[rout] = sc;
+            if (J.store == 1) {
+                __half* o = (__half*)J.out_lp + (long long)rout * J.out_stride;
+                for (int c = tid; c < J.N; c += T) o[c] = __float2half_rn(outf[c] * inv);
+            } else {
+                __hip_bfloat16* o = (__hip_bfloat16*)J.out_lp + (long long)rout * J.out_stride;
+                for (int c = tid; c < J.N; c += T) o[c] = __float2bfloat16(outf[c] * inv);
+            }
+        }
+    }
+}
+
+void launch_filter(const FilterJob& job, hipStream_t stream)
+{
+    const int half = job.n / 2;
+    int T = half / kMaxBfly;
+    if (T < 64) T = 64;
+    if (T > 1024) T = 1024;
+    const size_t lds = sizeof(float2) * (size_t)job.n;
+    // transforms longer than 8192 points need more than the default 64 KiB of LDS
+    if (lds > 65536) {
+        (void)hipFuncSetAttribute((const void*)&filter_rows_kernel<0>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        (void)hipFuncSetAttribute((const void*)&filter_rows_kernel<1>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    }
+    if (job.nrows <= 0) return;
+    if (job.in_kind == 0)
+        hipLaunchKernelGGL(filter_rows_kernel<0>, dim3(job.nrows), dim3(T), lds, stream, job);
+    else
+        hipLaunchKernelGGL(filter_rows_kernel<1>, dim3(job.nrows), dim3(T), lds, stream, job);
+}
+
+// ------------------------------------------------ compatibility kernels
+
+// cudaReconst.cu:230-240 (formula 1) / general.cpp:2131-2136 (formula 0);
+// `scale` is BACKPROJ_SCALE (100) for the reference LUT.
+__global__ void px2igp_kernel(const float* px, int* igp, int N, int formula, float scale)
+{
+    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ix >= N) return;
+    const int iy = blockIdx.y;
+    float p0, p1p0;
+    if (formula == 0) {
+        p0 = px[ix] * scale;
+        p1p0 = (ix == N - 1) ? 0.0f : (px[ix + 1] - px[ix]) / 16.f * scale;
+    } else {
+        p0 = px[ix] * scale;
+        p1p0 = (ix == N - 1) ? 0.0f : (px[ix + 1] * scale - p0) / 16.f;
+    }
+    igp[ix * 16 + iy] = (int)(p0 + p1p0 * (float)iy);
+}
+
+void launch_px2igp(const float* d_px, int* d_igp, int N, int formula, float scale, int block,
+                   hipStream_t stream)
+{
+    if (block <= 0 || block > 1024) block = 256;
+    hipLaunchKernelGGL(px2igp_kernel, dim3((N + block - 1) / block, 16), dim3(block), 0, stream,
+                       d_px, d_igp, N, formula, scale);
+}
+
+// One projection's LUT added into the int32 image (cudaReconst.cu:31-39).
+// Every pixel belongs to one thread and launches on a stream are ordered, so
+// the reference's atomicAdd is a plain add here.  t uses the reference CPU
+// operation order ix*fcos + (iy*fsin + foffset) without contraction.
+__global__ void bp_lut_global_kernel(int* ifp, const int* igp, int N, float fcos, float fsin,
+                                     float foffset, int rule)
+{
+    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ix >= N) return;
+    const int iy = blockIdx.y;
+    const float fyoff = (float)iy * fsin + foffset;
+    const float t = (float)ix * fcos + fyoff;
+    const int ng = N * 16;
+    int idx;
+    if (rule == 0) {
+        if (!(t >= 0.f && t < (float)ng)) return;
+        idx = (int)t;
+    } else {
+        if (!(t > -2147483648.f && t < 2147483648.f)) return;
+        idx = (int)t;
+        if (idx < 0 || idx >= ng) return;
+    }
+    unsigned int* p = (unsigned int*)ifp + (long long)iy * N + ix;
+    *p = *p + (unsigned int)igp[idx];
+}
+
+void launch_bp_lut_global(int* d_ifp, const int* d_igp, int N, float fcos, float fsin,
+                          float foffset, int rule, int block, hipStream_t stream)
+{
+    if (block <= 0 || block > 1024) block = 256;
+    hipLaunchKernelGGL(bp_lut_global_kernel, dim3((N + block - 1) / block, N), dim3(block), 0,
+                       stream, d_ifp, d_igp, N, fcos, fsin, foffset, rule);
+}
+
+__global__ void zero_u32_kernel(unsigned int* p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+void launch_zero_u32(unsigned int* p, int n, hipStream_t stream)
+{
+    hipLaunchKernelGGL(zero_u32_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, p, n);
+}
+
+}  // namespace rv

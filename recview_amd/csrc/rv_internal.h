@@ -1,0 +1,91 @@
+// rv_internal.h -- structures shared by the host orchestration (rv_host.cpp,
+// rv_plan.cpp) and the CDNA4 kernels (rv_filter.hip, rv_bp.hip).  Plain C++,
+// no HIP types in the layouts, so host and device agree on every field.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rv {
+
+// ------------------------------------------------------------- filter stage
+// One work item = one sinogram row.  Row prep follows cudaReconstHost.cpp:733-751
+// (pad_mode 0) or cudaReconst.cu:123-156 (pad_mode 1); the transform is the
+// CFft FFT1Rev / filter / FFT1 sequence (cfft.cpp:193-434), crop at `crop`.
+struct FilterJob {
+    const void* sino;      // input rows (int16 or float)
+    long long in_stride;   // elements between input rows
+    const int* row_in;     // [nrows] input row index per work row (nullptr: identity)
+    const int* row_out;    // [nrows] output row index per work row (nullptr: identity)
+    int nrows;
+    int in_kind;           // 0 int16, 1 float
+    int W, z, ic, n, np, N;
+    int place;             // padded index of sample ic: n/2-1 (host) or n/2 (CUDAFFT)
+    int pad_mode;          // 0 replicate + integer zoom division, 1 zero + float zoom
+    int crop;              // first padded index of the N cropped outputs
+    const float* filt;     // [n]
+    const float* tw;       // [n] interleaved (re, im) twiddles, n/2 entries
+    float* out;            // [rows][out_stride] FP32 rows (nullptr when store != 0)
+    long long out_stride;
+    int store;             // 0 f32, 1 f16, 2 bf16
+    void* out_lp;          // [rows][out_stride] 16-bit rows
+    float* out_scale;      // [rows] per-row scale (power of two)
+};
+
+void launch_filter(const FilterJob& job, hipStream_t stream);
+
+// LUT build (cudaReconst.cu:230-246) and single-projection LUT back-projection
+// (cudaReconst.cu:31-39, 195-228) for the kernel-wrapper entry points.
+void launch_px2igp(const float* d_px, int* d_igp, int N, int formula, float scale, int block,
+                   hipStream_t stream);
+void launch_bp_lut_global(int* d_ifp, const int* d_igp, int N, float fcos, float fsin,
+                          float foffset, int rule, int block, hipStream_t stream);
+
+// ------------------------------------------------------- back-projection
+enum BpMode { BP_LINEAR = 0, BP_LUT_F32 = 1, BP_LUT_I32 = 2 };
+
+// Per-angle table entry.  LINEAR: detector coordinate of pixel (ix, iy) of the
+// output grid is u = u0 + ix*dux + iy*duy (all double).  LUT16: the reference
+// float triple t = ix*fcos + (iy*fsin + foffset) in 1/16-sample units.
+struct AngleEntry {
+    double u0, dux, duy;   // LINEAR geometry (FP64, per angle)
+    float fcos, fsin, foffset;
+    int row;               // row index into the slice's filtered rows
+};
+
+struct BpSlice {
+    const void* rows;       // filtered rows (float / half / bf16)
+    const float* scales;    // per-row scale (16-bit stores), else nullptr
+    long long row_stride;   // elements between rows
+    int row_len;            // samples per row (N)
+    int nang;               // angles
+    const AngleEntry* ang;  // [nang]
+    void* out;              // image out_n * out_n (float or int32)
+    int out_n;              // image side
+    int tiles_x, tiles_y;
+    int tile_base;          // first global work item of this slice
+    int accumulate;         // 1: add into existing image
+    int lut_formula;        // LUT modes
+    int store;              // 0 f32, 1 f16, 2 bf16
+    float out_scale;        // multiplies the final image (float modes)
+};
+
+struct BpLaunch {
+    const BpSlice* slices;  // device array
+    int nslices;
+    int total_items;        // sum of tiles
+    const int* item_order;  // optional permutation of work items (cost-sorted)
+    unsigned int* counter;  // persistent work counter (nullptr: one item per block)
+    int mode;               // BpMode
+    int rule;               // LUT: 0 floor (float mask), 1 trunc
+    int chunk;              // angles per LDS chunk
+    int tile;               // tile config id
+};
+
+void launch_bp(const BpLaunch& L, int grid, hipStream_t stream);
+int bp_tile_side(int tile);                 // tile width in pixels for a config id
+int bp_tile_h(int tile);                    // tile height in pixels
+int bp_default_chunk(int mode, int tile);   // angles per chunk
+size_t bp_lds_bytes(int mode, int tile, int chunk);
+void launch_zero_u32(unsigned int* p, int n, hipStream_t stream);
+
+}  // namespace rv

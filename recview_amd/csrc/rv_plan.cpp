@@ -1,0 +1,385 @@
+// rv_plan.cpp -- native batched API (Part 2 of recview_hip.h).
+//
+// A plan fixes everything that does not depend on the sinogram values: the
+// geometry of every slice, its filter table, the FP64 angle table, the
+// filtered-sinogram workspace in HBM and the cost-sorted tile work list.
+// Execution is then: one filter launch per slice (LDS FFT, all rows), one
+// back-projection launch for the whole batch (tiles of every slice, in one
+// grid or through a persistent work counter), optionally captured once into a
+// hipGraph and replayed.  The reference has no counterpart: it reconstructs
+// one slice at a time (gazoDoc.cpp:1965-1991).
+#include "recview_hip.h"
+#include "rv_common.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+namespace {
+
+struct SliceState {
+    RvSliceSpec spec;
+    std::vector<float> fdeg;
+    rv::Geom g;
+    int out_n = 0;
+    int nang = 0;
+    size_t rows_off = 0;    // element offset of row 0 in the workspace
+    size_t ang_off = 0;     // first entry in the angle table
+    size_t filt_off = 0;    // first entry in the filter tables
+    int tiles_x = 0, tiles_y = 0, tile_base = 0;
+};
+
+}  // namespace
+
+struct rv_plan_opaque {
+    int dev = 0;
+    RvOptions opt{};
+    int mode = rv::BP_LINEAR;
+    int store = 0;
+    int tile = 0, chunk = 32;
+    std::vector<SliceState> sl;
+    void* d_rows = nullptr;
+    float* d_scales = nullptr;
+    float* d_filts = nullptr;
+    rv::AngleEntry* d_ang = nullptr;
+    rv::BpSlice* d_slices = nullptr;
+    int* d_order = nullptr;
+    unsigned int* d_counter = nullptr;
+    std::vector<rv::BpSlice> h_slices;
+    std::vector<void*> cur_out;
+    int total_items = 0;
+    int grid = 0;
+    hipStream_t cap_stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    std::vector<const void*> g_sino;
+    std::vector<void*> g_out;
+};
+
+namespace {
+
+void plan_free(rv_plan_t p)
+{
+    if (!p) return;
+    (void)hipSetDevice(p->dev);
+    if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);
+    (void)hipFree(p->d_rows);
+    (void)hipFree(p->d_scales);
+    (void)hipFree(p->d_filts);
+    (void)hipFree(p->d_ang);
+    (void)hipFree(p->d_slices);
+    (void)hipFree(p->d_order);
+    (void)hipFree(p->d_counter);
+    delete p;
+}
+
+int fail(rv_plan_t p, const std::string& m)
+{
+    rv::set_message(m);
+    plan_free(p);
+    return -1;
+}
+
+bool upload_slices(rv_plan_t p, void* const* d_out)
+{
+    bool changed = false;
+    for (size_t s = 0; s < p->sl.size(); s++) {
+        if (p->cur_out[s] != d_out[s]) {
+            p->cur_out[s] = d_out[s];
+            p->h_slices[s].out = d_out[s];
+            changed = true;
+        }
+    }
+    if (!changed) return true;
+    return rv::hip_ok(hipMemcpy(p->d_slices, p->h_slices.data(), sizeof(rv::BpSlice) * p->h_slices.size(),
+                                hipMemcpyHostToDevice), "slice table upload");
+}
+
+int launch_filters(rv_plan_t p, const void* const* d_sino, hipStream_t st)
+{
+    const size_t esz = (p->store == 0) ? 4 : 2;
+    for (size_t s = 0; s < p->sl.size(); s++) {
+        const SliceState& S = p->sl[s];
+        const rv::Geom& g = S.g;
+        const float* tw = rv::device_twiddles(p->dev, g.ndimp);
+        if (!tw) { rv::set_message("twiddle table allocation failed"); return -1; }
+        rv::FilterJob J{};
+        J.sino = d_sino[s];
+        J.in_stride = g.W;
+        J.nrows = S.nang;
+        J.in_kind = (S.spec.sino_dtype == RV_SINO_I16) ? 0 : 1;
+        J.W = g.W; J.z = g.z; J.ic = g.ic; J.n = g.n; J.np = g.ndimp; J.N = g.N;
+        const bool zero = (S.spec.pad_mode == RV_PAD_ZERO);
+        J.place = zero ? g.n / 2 : g.n / 2 - 1;
+        J.pad_mode = zero ? 1 : 0;
+        J.crop = zero ? g.n / 2 - g.h : g.ihoffset;
+        J.filt = p->d_filts + S.filt_off;
+        J.tw = tw;
+        J.out_stride = g.N;
+        J.store = p->store;
+        if (p->store == 0) {
+            J.out = (float*)p->d_rows + S.rows_off;
+        } else {
+            J.out_lp = (unsigned char*)p->d_rows + S.rows_off * esz;
+            J.out_scale = p->d_scales + S.ang_off;
+        }
+        rv::launch_filter(J, st);
+    }
+    return rv::hip_ok(hipGetLastError(), "filter launch") ? 0 : -1;
+}
+
+int launch_backprojection(rv_plan_t p, hipStream_t st)
+{
+    rv::BpLaunch L{};
+    L.slices = p->d_slices;
+    L.nslices = (int)p->sl.size();
+    L.total_items = p->total_items;
+    L.item_order = p->d_order;
+    L.counter = p->opt.persistent ? p->d_counter : nullptr;
+    L.mode = p->mode;
+    L.rule = p->opt.index_rule ? 1 : 0;
+    L.chunk = p->chunk;
+    L.tile = p->tile;
+    if (L.counter) rv::launch_zero_u32(p->d_counter, 1, st);
+    rv::launch_bp(L, p->grid, st);
+    return rv::hip_ok(hipGetLastError(), "back-projection launch") ? 0 : -1;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslices, const RvOptions* opt)
+{
+    *out = nullptr;
+    if (nslices <= 0 || !specs || !opt) { rv::set_message("no slices"); return -1; }
+    if (!rv::hip_ok(hipSetDevice(device), "hipSetDevice")) return -1;
+    rv_plan_t p = new rv_plan_opaque();
+    p->dev = device;
+    p->opt = *opt;
+    p->mode = opt->interp;
+    if (p->mode < 0 || p->mode > 2) return fail(p, "unknown interpolation mode");
+    p->store = (p->mode == rv::BP_LINEAR) ? opt->store : 0;
+    if (p->store < 0 || p->store > 2) return fail(p, "unknown storage precision");
+    p->tile = (opt->tile >= 0 && opt->tile <= 3) ? opt->tile : 0;
+    p->chunk = opt->angles_per_chunk > 0 ? opt->angles_per_chunk : rv::bp_default_chunk(p->mode, p->tile);
+    if (p->chunk > 256) p->chunk = 256;
+    while (p->chunk > 1 && rv::bp_lds_bytes(p->mode, p->tile, p->chunk) > 160 * 1024) p->chunk /= 2;
+
+    const int tw = rv::bp_tile_side(p->tile), th = rv::bp_tile_h(p->tile);
+    size_t rows_elems = 0, nang_total = 0, filt_total = 0;
+    p->sl.resize(nslices);
+    for (int s = 0; s < nslices; s++) {
+        SliceState& S = p->sl[s];
+        S.spec = specs[s];
+        if (!specs[s].fdeg || specs[s].P <= 0) return fail(p, "slice without angles");
+        S.fdeg.assign(specs[s].fdeg, specs[s].fdeg + specs[s].P);
+        S.spec.fdeg = nullptr;
+        S.g = rv::geometry(S.spec.W, S.spec.iInterpolation, S.spec.center, true);
+        if (S.spec.W < 4 || S.g.ndimp < 3 || S.g.n > 16384)
+            return fail(p, "detector width / zoom outside the supported FFT range (n <= 16384)");
+        if (S.spec.out_n > 0) {
+            if (p->mode != rv::BP_LINEAR) return fail(p, "custom output grids need the LINEAR mode");
+            if (S.g.z != 1) return fail(p, "custom output grids need iInterpolation = 2");
+            if (!(S.spec.pix_zoom >= 1.0)) return fail(p, "pix_zoom must be >= 1");
+            S.out_n = S.spec.out_n;
+        } else {
+            S.out_n = S.g.N;
+        }
+        S.nang = S.spec.P;
+        S.rows_off = rows_elems;
+        S.ang_off = nang_total;
+        S.filt_off = filt_total;
+        rows_elems += (size_t)S.nang * S.g.N;
+        nang_total += (size_t)S.nang;
+        filt_total += (size_t)S.g.n;
+        S.tiles_x = (S.out_n + tw - 1) / tw;
+        S.tiles_y = (S.out_n + th - 1) / th;
+    }
+
+    // filter tables and angle tables
+    std::vector<float> hf(filt_total);
+    std::vector<rv::AngleEntry> ha(nang_total);
+    for (auto& S : p->sl) {
+        rv::set_filter(S.g.n, S.spec.filter, S.spec.cutoff, S.spec.order, hf.data() + S.filt_off);
+        const double fc = (double)S.g.fcenter16 / 16.0;
+        const double h = S.g.h;
+        for (int a = 0; a < S.nang; a++) {
+            rv::AngleEntry& e = ha[S.ang_off + a];
+            std::memset(&e, 0, sizeof(e));
+            const double th = rv::theta_rad(S.fdeg[a], S.spec.tilt);
+            const double c = std::cos(th), sn = -std::sin(th);
+            if (S.spec.out_n > 0) {
+                const double zf = S.spec.pix_zoom;
+                e.u0 = (S.spec.x0 - h) * c + (S.spec.y0 - h) * sn + fc;
+                e.dux = c / zf;
+                e.duy = sn / zf;
+            } else {
+                e.u0 = -h * c - h * sn + fc;
+                e.dux = c;
+                e.duy = sn;
+            }
+            rv::lut_params(S.fdeg[a], S.spec.tilt, S.g, 0, &e.fcos, &e.fsin, &e.foffset);
+            e.row = a;
+        }
+    }
+
+    // work items, cost-sorted (longest projection lists first)
+    std::vector<int> slice_order(nslices);
+    std::iota(slice_order.begin(), slice_order.end(), 0);
+    std::stable_sort(slice_order.begin(), slice_order.end(),
+                     [&](int a, int b) { return p->sl[a].nang > p->sl[b].nang; });
+    int base = 0;
+    for (int s = 0; s < nslices; s++) {
+        p->sl[s].tile_base = base;
+        base += p->sl[s].tiles_x * p->sl[s].tiles_y;
+    }
+    p->total_items = base;
+    std::vector<int> order;
+    order.reserve(base);
+    for (int s : slice_order) {
+        const SliceState& S = p->sl[s];
+        for (int t = 0; t < S.tiles_x * S.tiles_y; t++) order.push_back(S.tile_base + t);
+    }
+
+    const size_t esz = (p->store == 0) ? 4 : 2;
+    if (!rv::hip_ok(hipMalloc(&p->d_rows, esz * rows_elems), "filtered-sinogram workspace") ||
+        !rv::hip_ok(hipMalloc(&p->d_scales, sizeof(float) * nang_total), "row scales") ||
+        !rv::hip_ok(hipMalloc(&p->d_filts, sizeof(float) * filt_total), "filters") ||
+        !rv::hip_ok(hipMalloc(&p->d_ang, sizeof(rv::AngleEntry) * nang_total), "angle table") ||
+        !rv::hip_ok(hipMalloc(&p->d_slices, sizeof(rv::BpSlice) * nslices), "slice table") ||
+        !rv::hip_ok(hipMalloc(&p->d_order, sizeof(int) * order.size()), "work list") ||
+        !rv::hip_ok(hipMalloc(&p->d_counter, 64), "work counter"))
+        return fail(p, std::string("plan allocation failed: ") + RecviewLastMessage());
+
+    p->h_slices.resize(nslices);
+    p->cur_out.assign(nslices, nullptr);
+    for (int s = 0; s < nslices; s++) {
+        const SliceState& S = p->sl[s];
+        rv::BpSlice& B = p->h_slices[s];
+        std::memset(&B, 0, sizeof(B));
+        B.rows = (const unsigned char*)p->d_rows + S.rows_off * esz;
+        B.scales = (p->store == 0) ? nullptr : p->d_scales + S.ang_off;
+        B.row_stride = S.g.N;
+        B.row_len = S.g.N;
+        B.nang = S.nang;
+        B.ang = p->d_ang + S.ang_off;
+        B.out = nullptr;
+        B.out_n = S.out_n;
+        B.tiles_x = S.tiles_x;
+        B.tiles_y = S.tiles_y;
+        B.tile_base = S.tile_base;
+        B.accumulate = 0;
+        B.lut_formula = opt->lut_formula ? 1 : 0;
+        B.store = p->store;
+        B.out_scale = 1.f;
+    }
+    if (!rv::hip_ok(hipMemcpy(p->d_filts, hf.data(), sizeof(float) * filt_total, hipMemcpyHostToDevice), "filters") ||
+        !rv::hip_ok(hipMemcpy(p->d_ang, ha.data(), sizeof(rv::AngleEntry) * nang_total, hipMemcpyHostToDevice), "angles") ||
+        !rv::hip_ok(hipMemcpy(p->d_order, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice), "order") ||
+        !rv::hip_ok(hipMemcpy(p->d_slices, p->h_slices.data(), sizeof(rv::BpSlice) * nslices, hipMemcpyHostToDevice), "slices"))
+        return fail(p, std::string("plan upload failed: ") + RecviewLastMessage());
+
+    if (opt->persistent) {
+        const int per_cu = rv::bp_blocks_per_cu(p->mode, p->tile, p->chunk);
+        const int g = per_cu * rv::device_cu_count(device);
+        p->grid = std::max(1, std::min(g, p->total_items));
+    } else {
+        p->grid = p->total_items;
+    }
+    *out = p;
+    return 0;
+}
+
+int rvPlanFilter(rv_plan_t p, const void* const* d_sino, rv_stream_t stream)
+{
+    if (!p) return -1;
+    if (!rv::hip_ok(hipSetDevice(p->dev), "hipSetDevice")) return -1;
+    return launch_filters(p, d_sino, (hipStream_t)stream);
+}
+
+int rvPlanBackproject(rv_plan_t p, void* const* d_out, rv_stream_t stream)
+{
+    if (!p) return -1;
+    if (!rv::hip_ok(hipSetDevice(p->dev), "hipSetDevice")) return -1;
+    if (!upload_slices(p, d_out)) return -1;
+    return launch_backprojection(p, (hipStream_t)stream);
+}
+
+int rvPlanExecute(rv_plan_t p, const void* const* d_sino, void* const* d_out, rv_stream_t stream)
+{
+    if (!p) return -1;
+    if (!rv::hip_ok(hipSetDevice(p->dev), "hipSetDevice")) return -1;
+    if (!upload_slices(p, d_out)) return -1;
+    if (launch_filters(p, d_sino, (hipStream_t)stream)) return -1;
+    return launch_backprojection(p, (hipStream_t)stream);
+}
+
+int rvPlanExecuteGraph(rv_plan_t p, const void* const* d_sino, void* const* d_out, rv_stream_t stream)
+{
+    if (!p) return -1;
+    if (!rv::hip_ok(hipSetDevice(p->dev), "hipSetDevice")) return -1;
+    const size_t ns = p->sl.size();
+    bool same = (p->gexec != nullptr) && p->g_sino.size() == ns;
+    for (size_t s = 0; same && s < ns; s++) same = (p->g_sino[s] == d_sino[s]) && (p->g_out[s] == d_out[s]);
+    if (!same) {
+        if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }
+        if (p->graph) { (void)hipGraphDestroy(p->graph); p->graph = nullptr; }
+        if (!p->cap_stream && !rv::hip_ok(hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking), "capture stream"))
+            return -1;
+        if (!upload_slices(p, d_out)) return -1;
+        // The graph holds kernel nodes only (no memset/memcpy nodes).
+        if (!rv::hip_ok(hipStreamBeginCapture(p->cap_stream, hipStreamCaptureModeThreadLocal), "begin capture"))
+            return -1;
+        int rc = launch_filters(p, d_sino, p->cap_stream);
+        if (!rc) rc = launch_backprojection(p, p->cap_stream);
+        hipGraph_t g = nullptr;
+        const bool ended = rv::hip_ok(hipStreamEndCapture(p->cap_stream, &g), "end capture");
+        if (rc || !ended) { if (g) (void)hipGraphDestroy(g); return -1; }
+        p->graph = g;
+        if (!rv::hip_ok(hipGraphInstantiate(&p->gexec, g, nullptr, nullptr, 0), "graph instantiate")) return -1;
+        p->g_sino.assign(d_sino, d_sino + ns);
+        p->g_out.assign(d_out, d_out + ns);
+    }
+    return rv::hip_ok(hipGraphLaunch(p->gexec, (hipStream_t)stream), "graph launch") ? 0 : -1;
+}
+
+int rvPlanFilteredRows(rv_plan_t p, int s, void** rows, float** scales, int* row_len)
+{
+    if (!p || s < 0 || s >= (int)p->sl.size()) return -1;
+    if (rows) *rows = (void*)p->h_slices[s].rows;
+    if (scales) *scales = (float*)p->h_slices[s].scales;
+    if (row_len) *row_len = p->sl[s].g.N;
+    return 0;
+}
+
+int rvPlanSetFilteredRows(rv_plan_t p, int s, const float* d_rows, rv_stream_t stream)
+{
+    if (!p || s < 0 || s >= (int)p->sl.size() || p->store != 0) return -1;
+    if (!rv::hip_ok(hipSetDevice(p->dev), "hipSetDevice")) return -1;
+    const SliceState& S = p->sl[s];
+    return rv::hip_ok(hipMemcpyAsync((float*)p->d_rows + S.rows_off, d_rows,
+                                     sizeof(float) * (size_t)S.nang * S.g.N, hipMemcpyDeviceToDevice,
+                                     (hipStream_t)stream), "rows copy") ? 0 : -1;
+}
+
+int rvPlanInfo(rv_plan_t p, int s, int* out_n, int* n_fft, int* tiles, int* tile_w, int* tile_h)
+{
+    if (!p || s < 0 || s >= (int)p->sl.size()) return -1;
+    const SliceState& S = p->sl[s];
+    if (out_n) *out_n = S.out_n;
+    if (n_fft) *n_fft = S.g.n;
+    if (tiles) *tiles = S.tiles_x * S.tiles_y;
+    if (tile_w) *tile_w = rv::bp_tile_side(p->tile);
+    if (tile_h) *tile_h = rv::bp_tile_h(p->tile);
+    return 0;
+}
+
+void rvPlanDestroy(rv_plan_t p) { plan_free(p); }
+
+}  // extern "C"

@@ -1,0 +1,169 @@
+"""Native batched FBP (Part 2 of the C ABI): plans over many slices on one GPU.
+
+torch supplies device memory and streams only; every operation on the data
+runs in librecview_hip.so's kernels.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+
+INTERP = {"linear": 0, "lut16_f32": 1, "lut16_i32": 2}
+STORE = {"f32": 0, "f16": 1, "bf16": 2}
+SINO = {"i16": 0, "f32": 1}
+PAD = {"replicate": 0, "zero": 1}
+FILTERS = {"hann": 0, "hamming": 1, "ramp": 2, "parzen": 3, "butterworth": 4, "shepp-logan": 5}
+
+
+def _stream_handle(stream):
+    if stream is None:
+        import torch
+        stream = torch.cuda.current_stream()
+    return C.c_void_p(stream.cuda_stream)
+
+
+def _ptrs(tensors):
+    arr = (C.c_void_p * len(tensors))()
+    for i, t in enumerate(tensors):
+        if not t.is_cuda or not t.is_contiguous():
+            raise ValueError("tensors must be contiguous device tensors")
+        arr[i] = t.data_ptr()
+    return arr
+
+
+class SliceSpec:
+    """One slice: W detector samples x P projections (angles fdeg, degrees)."""
+
+    def __init__(self, W, fdeg, center, iInterpolation=2, tilt=0.0, filter="hann", cutoff=0.5,
+                 order=6.0, sino_dtype="f32", pad="replicate", out_n=0, pix_zoom=1.0, x0=0.0,
+                 y0=0.0):
+        self.W = int(W)
+        self.fdeg = np.ascontiguousarray(fdeg, np.float32)
+        self.P = len(self.fdeg)
+        self.center = float(center)
+        self.iInterpolation = int(iInterpolation)
+        self.tilt = float(tilt)
+        self.filter = FILTERS[filter] if isinstance(filter, str) else int(filter)
+        self.cutoff = float(cutoff)
+        self.order = float(order)
+        self.sino_dtype = SINO[sino_dtype]
+        self.pad = PAD[pad]
+        self.out_n = int(out_n)
+        self.pix_zoom = float(pix_zoom)
+        self.x0 = float(x0)
+        self.y0 = float(y0)
+
+    def to_c(self):
+        s = _lib.RvSliceSpec()
+        s.W, s.P, s.iInterpolation, s.center, s.tilt = self.W, self.P, self.iInterpolation, self.center, self.tilt
+        s.fdeg = self.fdeg.ctypes.data_as(C.POINTER(C.c_float))
+        s.filter, s.cutoff, s.order = self.filter, self.cutoff, self.order
+        s.sino_dtype, s.pad_mode = self.sino_dtype, self.pad
+        s.out_n, s.pix_zoom, s.x0, s.y0 = self.out_n, self.pix_zoom, self.x0, self.y0
+        return s
+
+
+class FbpPlan:
+    """A reconstruction plan for a batch of slices (see rvPlanCreate)."""
+
+    def __init__(self, slices, interp="linear", store="f32", lut_formula=1, index_rule=1,
+                 persistent=False, tile=0, angles_per_chunk=0, device=0):
+        self.slices = list(slices)
+        L = _lib.lib()
+        specs = (_lib.RvSliceSpec * len(self.slices))(*[s.to_c() for s in self.slices])
+        opt = _lib.RvOptions(INTERP[interp], STORE[store], int(lut_formula), int(index_rule),
+                             int(persistent), int(tile), int(angles_per_chunk))
+        self.interp = interp
+        self.store = store
+        h = C.c_void_p()
+        _lib.check(L.rvPlanCreate(C.byref(h), int(device), specs, len(self.slices), C.byref(opt)),
+                   "rvPlanCreate")
+        self._h = h
+
+    def info(self, s):
+        v = [C.c_int() for _ in range(5)]
+        _lib.check(_lib.lib().rvPlanInfo(self._h, s, *[C.byref(x) for x in v]), "rvPlanInfo")
+        out_n, n_fft, tiles, tw, th = (x.value for x in v)
+        return {"out_n": out_n, "n_fft": n_fft, "tiles": tiles, "tile_w": tw, "tile_h": th}
+
+    def out_n(self, s):
+        return self.info(s)["out_n"]
+
+    def execute(self, sinos, outs, stream=None, graph=False):
+        fn = _lib.lib().rvPlanExecuteGraph if graph else _lib.lib().rvPlanExecute
+        _lib.check(fn(self._h, _ptrs(sinos), _ptrs(outs), _stream_handle(stream)), "rvPlanExecute")
+
+    def filter(self, sinos, stream=None):
+        _lib.check(_lib.lib().rvPlanFilter(self._h, _ptrs(sinos), _stream_handle(stream)), "rvPlanFilter")
+
+    def backproject(self, outs, stream=None):
+        _lib.check(_lib.lib().rvPlanBackproject(self._h, _ptrs(outs), _stream_handle(stream)),
+                   "rvPlanBackproject")
+
+    def filtered_rows(self, s):
+        rows, scales, n = C.c_void_p(), C.c_void_p(), C.c_int()
+        _lib.check(_lib.lib().rvPlanFilteredRows(self._h, s, C.byref(rows), C.byref(scales), C.byref(n)),
+                   "rvPlanFilteredRows")
+        return rows.value, scales.value, n.value
+
+    def filtered_rows_tensor(self, s):
+        """Copy of slice s's filtered rows as a float32 [P, N] device tensor."""
+        import torch
+        rows, scales, n = self.filtered_rows(s)
+        P = self.slices[s].P
+        if self.store == "f32":
+            src = _wrap_device(rows, (P, n), torch.float32)
+            return src.clone()
+        dt = torch.float16 if self.store == "f16" else torch.bfloat16
+        src = _wrap_device(rows, (P, n), dt)
+        sc = _wrap_device(scales, (P,), torch.float32)
+        return src.float() * sc[:, None]
+
+    def set_filtered_rows(self, s, rows, stream=None):
+        _lib.check(_lib.lib().rvPlanSetFilteredRows(self._h, s, C.c_void_p(rows.data_ptr()),
+                                                    _stream_handle(stream)), "rvPlanSetFilteredRows")
+
+    def close(self):
+        if self._h:
+            _lib.lib().rvPlanDestroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class _DeviceArray:
+    def __init__(self, ptr, n, typestr):
+        self.__cuda_array_interface__ = {"shape": (n,), "typestr": typestr, "data": (ptr, False),
+                                         "version": 2, "strides": None}
+
+
+def _wrap_device(ptr, shape, dtype):
+    """A torch view of library-owned device memory (valid while the plan lives)."""
+    import torch
+    n = int(np.prod(shape))
+    esz = torch.empty((), dtype=dtype).element_size()
+    t = torch.as_tensor(_DeviceArray(ptr, n, "<f4" if esz == 4 else "<i2"), device="cuda")
+    return t.view(dtype).view(shape)
+
+
+def reconstruct(sino, fdeg, center, out=None, stream=None, **kw):
+    """One-slice convenience wrapper: float32 or int16 device sinogram [P, W] -> image."""
+    import torch
+    dtype = "i16" if sino.dtype == torch.int16 else "f32"
+    plan_kw = {k: kw.pop(k) for k in list(kw) if k in ("interp", "store", "lut_formula", "index_rule",
+                                                       "persistent", "tile", "angles_per_chunk")}
+    spec = SliceSpec(sino.shape[1], fdeg, center, sino_dtype=dtype, **kw)
+    plan = FbpPlan([spec], **plan_kw)
+    n = plan.out_n(0)
+    if out is None:
+        odt = torch.int32 if plan.interp == "lut16_i32" else torch.float32
+        out = torch.empty(n, n, device=sino.device, dtype=odt)
+    plan.execute([sino.contiguous()], [out], stream=stream)
+    torch.cuda.synchronize()
+    plan.close()
+    return out

@@ -1,0 +1,198 @@
+"""GPU parity of the native plan API (rvPlan*) against the CPU oracle.
+
+Tolerances: integer modes are bit-exact; LUT16_F32 is within 1e-5 of the image
+dynamic range of R-f32 (reference FP32 path); the production LINEAR mode is
+compared with the FP64 oracle O-f64 (RMS and max stated relative to the
+dynamic range inside the reconstruction circle).
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import pyoracle as O
+from recview_amd import fbp, phantom
+
+pytestmark = pytest.mark.gpu
+
+
+def _sino(W, P, seed, c_off=0.37, count=8, photons=None):
+    c_det = W / 2 + c_off
+    fdeg = phantom.angles_deg(P)
+    ell = phantom.random_ellipses(seed, count=count)
+    s = phantom.sinogram(ell, W, fdeg, c_det, scale=W)
+    if photons:
+        g = torch.Generator().manual_seed(seed)
+        s = phantom.poisson(s, photons, generator=g).double()
+    return s.float().numpy(), fdeg, c_det
+
+
+def _circle(N, frac=0.45):
+    yy, xx = np.mgrid[0:N, 0:N]
+    return ((xx - N / 2) ** 2 + (yy - N / 2) ** 2) <= (frac * N) ** 2
+
+
+def _errors(got, ref, mask):
+    rng = ref[mask].max() - ref[mask].min()
+    d = (got.astype(np.float64) - ref)[mask]
+    return float(np.sqrt(np.mean(d * d)) / rng), float(np.abs(d).max() / rng)
+
+
+def _run(specs, sinos, **kw):
+    plan = fbp.FbpPlan(specs, **kw)
+    outs = []
+    for s in range(len(specs)):
+        n = plan.out_n(s)
+        odt = torch.int32 if kw.get("interp") == "lut16_i32" else torch.float32
+        outs.append(torch.empty(n, n, dtype=odt, device="cuda"))
+    plan.execute([x.cuda() for x in sinos], outs)
+    torch.cuda.synchronize()
+    return plan, [o.cpu().numpy() for o in outs]
+
+
+@pytest.mark.parametrize("W,P,interp,formula,rule", [
+    (64, 45, 2, 1, 1), (96, 60, 2, 0, 0), (64, 40, 3, 1, 0), (200, 90, 2, 0, 1)])
+def test_plan_lut16_i32_bit_exact(W, P, interp, formula, rule):
+    print(f"case W={W} P={P} interp={interp}", flush=True)
+    s, fdeg, c = _sino(W, P, 300 + W)
+    rows = phantom.to_int16(torch.from_numpy(s)).numpy()
+    spec = fbp.SliceSpec(W, fdeg, c, iInterpolation=interp, sino_dtype="i16")
+    _, (img,) = _run([spec], [torch.from_numpy(rows)], interp="lut16_i32", lut_formula=formula,
+                     index_rule=rule)
+    g = O.geometry(W, interp, c)
+    ref = O.recon_rint(rows, fdeg, g, O.set_filter(g.ndim, 0), formula=formula, rule=rule)
+    assert np.array_equal(img, ref)
+
+
+@pytest.mark.parametrize("W,P,rule", [(128, 90, 0), (256, 180, 1)])
+def test_plan_lut16_f32_vs_reference_fp32(W, P, rule):
+    """LUT16 in float against R-f32: max |diff| <= 1e-5 of the dynamic range."""
+    s, fdeg, c = _sino(W, P, 500 + W)
+    spec = fbp.SliceSpec(W, fdeg, c)
+    _, (img,) = _run([spec], [torch.from_numpy(s)], interp="lut16_f32", index_rule=rule)
+    g = O.geometry(W, 2, c)
+    ref = O.recon_rf32(s, fdeg, g, O.set_filter(g.ndim, 0), rule=rule)
+    rms, mx = _errors(img, ref, _circle(W))
+    print(f"LUT16_F32 vs R-f32 W={W} P={P}: rms {rms:.2e} max {mx:.2e} of range")
+    assert mx <= 1e-5
+
+
+def test_plan_filtered_rows_bit_exact():
+    """The LDS FFT filter reproduces CFft's rounding: float rows identical to the oracle."""
+    for W, P, interp in ((128, 20, 2), (300, 10, 2), (64, 8, 4)):
+        s, fdeg, c = _sino(W, P, 700 + W)
+        spec = fbp.SliceSpec(W, fdeg, c, iInterpolation=interp, filter="hamming")
+        plan = fbp.FbpPlan([spec])
+        plan.filter([torch.from_numpy(s).cuda()])
+        torch.cuda.synchronize()
+        got = plan.filtered_rows_tensor(0).cpu().numpy()
+        g = O.geometry(W, interp, c)
+        ref = O.filter_rows(s, g, O.set_filter(g.ndim, 1), kind=1)
+        plan.close()
+        assert np.array_equal(got, ref), f"W={W}: max {np.abs(got - ref).max()}"
+
+
+def test_plan_linear_bp_vs_fp64():
+    """LINEAR back-projection of identical filtered rows vs FP64 linear interpolation."""
+    W, P = 256, 200
+    s, fdeg, c = _sino(W, P, 901)
+    g = O.geometry(W, 2, c)
+    px = O.filter_rows(s, g, O.set_filter(g.ndim, 0), kind=1)
+    plan = fbp.FbpPlan([fbp.SliceSpec(W, fdeg, c)])
+    plan.set_filtered_rows(0, torch.from_numpy(px).cuda())
+    out = torch.empty(W, W, device="cuda")
+    plan.backproject([out])
+    torch.cuda.synchronize()
+    ref = O.bp_rows_linear_f64(px, fdeg, g)
+    rms, mx = _errors(out.cpu().numpy(), ref, _circle(W))
+    print(f"LINEAR BP vs FP64: rms {rms:.2e} max {mx:.2e}")
+    assert rms <= 1e-6 and mx <= 1e-5
+
+
+@pytest.mark.parametrize("store,rms_tol", [("f32", 2e-6), ("f16", 1e-4), ("bf16", 2e-3)])
+def test_plan_linear_full_vs_fp64(store, rms_tol):
+    """Whole pipeline (filter + LINEAR BP) vs O-f64 on a noisy phantom; FP16/BF16
+    filtered sinograms with per-row power-of-two scales."""
+    W, P = 256, 180
+    s, fdeg, c = _sino(W, P, 1234, photons=1e4)
+    _, (img,) = _run([fbp.SliceSpec(W, fdeg, c)], [torch.from_numpy(s)], store=store)
+    g = O.geometry(W, 2, c)
+    ref = O.recon_of64(s, fdeg, g, O.set_filter(g.ndim, 0))
+    rms, mx = _errors(img, ref, _circle(W))
+    print(f"LINEAR store={store} vs O-f64: rms {rms:.2e} max {mx:.2e} of range")
+    assert rms <= rms_tol
+
+
+def test_plan_batch_uneven_equals_single():
+    """Slices of different width / projection count in one plan give the same
+    images as one plan per slice (the work list only reorders tiles)."""
+    cases = [(64, 30), (96, 45), (128, 60), (80, 90)]
+    specs, sinos = [], []
+    for i, (W, P) in enumerate(cases):
+        s, fdeg, c = _sino(W, P, 40 + i)
+        specs.append(fbp.SliceSpec(W, fdeg, c))
+        sinos.append(torch.from_numpy(s))
+    for persistent in (False, True):
+        _, batch = _run(specs, sinos, persistent=persistent)
+        for i in range(len(cases)):
+            _, (one,) = _run([specs[i]], [sinos[i]])
+            assert np.array_equal(batch[i], one)
+
+
+def test_plan_graph_replay():
+    """hipGraph replays (kernel nodes only) equal the eager result every time,
+    with eager work between replays."""
+    specs, sinos = [], []
+    for i, (W, P) in enumerate([(64, 40), (128, 50)]):
+        s, fdeg, c = _sino(W, P, 60 + i)
+        specs.append(fbp.SliceSpec(W, fdeg, c))
+        sinos.append(torch.from_numpy(s).cuda())
+    plan = fbp.FbpPlan(specs, persistent=True)
+    outs = [torch.empty(plan.out_n(i), plan.out_n(i), device="cuda") for i in range(2)]
+    plan.execute(sinos, outs)
+    torch.cuda.synchronize()
+    eager = [o.clone() for o in outs]
+    for rep in range(3):
+        for o in outs:
+            o.fill_(float("nan"))
+        plan.execute(sinos, outs, graph=True)
+        torch.cuda.synchronize()
+        junk = torch.randn(1000, device="cuda").sum()       # eager work between replays
+        torch.cuda.synchronize()
+        for o, e in zip(outs, eager):
+            assert torch.equal(o, e), f"replay {rep}"
+        del junk
+
+
+def test_plan_tiles_agree():
+    """Every tile configuration gives the same LUT16 integer image (bit-exact)
+    and the same LINEAR image to FP32 rounding."""
+    W, P = 160, 64
+    s, fdeg, c = _sino(W, P, 77)
+    rows = phantom.to_int16(torch.from_numpy(s))
+    ref_i = ref_f = None
+    for tile in range(4):
+        _, (im,) = _run([fbp.SliceSpec(W, fdeg, c, sino_dtype="i16")], [rows], interp="lut16_i32",
+                        tile=tile)
+        _, (fl,) = _run([fbp.SliceSpec(W, fdeg, c)], [torch.from_numpy(s)], tile=tile)
+        if ref_i is None:
+            ref_i, ref_f = im, fl
+        assert np.array_equal(im, ref_i)
+        rms, mx = _errors(fl, ref_f.astype(np.float64), _circle(W))
+        assert mx <= 1e-5
+
+
+def test_plan_zoomed_roi():
+    """Off-centre zoomed sub-region: pitch 1 with an integer origin equals the crop
+    of the full grid; pitch 1/2 coincides with the full grid on even pixels."""
+    W, P = 128, 90
+    s, fdeg, c = _sino(W, P, 88)
+    _, (full,) = _run([fbp.SliceSpec(W, fdeg, c)], [torch.from_numpy(s)])
+    x0, y0 = 40, 24
+    _, (roi1,) = _run([fbp.SliceSpec(W, fdeg, c, out_n=64, pix_zoom=1.0, x0=x0, y0=y0)],
+                      [torch.from_numpy(s)])
+    crop = full[y0:y0 + 64, x0:x0 + 64]
+    rng = full.max() - full.min()
+    assert np.abs(roi1 - crop).max() <= 1e-5 * rng
+    _, (roi2,) = _run([fbp.SliceSpec(W, fdeg, c, out_n=64, pix_zoom=2.0, x0=x0, y0=y0)],
+                      [torch.from_numpy(s)])
+    assert np.abs(roi2[::2, ::2] - full[y0:y0 + 32, x0:x0 + 32]).max() <= 1e-5 * rng

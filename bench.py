@@ -1,0 +1,209 @@
+"""Benchmark: filtered back-projection throughput on one MI355X.
+
+Default workload (the configuration the headline metric is quoted on): a
+64-slice stack, each slice 1800 projections over 180 deg x 2048 detector
+samples (float32 line integrals of a seeded ellipsoid phantom with Poisson
+noise at 1e4 photons/ray), reconstructed to 2048 x 2048 float32 with the Hann
+filter and linear detector interpolation.  One step = filter + back-projection
+of the whole stack, sinograms already resident in HBM, images written to HBM.
+
+Prints one JSON line.  value = slices/s for the whole job; tau = time per
+pixel*projection (README.md:96 of the reference); roofline = FP32 VALU
+fraction of the back-projection kernel (6 algorithmic FLOP per pixel*projection
+update), timed with HIP events on the launch stream; cpu_baseline = the
+reference CPU path (projx64 AVX-512 loop restated in C, reference threading)
+timed on this host.
+"""
+import argparse
+import json
+import math
+import os
+import re
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "FBP tau (ns/pixel·projection) + slices/s at 2048²×1800; sparse+CNN PSNR"
+REF_TAU_NS = 0.029          # RTX A4000, Build-220422, README.md:53 (2048^2 x 1800)
+FP32_PEAK_TFLOPS = 157.3    # MI355X published FP32 vector peak
+FLOP_PER_UPDATE = 6         # t = fma, frac, lerp fma, accumulate
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_stack(nslices, P, W, seed, device, photons=1e4):
+    """Float32 sinograms [nslices][P][W] of a seeded 40-ellipsoid phantom."""
+    import torch
+    from recview_amd import phantom
+    # attenuation per sample scaled with the width so that rays are not photon starved
+    ells = phantom.random_ellipsoids(seed, count=40, mu_range=(0.0, 4.0 / W))
+    fdeg = phantom.angles_deg(P)
+    c_det = W / 2 + 0.37
+    gen = torch.Generator(device=device).manual_seed(seed)
+    out = torch.empty(nslices, P, W, dtype=torch.float32, device=device)
+    for s in range(nslices):
+        z = -0.3 + 0.6 * (s + 0.5) / nslices
+        ell = phantom.ellipsoid_slice(ells, z)
+        sino = phantom.sinogram(ell, W, fdeg, c_det, scale=W, device=device)
+        out[s] = phantom.poisson(sino, photons, generator=gen)
+    return out, fdeg, c_det
+
+
+def read_traffic(name):
+    """HBM bytes per launch of the back-projection kernel, from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated)."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(sino_f32, fdeg, c_det, threads):
+    """The reference CPU reconstruction of one slice (int16 input, LUT16, AVX-512)."""
+    import numpy as np
+    import torch
+    from oracle import pyoracle as O
+    from recview_amd import phantom
+    rows = phantom.to_int16(sino_f32.double().cpu()).numpy()
+    W = rows.shape[1]
+    g = O.geometry(W, 2, c_det, float_center=False)
+    filt = O.set_filter(g.ndim, 0, 0.5, 6.0)
+    # small warm-up to fault in code pages, then the timed slice
+    O.cpu_baseline(rows[:8], fdeg[:8], g, filt, threads)
+    secs, _ = O.cpu_baseline(rows, fdeg, g, filt, threads)
+    isa = {2: "avx512", 1: "avx2", 0: "scalar"}[O.cpu_isa()]
+    return secs, isa
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--slices", type=int, default=64)
+    ap.add_argument("--proj", type=int, default=1800)
+    ap.add_argument("--width", type=int, default=2048)
+    ap.add_argument("--interp", default="linear", choices=["linear", "lut16_f32", "lut16_i32"])
+    ap.add_argument("--store", default="f32", choices=["f32", "f16", "bf16"])
+    ap.add_argument("--tile", type=int, default=0)
+    ap.add_argument("--chunk", type=int, default=0)
+    ap.add_argument("--persistent", type=int, default=1)
+    ap.add_argument("--graph", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(slices shard across GPUs with no communication; see the design notes)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from recview_amd import fbp
+
+    dev = torch.device("cuda:0")
+    S, P, W = args.slices, args.proj, args.width
+    t0 = time.time()
+    sinos, fdeg, c_det = make_stack(S, P, W, 1002, dev)
+    if args.interp == "lut16_i32":
+        from recview_amd import phantom
+        sinos = phantom.to_int16(sinos)
+    torch.cuda.synchronize()
+    log(f"inputs: {S} x {P} x {W} generated in {time.time() - t0:.1f}s")
+
+    dtype = "i16" if args.interp == "lut16_i32" else "f32"
+    specs = [fbp.SliceSpec(W, fdeg, c_det, sino_dtype=dtype) for _ in range(S)]
+    plan = fbp.FbpPlan(specs, interp=args.interp, store=args.store, persistent=bool(args.persistent),
+                       tile=args.tile, angles_per_chunk=args.chunk)
+    N = plan.out_n(0)
+    odt = torch.int32 if args.interp == "lut16_i32" else torch.float32
+    images = torch.empty(S, N, N, dtype=odt, device=dev)
+    sino_list = [sinos[s] for s in range(S)]
+    out_list = [images[s] for s in range(S)]
+    stream = torch.cuda.current_stream()
+
+    def step():
+        plan.execute(sino_list, out_list, stream=stream, graph=bool(args.graph))
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+
+    # Stage timing with HIP events on the launch stream (eager, one pass).
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    bp_ms = []
+    for _ in range(max(1, min(3, args.steps))):
+        ev[0].record(stream)
+        plan.filter(sino_list, stream=stream)
+        ev[1].record(stream)
+        plan.backproject(out_list, stream=stream)
+        ev[2].record(stream)
+        torch.cuda.synchronize()
+        bp_ms.append(ev[1].elapsed_time(ev[2]))
+    filt_ms = ev[0].elapsed_time(ev[1])
+    bp_ms = sum(bp_ms) / len(bp_ms)
+
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    ms_per_step = 1e3 * elapsed / args.steps
+    slices_per_s = S * args.steps / elapsed
+    updates = float(N) * N * P * S
+    tau_ns = elapsed / args.steps / updates * 1e9
+    achieved = FLOP_PER_UPDATE * updates / (bp_ms * 1e-3) / 1e12
+    if not torch.isfinite(images.float()).all():
+        raise RuntimeError("non-finite image values")
+
+    cpu = None
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        secs, isa = cpu_baseline(sinos[0] if args.interp != "lut16_i32" else sinos[0].float() / 3000.0,
+                                 fdeg, c_det, threads)
+        cpu = {"value": round(1.0 / secs, 4), "unit": "slices/s", "cores": threads, "kind": "port",
+               "sample": f"1 slice {P}x{W} -> {N}^2, int16 input, reference LUT16 path "
+                         f"({isa} restatement of projx64, row bands + per-thread filtering), "
+                         f"tau {secs / (N * N * P) * 1e9:.4f} ns"}
+
+    traffic = read_traffic("bp_linear_c2_traffic.md") if (args.interp == "linear" and args.store == "f32"
+                                                         and S == 64 and P == 1800 and W == 2048) else None
+    result = {
+        "metric": METRIC,
+        "value": round(slices_per_s, 3),
+        "unit": "slices/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": round(REF_TAU_NS / tau_ns, 3),
+        "dtype": "f32" if args.interp != "lut16_i32" else "int32",
+        "data": "synthetic",
+        "tau_ns": round(tau_ns, 7),
+        "tau_bp_ns": round(bp_ms * 1e-3 / updates * 1e9, 7),
+        "filter_ms": round(filt_ms, 3),
+        "bp_ms": round(bp_ms, 3),
+        "config": {"workload": f"C2: {S}-slice stack, {P} proj x {W} det -> {N}^2, "
+                               f"{args.interp} interpolation, {args.store} filtered sinogram",
+                   "slices": S, "projections": P, "detector": W, "image": N,
+                   "interp": args.interp, "store": args.store, "tile": args.tile,
+                   "persistent": bool(args.persistent), "graph": bool(args.graph)},
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
+                     "traffic": traffic},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(result), flush=True)
+    plan.close()
+
+
+if __name__ == "__main__":
+    main()

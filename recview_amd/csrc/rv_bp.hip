@@ -337,10 +337,18 @@ static const void* kernel_for(int mode, int tile)
     return kernel_ptr<BP_LUT_I32>(tile);
 }
 
-// Kernels that use more than 64 KiB of LDS must say so once per process.
+// Kernels that use more than 64 KiB of LDS must say so (once per kernel and
+// size, so that a later launch inside a stream capture makes no such call).
 static void allow_lds(const void* f, size_t lds)
 {
-    if (lds > 65536) (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    static const void* seen_f[64];
+    static size_t seen_lds[64];
+    static int nseen = 0;
+    if (lds <= 65536) return;
+    for (int i = 0; i < nseen; i++)
+        if (seen_f[i] == f && seen_lds[i] >= lds) return;
+    (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (nseen < 64) { seen_f[nseen] = f; seen_lds[nseen] = lds; nseen++; }
 }
 
 int bp_blocks_per_cu(int mode, int tile, int chunk)

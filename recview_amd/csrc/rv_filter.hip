@@ -244,6 +244,20 @@ __global__ void __launch_bounds__(1024) filter_rows_kernel(FilterJob J)
     }
 }
 
+// Transforms longer than 8192 points need more than the default 64 KiB of LDS;
+// raised once, outside any stream capture.
+void prepare_filter(int n)
+{
+    static int done = 0;
+    const size_t lds = sizeof(float2) * (size_t)n;
+    if (lds <= 65536 || n <= done) return;
+    (void)hipFuncSetAttribute((const void*)&filter_rows_kernel<0>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    (void)hipFuncSetAttribute((const void*)&filter_rows_kernel<1>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    done = n;
+}
+
 void launch_filter(const FilterJob& job, hipStream_t stream)
 {
     const int half = job.n / 2;
@@ -251,13 +265,7 @@ void launch_filter(const FilterJob& job, hipStream_t stream)
     if (T < 64) T = 64;
     if (T > 1024) T = 1024;
     const size_t lds = sizeof(float2) * (size_t)job.n;
-    // transforms longer than 8192 points need more than the default 64 KiB of LDS
-    if (lds > 65536) {
-        (void)hipFuncSetAttribute((const void*)&filter_rows_kernel<0>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        (void)hipFuncSetAttribute((const void*)&filter_rows_kernel<1>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    }
+    prepare_filter(job.n);
     if (job.nrows <= 0) return;
     if (job.in_kind == 0)
         hipLaunchKernelGGL(filter_rows_kernel<0>, dim3(job.nrows), dim3(T), lds, stream, job);

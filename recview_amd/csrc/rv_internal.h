@@ -32,6 +32,7 @@ struct FilterJob {
 };
 
 void launch_filter(const FilterJob& job, hipStream_t stream);
+void prepare_filter(int n);   // kernel attributes for an n-point transform (not capturable)
 
 // LUT build (cudaReconst.cu:230-246) and single-projection LUT back-projection
 // (cudaReconst.cu:31-39, 195-228) for the kernel-wrapper entry points.

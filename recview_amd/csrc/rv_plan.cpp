@@ -285,6 +285,15 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
         !rv::hip_ok(hipMemcpy(p->d_slices, p->h_slices.data(), sizeof(rv::BpSlice) * nslices, hipMemcpyHostToDevice), "slices"))
         return fail(p, std::string("plan upload failed: ") + RecviewLastMessage());
 
+    // Everything that allocates or sets kernel attributes happens here, so that
+    // execution (and hipGraph capture of it) only launches kernels.
+    int max_n = 0;
+    for (const auto& S : p->sl) {
+        if (!rv::device_twiddles(device, S.g.ndimp)) return fail(p, "twiddle table allocation failed");
+        max_n = std::max(max_n, S.g.n);
+    }
+    rv::prepare_filter(max_n);
+    (void)rv::bp_blocks_per_cu(p->mode, p->tile, p->chunk);
     if (opt->persistent) {
         const int per_cu = rv::bp_blocks_per_cu(p->mode, p->tile, p->chunk);
         const int g = per_cu * rv::device_cu_count(device);

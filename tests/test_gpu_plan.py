@@ -161,6 +161,15 @@ def test_plan_graph_replay():
         for o, e in zip(outs, eager):
             assert torch.equal(o, e), f"replay {rep}"
         del junk
+    # a plan whose very first execution is a capture (nothing cached beforehand)
+    spec = fbp.SliceSpec(96, phantom.angles_deg(30), 48.3)
+    s, _, _ = _sino(96, 30, 61, c_off=0.3)
+    plan2 = fbp.FbpPlan([spec], persistent=True)
+    o2 = torch.empty(96, 96, device="cuda")
+    plan2.execute([torch.from_numpy(s).cuda()], [o2], graph=True)
+    torch.cuda.synchronize()
+    _, (ref,) = _run([spec], [torch.from_numpy(s)])
+    assert np.array_equal(o2.cpu().numpy(), ref)
 
 
 def test_plan_tiles_agree():

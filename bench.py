@@ -92,6 +92,7 @@ def main():
     ap.add_argument("--store", default="f32", choices=["f32", "f16", "bf16"])
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=0)
+    ap.add_argument("--lds-half", type=int, default=0, help="FP16 pairs in LDS for 16-bit stores")
     ap.add_argument("--persistent", type=int, default=1)
     ap.add_argument("--graph", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -119,7 +120,7 @@ def main():
     dtype = "i16" if args.interp == "lut16_i32" else "f32"
     specs = [fbp.SliceSpec(W, fdeg, c_det, sino_dtype=dtype) for _ in range(S)]
     plan = fbp.FbpPlan(specs, interp=args.interp, store=args.store, persistent=bool(args.persistent),
-                       tile=args.tile, angles_per_chunk=args.chunk)
+                       tile=args.tile, angles_per_chunk=args.chunk, lds_half=bool(args.lds_half))
     N = plan.out_n(0)
     odt = torch.int32 if args.interp == "lut16_i32" else torch.float32
     images = torch.empty(S, N, N, dtype=odt, device=dev)
@@ -195,7 +196,8 @@ def main():
                                f"{args.interp} interpolation, {args.store} filtered sinogram",
                    "slices": S, "projections": P, "detector": W, "image": N,
                    "interp": args.interp, "store": args.store, "tile": args.tile,
-                   "persistent": bool(args.persistent), "graph": bool(args.graph)},
+                   "persistent": bool(args.persistent), "graph": bool(args.graph),
+                   "lds_half": bool(args.lds_half)},
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
                      "traffic": traffic},

@@ -231,6 +231,7 @@ typedef struct {
     int persistent;      /* 1: persistent work-queue back-projection */
     int tile;            /* 0 = autotuned, else tile config id */
     int angles_per_chunk;/* 0 = default */
+    int lds_half;        /* FP16/BF16 stores: 0 = FP32 pairs in LDS (default), 1 = FP16 pairs */
 } RvOptions;
 
 typedef struct rv_plan_opaque* rv_plan_t;

@@ -53,7 +53,7 @@ class RvOptions(C.Structure):
     _fields_ = [
         ("interp", C.c_int), ("store", C.c_int), ("lut_formula", C.c_int),
         ("index_rule", C.c_int), ("persistent", C.c_int), ("tile", C.c_int),
-        ("angles_per_chunk", C.c_int),
+        ("angles_per_chunk", C.c_int), ("lds_half", C.c_int),
     ]
 
 

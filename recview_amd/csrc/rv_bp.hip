@@ -16,15 +16,18 @@
 //    atomic work counter over a cost-sorted list (uneven slices in one launch).
 //
 // Modes:
-//  BP_LINEAR  : production.  LDS holds per-sample pairs (a_j, d_j) with
-//               d_j = p[k+1]-p[k] and a_j = p[k] - j*d_j (j = sample index
-//               within the segment), so value(u) = a_j + u*d_j for the
-//               segment-relative coordinate u: one FMA, no fract.
-//  BP_LUT_F32 : the reference's 16-sub-sample LUT (cudaReconst.cu:230-240)
-//               in float, float sums; index t = ix*fcos + (iy*fsin + foffset)
-//               in the reference's FP32 operation order.
-//  BP_LUT_I32 : the reference LUT in int32 (x BACKPROJ_SCALE), int32 sums --
-//               bit-exact with the reference integer path.
+//  BP_LINEAR   : production FP32.  LDS holds per-sample pairs (a_j, d_j) with
+//                d_j = p[k+1]-p[k] and a_j = p[k] - j*d_j (j = sample index
+//                within the segment), so value(u) = a_j + u*d_j for the
+//                segment-relative coordinate u: one FMA, no fract.  8 B/update.
+//  BP_LINEAR_H : FP16/BF16 filtered sinogram.  LDS holds FP16 pairs (p_k, d_k)
+//                rescaled to one power-of-two scale per chunk of angles, value =
+//                p + fract(u)*d (mixed-precision FMA), FP32 sums.  4 B/update.
+//  BP_LUT_F32  : the reference's 16-sub-sample LUT (cudaReconst.cu:230-240)
+//                in float, float sums; index t = ix*fcos + (iy*fsin + foffset)
+//                in the reference's FP32 operation order.
+//  BP_LUT_I32  : the reference LUT in int32 (x BACKPROJ_SCALE), int32 sums --
+//                bit-exact with the reference integer path.
 // This file is compiled with -ffp-contract=off; FMAs are written explicitly.
 #include "rv_internal.h"
 #include <hip/hip_fp16.h>
@@ -46,6 +49,14 @@ struct AccT { using type = float; };
 template <>
 struct AccT<BP_LUT_I32> { using type = int; };
 
+constexpr bool is_linear(int mode) { return mode == BP_LINEAR || mode == BP_LINEAR_H; }
+
+// LDS bytes per staged angle
+constexpr int entry_bytes(int mode, int seg)
+{
+    return mode == BP_LINEAR ? seg * 8 : (mode == BP_LINEAR_H ? seg * 4 : seg * 16 * 4);
+}
+
 __device__ __forceinline__ float load_row_val(const BpSlice& S, int row, int k)
 {
     const long long off = (long long)row * S.row_stride + k;
@@ -61,7 +72,7 @@ __global__ void __launch_bounds__(256) bp_tiled_kernel(BpLaunch L)
     using C = TileCfg<PX, PY>;
     using Acc = typename AccT<MODE>::type;
     constexpr int SEG = C::SEG;
-    constexpr int EPA = (MODE == BP_LINEAR) ? SEG : 16 * SEG;   // LDS entries per angle
+    constexpr int EPA = (MODE == BP_LINEAR || MODE == BP_LINEAR_H) ? SEG : 16 * SEG;   // entries per angle
     const int chunk = L.chunk;
 
     // LDS layout (pointer arithmetic only, so every access stays a ds_* op):
@@ -72,11 +83,14 @@ __global__ void __launch_bounds__(256) bp_tiled_kernel(BpLaunch L)
     int* rws = s0s + chunk;
     void* data = (void*)(lds4 + chunk + (2 * chunk + 3) / 4);
     __shared__ int item_sh;
+    __shared__ float chunk_scale;
 
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
     const int wx0 = (wave & 1) * C::WX, wy0 = (wave >> 1) * C::WY;
     const int lx = lane & 7, ly = lane >> 3;
+    // per-lane offset of the first pixel of the lane inside the tile
+    const float dxl = (float)(wx0 + lx), dyl = (float)(wy0 + ly);
 
     int iter = 0;
     while (true) {
@@ -111,24 +125,18 @@ __global__ void __launch_bounds__(256) bp_tiled_kernel(BpLaunch L)
 #pragma unroll
             for (int i = 0; i < PX; i++) acc[j][i] = (Acc)0;
 
-        // per-lane pixel offsets inside the tile
-        float dxf[PX], dyf[PY];
-#pragma unroll
-        for (int i = 0; i < PX; i++) dxf[i] = (float)(wx0 + lx + 8 * i);
-#pragma unroll
-        for (int j = 0; j < PY; j++) dyf[j] = (float)(wy0 + ly + 8 * j);
-
         for (int a0 = 0; a0 < S.nang; a0 += chunk) {
             // ---- per-angle tile geometry
             if (tid < chunk) {
                 const int ga = a0 + tid;
                 if (ga < S.nang) {
                     const AngleEntry e = S.ang[ga];
-                    if (MODE == BP_LINEAR) {
+                    if (is_linear(MODE)) {
                         const double ut = e.u0 + (double)X0 * e.dux + (double)Y0 * e.duy;
                         const double umin = ut + fmin(0.0, (C::TW - 1) * e.dux) + fmin(0.0, (C::TH - 1) * e.duy);
                         const int s0 = (int)floor(umin) - 1;
-                        tab[tid] = make_float4((float)e.dux, (float)e.duy, (float)(ut - (double)s0), 0.f);
+                        const float sc = (MODE == BP_LINEAR_H) ? S.scales[e.row] : 0.f;
+                        tab[tid] = make_float4((float)e.dux, (float)e.duy, (float)(ut - (double)s0), sc);
                         s0s[tid] = s0;
                     } else {
                         const double fc = e.fcos, fs = e.fsin;
@@ -140,10 +148,18 @@ __global__ void __launch_bounds__(256) bp_tiled_kernel(BpLaunch L)
                     }
                     rws[tid] = e.row;
                 } else {
-                    tab[tid] = (MODE == BP_LINEAR) ? make_float4(0.f, 0.f, 1.f, 0.f)
-                                                   : make_float4(0.f, 0.f, 0.f, 0.f);
-                    s0s[tid] = (MODE == BP_LINEAR) ? -(1 << 29) : 0;
+                    tab[tid] = is_linear(MODE) ? make_float4(0.f, 0.f, 1.f, 0.f) : make_float4(0.f, 0.f, 0.f, 0.f);
+                    s0s[tid] = is_linear(MODE) ? -(1 << 29) : 0;
                     rws[tid] = -1;
+                }
+            }
+            if (MODE == BP_LINEAR_H) {
+                // one power-of-two scale for the chunk: the largest row scale (chunk <= 64)
+                __syncthreads();
+                if (tid < 64) {
+                    float m = (tid < chunk) ? tab[tid].w : 0.f;
+                    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+                    if (tid == 0) chunk_scale = (m > 0.f) ? m : 1.f;
                 }
             }
             __syncthreads();
@@ -167,6 +183,23 @@ __global__ void __launch_bounds__(256) bp_tiled_kernel(BpLaunch L)
                         }
                     }
                     pairs[idx] = make_float2(av, dv);
+                }
+            } else if (MODE == BP_LINEAR_H) {
+                __half2* pairs = (__half2*)data;
+                const float inv = 1.f / chunk_scale;              // power of two: exact
+                for (int idx = tid; idx < chunk * SEG; idx += C::T) {
+                    const int a = idx / SEG, j = idx - a * SEG;
+                    const int row = rws[a];
+                    const int k = s0s[a] + j;
+                    float pv = 0.f, dv = 0.f;
+                    if (row >= 0 && k >= 0 && k < N) {
+                        pv = __half2float(__float2half_rn(load_row_val(S, row, k) * inv));
+                        if (k < N - 1) {
+                            const float p1 = __half2float(__float2half_rn(load_row_val(S, row, k + 1) * inv));
+                            dv = p1 - pv;
+                        }
+                    }
+                    pairs[idx] = __halves2half2(__float2half_rn(pv), __float2half_rn(dv));
                 }
             } else {
                 Acc* lut = (Acc*)data;
@@ -210,18 +243,48 @@ __global__ void __launch_bounds__(256) bp_tiled_kernel(BpLaunch L)
                 for (int a = 0; a < chunk; a++) {
                     const float4 tb = tab[a];
                     const float2* seg = pairs + a * SEG;
+                    // u(i, j) = base + 8 i cos + 8 j sin, base at the lane's first pixel
+                    const float base = fmaf(dyl, tb.y, fmaf(dxl, tb.x, tb.z));
 #pragma unroll
                     for (int j = 0; j < PY; j++) {
-                        const float ro = fmaf(dyf[j], tb.y, tb.z);
+                        const float bj = (j == 0) ? base : fmaf((float)(8 * j), tb.y, base);
 #pragma unroll
                         for (int i = 0; i < PX; i++) {
                             // u >= 1 by construction of s0, so truncation is the floor
-                            const float u = fmaf(dxf[i], tb.x, ro);
+                            const float u = (i == 0) ? bj : fmaf((float)(8 * i), tb.x, bj);
                             const float2 ad = seg[(int)u];
                             acc[j][i] += fmaf(u, ad.y, ad.x);
                         }
                     }
                 }
+            } else if (MODE == BP_LINEAR_H) {
+                const __half2* pairs = (const __half2*)data;
+                float cacc[PY][PX];
+#pragma unroll
+                for (int j = 0; j < PY; j++)
+#pragma unroll
+                    for (int i = 0; i < PX; i++) cacc[j][i] = 0.f;
+                for (int a = 0; a < chunk; a++) {
+                    const float4 tb = tab[a];
+                    const __half2* seg = pairs + a * SEG;
+                    const float base = fmaf(dyl, tb.y, fmaf(dxl, tb.x, tb.z));
+#pragma unroll
+                    for (int j = 0; j < PY; j++) {
+                        const float bj = (j == 0) ? base : fmaf((float)(8 * j), tb.y, base);
+#pragma unroll
+                        for (int i = 0; i < PX; i++) {
+                            const float u = (i == 0) ? bj : fmaf((float)(8 * i), tb.x, bj);
+                            const __half2 pd = seg[(int)u];
+                            const float f = __builtin_amdgcn_fractf(u);
+                            cacc[j][i] += fmaf(f, __high2float(pd), __low2float(pd));
+                        }
+                    }
+                }
+                const float cs = chunk_scale;
+#pragma unroll
+                for (int j = 0; j < PY; j++)
+#pragma unroll
+                    for (int i = 0; i < PX; i++) acc[j][i] = fmaf(cacc[j][i], cs, acc[j][i]);
             } else {
                 const Acc* lut = (const Acc*)data;
                 for (int a = 0; a < chunk; a++) {
@@ -230,11 +293,11 @@ __global__ void __launch_bounds__(256) bp_tiled_kernel(BpLaunch L)
                     const Acc* la = lut + (size_t)a * EPA;
 #pragma unroll
                     for (int j = 0; j < PY; j++) {
-                        const float fy = (float)(Y0 + (int)dyf[j]);
+                        const float fy = (float)(Y0 + (int)dyl + 8 * j);
                         const float rt = fy * tb.y + tb.z;              // iy*fsin + foffset
 #pragma unroll
                         for (int i = 0; i < PX; i++) {
-                            const float fx = (float)(X0 + (int)dxf[i]);
+                            const float fx = (float)(X0 + (int)dxl + 8 * i);
                             const float tt = fx * tb.x + rt;            // ix*fcos + (...)
                             const int idx = (L.rule == 0) ? (int)floorf(tt) : (int)tt;
                             const Acc v = la[idx - base];
@@ -252,11 +315,11 @@ __global__ void __launch_bounds__(256) bp_tiled_kernel(BpLaunch L)
         // ---- write the tile
 #pragma unroll
         for (int j = 0; j < PY; j++) {
-            const int iy = Y0 + (int)dyf[j];
+            const int iy = Y0 + (int)dyl + 8 * j;
             if (iy >= S.out_n) continue;
 #pragma unroll
             for (int i = 0; i < PX; i++) {
-                const int ix = X0 + (int)dxf[i];
+                const int ix = X0 + (int)dxl + 8 * i;
                 if (ix >= S.out_n) continue;
                 const long long o = (long long)iy * S.out_n + ix;
                 if (MODE == BP_LUT_I32) {
@@ -285,7 +348,7 @@ int bp_tile_side(int tile)
     }
 }
 
-static int tile_h(int tile)
+int bp_tile_h(int tile)
 {
     switch (tile) {
     case 1: return 128;
@@ -308,15 +371,17 @@ static int tile_seg(int tile)
 int bp_default_chunk(int mode, int tile)
 {
     if (mode == BP_LINEAR) return (tile == 2) ? 64 : 32;
+    if (mode == BP_LINEAR_H) return (tile == 2) ? 64 : 32;
     return (tile == 2) ? 16 : 8;
 }
+
+int bp_max_chunk(int mode) { return mode == BP_LINEAR_H ? 64 : 256; }
 
 size_t bp_lds_bytes(int mode, int tile, int chunk)
 {
     const int seg = tile_seg(tile);
     const size_t head = (size_t)chunk * 16 + (size_t)((2 * chunk + 3) / 4) * 16;
-    const size_t body = (mode == BP_LINEAR) ? (size_t)chunk * seg * 8 : (size_t)chunk * seg * 16 * 4;
-    return head + body;
+    return head + (size_t)chunk * entry_bytes(mode, seg);
 }
 
 template <int MODE>
@@ -333,6 +398,7 @@ static const void* kernel_ptr(int tile)
 static const void* kernel_for(int mode, int tile)
 {
     if (mode == BP_LINEAR) return kernel_ptr<BP_LINEAR>(tile);
+    if (mode == BP_LINEAR_H) return kernel_ptr<BP_LINEAR_H>(tile);
     if (mode == BP_LUT_F32) return kernel_ptr<BP_LUT_F32>(tile);
     return kernel_ptr<BP_LUT_I32>(tile);
 }
@@ -374,11 +440,10 @@ static void launch_mode(const BpLaunch& L, int grid, hipStream_t stream)
     }
 }
 
-int bp_tile_h(int tile) { return tile_h(tile); }
-
 void launch_bp(const BpLaunch& L, int grid, hipStream_t stream)
 {
     if (L.mode == BP_LINEAR) launch_mode<BP_LINEAR>(L, grid, stream);
+    else if (L.mode == BP_LINEAR_H) launch_mode<BP_LINEAR_H>(L, grid, stream);
     else if (L.mode == BP_LUT_F32) launch_mode<BP_LUT_F32>(L, grid, stream);
     else launch_mode<BP_LUT_I32>(L, grid, stream);
 }

@@ -42,7 +42,7 @@ void launch_bp_lut_global(int* d_ifp, const int* d_igp, int N, float fcos, float
                           float foffset, int rule, int block, hipStream_t stream);
 
 // ------------------------------------------------------- back-projection
-enum BpMode { BP_LINEAR = 0, BP_LUT_F32 = 1, BP_LUT_I32 = 2 };
+enum BpMode { BP_LINEAR = 0, BP_LUT_F32 = 1, BP_LUT_I32 = 2, BP_LINEAR_H = 3 };
 
 // Per-angle table entry.  LINEAR: detector coordinate of pixel (ix, iy) of the
 // output grid is u = u0 + ix*dux + iy*duy (all double).  LUT16: the reference
@@ -86,6 +86,7 @@ void launch_bp(const BpLaunch& L, int grid, hipStream_t stream);
 int bp_tile_side(int tile);                 // tile width in pixels for a config id
 int bp_tile_h(int tile);                    // tile height in pixels
 int bp_default_chunk(int mode, int tile);   // angles per chunk
+int bp_max_chunk(int mode);                 // largest chunk a mode supports
 size_t bp_lds_bytes(int mode, int tile, int chunk);
 void launch_zero_u32(unsigned int* p, int n, hipStream_t stream);
 

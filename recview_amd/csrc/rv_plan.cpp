@@ -166,9 +166,13 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
     if (p->mode < 0 || p->mode > 2) return fail(p, "unknown interpolation mode");
     p->store = (p->mode == rv::BP_LINEAR) ? opt->store : 0;
     if (p->store < 0 || p->store > 2) return fail(p, "unknown storage precision");
+    // 16-bit filtered sinograms are expanded to FP32 pairs in LDS by default (the
+    // kernel is LDS-bound at 8 B/update and VALU-bound at 4 B/update with the
+    // extra fract); lds_half = 1 selects the FP16-pair variant.
+    if (p->mode == rv::BP_LINEAR && p->store != 0 && opt->lds_half) p->mode = rv::BP_LINEAR_H;
     p->tile = (opt->tile >= 0 && opt->tile <= 3) ? opt->tile : 0;
     p->chunk = opt->angles_per_chunk > 0 ? opt->angles_per_chunk : rv::bp_default_chunk(p->mode, p->tile);
-    if (p->chunk > 256) p->chunk = 256;
+    if (p->chunk > rv::bp_max_chunk(p->mode)) p->chunk = rv::bp_max_chunk(p->mode);
     while (p->chunk > 1 && rv::bp_lds_bytes(p->mode, p->tile, p->chunk) > 160 * 1024) p->chunk /= 2;
 
     const int tw = rv::bp_tile_side(p->tile), th = rv::bp_tile_h(p->tile);
@@ -184,7 +188,7 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
         if (S.spec.W < 4 || S.g.ndimp < 3 || S.g.n > 16384)
             return fail(p, "detector width / zoom outside the supported FFT range (n <= 16384)");
         if (S.spec.out_n > 0) {
-            if (p->mode != rv::BP_LINEAR) return fail(p, "custom output grids need the LINEAR mode");
+            if (p->mode != rv::BP_LINEAR && p->mode != rv::BP_LINEAR_H) return fail(p, "custom output grids need the LINEAR mode");
             if (S.g.z != 1) return fail(p, "custom output grids need iInterpolation = 2");
             if (!(S.spec.pix_zoom >= 1.0)) return fail(p, "pix_zoom must be >= 1");
             S.out_n = S.spec.out_n;

@@ -108,13 +108,16 @@ def test_plan_linear_bp_vs_fp64():
     assert rms <= 1e-6 and mx <= 1e-5
 
 
-@pytest.mark.parametrize("store,rms_tol", [("f32", 2e-6), ("f16", 1e-4), ("bf16", 2e-3)])
-def test_plan_linear_full_vs_fp64(store, rms_tol):
+@pytest.mark.parametrize("store,lds_half,rms_tol", [("f32", False, 2e-6), ("f16", False, 1e-4),
+                                                    ("bf16", False, 2e-3), ("f16", True, 1e-4),
+                                                    ("bf16", True, 2e-3)])
+def test_plan_linear_full_vs_fp64(store, lds_half, rms_tol):
     """Whole pipeline (filter + LINEAR BP) vs O-f64 on a noisy phantom; FP16/BF16
-    filtered sinograms with per-row power-of-two scales."""
+    filtered sinograms with per-row power-of-two scales, expanded to FP32 pairs in
+    LDS or kept as FP16 pairs (per-chunk scale)."""
     W, P = 256, 180
     s, fdeg, c = _sino(W, P, 1234, photons=1e4)
-    _, (img,) = _run([fbp.SliceSpec(W, fdeg, c)], [torch.from_numpy(s)], store=store)
+    _, (img,) = _run([fbp.SliceSpec(W, fdeg, c)], [torch.from_numpy(s)], store=store, lds_half=lds_half)
     g = O.geometry(W, 2, c)
     ref = O.recon_of64(s, fdeg, g, O.set_filter(g.ndim, 0))
     rms, mx = _errors(img, ref, _circle(W))

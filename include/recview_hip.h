@@ -260,6 +260,17 @@ int rvPlanSetFilteredRows(rv_plan_t plan, int s, const float* d_rows, rv_stream_
 int rvPlanInfo(rv_plan_t plan, int s, int* out_n, int* n_fft, int* tiles, int* tile_w, int* tile_h);
 void rvPlanDestroy(rv_plan_t plan);
 
+/* Offset CT (gazoDoc.cpp:3536-3665, 2967-2978).  RecviewOffsetCTPairs pairs
+ * every sampled frame below 180 deg with its partner >= 180 deg further (host);
+ * returns the number of stitched rows or -21051 / -21052 as the reference.
+ * RecviewOffsetCTStitch writes the double-width rows [nout][out_width] on the
+ * device (int16 rows with the reference's truncations, or float rows);
+ * center = user centre + 0.5, deltac = deltaCent * j * binning. */
+int RecviewOffsetCTPairs(const float* fdeg, const char* binc, int nframes, int* pair);
+int RecviewOffsetCTStitch(const void* d_sino, int is_float, int W, const int* d_pair, int nout,
+                          double center, double deltac, int bin, int trim, int out_width,
+                          void* d_out, rv_stream_t stream);
+
 /* The filter design of gazoDoc.cpp:3747-3812 (plus 5 = Shepp-Logan, new). */
 void RecviewSetFilter(int ndim, int ifilter, double cutoff, double order, float* out);
 /* Geometry of cudaReconstHost.cpp:635-658: returns n (ndim) and writes N, zoom. */

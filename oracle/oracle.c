@@ -593,3 +593,105 @@ void orc_output_stage(int32_t* img, int N, int z, double center, int bin, int* p
     }
     *pimin = imin; *pimax = imax;
 }
+
+/* ------------------------------------------------------ offset-CT stitch -- */
+
+int orc_offset_stitch(const int16_t* sino, int nframes, int W, const float* fdeg,
+                      const char* binc, double center, double deltac, int bin, int trim,
+                      int out_width, int16_t* ofs)
+{
+    /* gazoDoc.cpp:2967-2978: first frame beyond 180 deg (nframes = iLenSinogr - 1) */
+    int iRevIdx = -1, i, k;
+    const int ixFrm = W;
+    const int ixlen = W / bin;
+    const int iNativeXdim = W * 2;
+    for (i = 0; i < nframes; i++) {
+        if (fdeg[i] > 180.) { iRevIdx = i; break; }
+    }
+    if (iRevIdx < 0) return -21051;
+    {
+        const int nout = iRevIdx;
+        memset(ofs, 0, sizeof(int16_t) * (size_t)nout * out_width);
+        for (i = 0; i < nout; i++) {
+            const double th = fdeg[i];
+            const int16_t* s1;
+            const int16_t* s2;
+            int16_t* o;
+            if (!(binc[i] & 1)) continue;
+            while ((fdeg[iRevIdx] - th < 180) | ((binc[iRevIdx] & 1) == 0)) {   /* :3566-3569 */
+                iRevIdx++;
+                if (iRevIdx >= nframes) return -21052;
+            }
+            s1 = sino + (size_t)i * ixlen;           /* rows are already binned: W / bin */
+            s2 = sino + (size_t)iRevIdx * ixlen;
+            o = ofs + (size_t)i * out_width;
+            if (center > ixFrm / 2.) {                                           /* :3575-3616 */
+                const double fc = (center + deltac + trim) / bin;
+                const int iCent = (int)fc;
+                const double dCent = fc - iCent;
+                const int g = ixFrm / bin - 1 - iCent;
+                short lastDens = 0;
+                for (k = trim / bin; k <= iCent - g; k++) o[k - trim / bin] = s1[k];
+                for (k = iCent - g + 1; k < (iNativeXdim - trim) / bin; k++) {
+                    const int idx = iCent - (k - iCent);
+                    if (dCent < 0.5) {
+                        if (idx >= 0 && idx < ixlen - 1) {
+                            const short ir0 = s2[idx + 1], ir1 = s2[idx];
+                            lastDens = (short)(ir0 + (ir1 - ir0) * (1 - 2 * dCent));
+                            if (k <= iCent + g) {
+                                const double ratio = (double)(k - (iCent - g)) / (g * 2 + 1);
+                                lastDens = (short)(ratio * lastDens + (1. - ratio) * s1[k]);
+                            }
+                        }
+                    } else {
+                        if (idx >= 1 && idx < ixlen) {
+                            const short ir0 = s2[idx], ir1 = s2[idx - 1];
+                            lastDens = (short)(ir1 + (ir0 - ir1) * (2 - 2 * dCent));
+                            if (k <= iCent + g) {
+                                const double ratio = (double)(k - (iCent - g)) / (g * 2 + 1);
+                                lastDens = (short)(ratio * lastDens + (1. - ratio) * s1[k]);
+                            }
+                        }
+                    }
+                    o[k - trim / bin] = lastDens;
+                }
+            } else {                                                             /* :3617-3660 */
+                const double fc = (center + deltac) / bin;
+                const int iCent = (int)(fc + 0.5);
+                const double dCent = fc - (int)fc;
+                const int g = iCent;
+                const int shift = (ixFrm - trim) / bin;
+                short lastDens = 0;
+                for (k = (ixFrm - 1 - trim) / bin; k >= iCent + g; k--) o[k + shift] = s1[k];
+                if (dCent < 0.5) {
+                    for (k = iCent + g - 1; k >= iCent * 2 + 1 - ixFrm / bin; k--) {
+                        const int idx = iCent + ((iCent - 1) - k) + 1;
+                        if (idx >= 0 && idx < ixlen - 1) {
+                            const short ir1 = s2[idx], ir2 = s2[idx + 1];
+                            lastDens = (short)(ir1 + (ir2 - ir1) * (2 * dCent));
+                            if (k >= iCent - g) {
+                                const double ratio = (double)((iCent + g) - k) / (g * 2 + 1);
+                                lastDens = (short)(ratio * lastDens + (1. - ratio) * s1[k]);
+                            }
+                        }
+                        o[k + shift] = lastDens;
+                    }
+                } else {
+                    for (k = iCent + g - 1; k >= iCent * 2 - ixFrm / bin; k--) {
+                        const int idx = iCent + ((iCent - 1) - k);
+                        if (idx >= 0 && idx < ixlen - 1) {
+                            const short ir0 = s2[idx], ir1 = s2[idx + 1];
+                            lastDens = (short)(ir0 + (ir1 - ir0) * (2 * dCent - 1));
+                            if (k >= iCent - g) {
+                                const double ratio = (double)((iCent + g) - k) / (g * 2 + 1);
+                                lastDens = (short)(ratio * lastDens + (1. - ratio) * s1[k]);
+                            }
+                        }
+                        o[k + shift] = lastDens;
+                    }
+                }
+            }
+        }
+        return nout;
+    }
+}

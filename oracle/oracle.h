@@ -108,6 +108,16 @@ void orc_bp_rows_linear_f64(const float* px, int P, const float* fdeg, float til
 void orc_output_stage(int32_t* img, int N, int z, double center_user_plus_half, int bin,
                       int* imin, int* imax);
 
+/* Offset-CT stitch (gazoDoc.cpp:3536-3665) for one multiplex row: pairs every
+ * sampled frame i < iRevIdx0 (the first frame beyond 180 deg, gazoDoc.cpp:2967-2978)
+ * with the first later sampled frame at >= 180 deg further, and writes the
+ * double-width row of frame i into ofs[i] (rows not written stay zero).
+ * center = user centre + 0.5 (gazoDoc.cpp:3560), deltac = deltaCent*j*bin.
+ * Returns the number of output rows, or a negative error (21051/21052). */
+int orc_offset_stitch(const int16_t* sino, int nframes, int W, const float* fdeg,
+                      const char* binc, double center, double deltac, int bin, int trim,
+                      int out_width, int16_t* ofs);
+
 #ifdef __cplusplus
 }
 #endif

@@ -64,6 +64,8 @@ def _declare(L):
     L.orc_recon_of64.argtypes = [fp, C.c_int, fp, C.c_float, gp, fp, dp]
     L.orc_bp_rows_linear_f64.argtypes = [fp, C.c_int, fp, C.c_float, gp, C.c_int, dp]
     L.orc_output_stage.argtypes = [ip, C.c_int, C.c_int, C.c_double, C.c_int, ip, ip]
+    L.orc_offset_stitch.argtypes = [sp, C.c_int, C.c_int, fp, C.c_char_p, C.c_double, C.c_double,
+                                    C.c_int, C.c_int, C.c_int, sp]
     L.orc_cpu_isa.restype = C.c_int
     L.orc_cpu_baseline.argtypes = [sp, C.c_int, fp, C.c_float, gp, fp, C.c_int, C.c_int, ip]
     L.orc_cpu_baseline.restype = C.c_double
@@ -256,3 +258,22 @@ def cpu_baseline(sino_i16, fdeg, g, filt, nthreads, isa=-1, tilt=0.0):
     if t < 0:
         raise MemoryError("cpu baseline allocation failed")
     return t, img
+
+
+def offset_stitch(sino_i16, fdeg, binc, center_plus_half, bin_=1, trim=0, deltac=0.0, out_width=None):
+    """Offset-CT stitch restated from gazoDoc.cpp:3536-3665 (int16 rows, already
+    binned: the raw frame width is row width x bin_)."""
+    sino = np.ascontiguousarray(sino_i16, np.int16)
+    nframes, Wb = sino.shape
+    W = Wb * bin_
+    fd = _f32(fdeg)
+    b = np.ascontiguousarray(binc, np.int8)
+    if out_width is None:
+        out_width = (2 * W - trim) // bin_
+    out = np.zeros((nframes, out_width), np.int16)
+    n = lib().orc_offset_stitch(_ptr(sino, C.c_int16), nframes, W, _ptr(fd, C.c_float),
+                                b.ctypes.data_as(C.c_char_p), float(center_plus_half), float(deltac),
+                                int(bin_), int(trim), int(out_width), _ptr(out, C.c_int16))
+    if n < 0:
+        raise ValueError(f"offset stitch error {-n}")
+    return out[:n]

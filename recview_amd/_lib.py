@@ -69,7 +69,7 @@ EXPORTS = [
     "RecviewGetErrors", "RecviewClearErrors", "RecviewLastMessage", "RecviewSetCompatOptions",
     "rvPlanCreate", "rvPlanExecute", "rvPlanExecuteGraph", "rvPlanFilter", "rvPlanBackproject",
     "rvPlanFilteredRows", "rvPlanSetFilteredRows", "rvPlanInfo", "rvPlanDestroy",
-    "RecviewSetFilter", "RecviewGeometry",
+    "RecviewSetFilter", "RecviewGeometry", "RecviewOffsetCTPairs", "RecviewOffsetCTStitch",
 ]
 
 
@@ -122,6 +122,9 @@ def _declare(L):
     L.RecviewSetFilter.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, fp]
     L.RecviewSetFilter.restype = None
     L.RecviewGeometry.argtypes = [C.c_int, C.c_int, ip, ip]
+    L.RecviewOffsetCTPairs.argtypes = [fp, C.c_char_p, C.c_int, ip]
+    L.RecviewOffsetCTStitch.argtypes = [vp, C.c_int, C.c_int, vp, C.c_int, C.c_double, C.c_double,
+                                        C.c_int, C.c_int, C.c_int, vp, vp]
 
 
 def lib():

@@ -83,6 +83,20 @@ struct BpLaunch {
 };
 
 void launch_bp(const BpLaunch& L, int grid, hipStream_t stream);
+
+// ------------------------------------------------------- offset-CT stitch
+// gazoDoc.cpp:3536-3665 for one multiplex row of every frame below 180 deg.
+struct StitchJob {
+    const void* sino;      // [frames][W] int16 or float
+    int W, bin, trim;
+    double center;         // user centre + 0.5 (gazoDoc.cpp:3560)
+    double deltac;         // deltaCent * j * bin
+    const int* pair;       // [nout] partner frame of output row i, -1 = row left zero
+    int nout, out_width;
+    void* out;             // [nout][out_width]
+    int is_float;
+};
+void launch_stitch(const StitchJob& J, hipStream_t stream);
 int bp_tile_side(int tile);                 // tile width in pixels for a config id
 int bp_tile_h(int tile);                    // tile height in pixels
 int bp_default_chunk(int mode, int tile);   // angles per chunk

@@ -7,16 +7,19 @@ noise at 1e4 photons/ray), reconstructed to 2048 x 2048 float32 with the Hann
 filter and linear detector interpolation.  One step = filter + back-projection
 of the whole stack, sinograms already resident in HBM, images written to HBM.
 
+--workload c3: 256 slices of uneven width {512..4096} and projection count
+{450..3600}, a quarter of them 360-degree offset-axis scans stitched on the GPU
+to double width, per-shape autotuned tiles, one hipGraph per step.
+
 Prints one JSON line.  value = slices/s for the whole job; tau = time per
 pixel*projection (README.md:96 of the reference); roofline = FP32 VALU
 fraction of the back-projection kernel (6 algorithmic FLOP per pixel*projection
 update), timed with HIP events on the launch stream; cpu_baseline = the
 reference CPU path (projx64 AVX-512 loop restated in C, reference threading)
-timed on this host.
+timed on this host (default workload only).
 """
 import argparse
 import json
-import math
 import os
 import re
 import sys
@@ -53,6 +56,39 @@ def make_stack(nslices, P, W, seed, device, photons=1e4):
     return out, fdeg, c_det
 
 
+def make_c3(nslices, seed, device, interp, store):
+    """256 uneven slices; a quarter are 360-degree offset-axis scans (stitched to 2W)."""
+    import numpy as np
+    import torch
+    from recview_amd import autotune, fbp, phantom
+    rng = phantom.SplitMix64(seed)
+    widths, projs = (512, 1024, 2048, 4096), (450, 900, 1800, 3600)
+    specs, sinos, shapes = [], [], []
+    table = autotune.load()
+    for s in range(nslices):
+        W = widths[rng.next_u64() % 4]
+        P = projs[rng.next_u64() % 4]
+        offset = (s % 4 == 3)
+        ell = phantom.random_ellipses(seed * 1000 + s, count=10, mu_range=(0.0, 2.0 / W))
+        if offset:
+            side = 1 if rng.next_u64() % 2 else -1
+            c_det = W / 2 + side * 0.3 * W + 0.37
+            fdeg = (np.arange(2 * P + 1) * (180.0 / P)).astype(np.float32)
+            fov = 2 * min(c_det, W - c_det) + W       # the stitched FOV
+            sino = phantom.sinogram(ell, W, fdeg, c_det, scale=0.9 * fov, device=device).float()
+            N, Pe = 2 * W, P + 1
+        else:
+            c_det = W / 2 + 0.37
+            fdeg = phantom.angles_deg(P)
+            sino = phantom.sinogram(ell, W, fdeg, c_det, scale=W, device=device).float()
+            N, Pe = W, P
+        tile = autotune.tile_for(N, Pe, interp, store, table)
+        specs.append(fbp.SliceSpec(W, fdeg, c_det, tile=tile, offset_ct=offset))
+        sinos.append(sino.contiguous())
+        shapes.append((N, Pe))
+    return specs, sinos, shapes
+
+
 def read_traffic(name):
     """HBM bytes per launch of the back-projection kernel, from the committed
     rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated)."""
@@ -65,8 +101,6 @@ def read_traffic(name):
 
 def cpu_baseline(sino_f32, fdeg, c_det, threads):
     """The reference CPU reconstruction of one slice (int16 input, LUT16, AVX-512)."""
-    import numpy as np
-    import torch
     from oracle import pyoracle as O
     from recview_amd import phantom
     rows = phantom.to_int16(sino_f32.double().cpu()).numpy()
@@ -85,7 +119,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--slices", type=int, default=64)
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3"])
+    ap.add_argument("--slices", type=int, default=0, help="default 64 (c2) / 256 (c3)")
     ap.add_argument("--proj", type=int, default=1800)
     ap.add_argument("--width", type=int, default=2048)
     ap.add_argument("--interp", default="linear", choices=["linear", "lut16_f32", "lut16_i32"])
@@ -108,41 +143,49 @@ def main():
     from recview_amd import fbp
 
     dev = torch.device("cuda:0")
-    S, P, W = args.slices, args.proj, args.width
     t0 = time.time()
-    sinos, fdeg, c_det = make_stack(S, P, W, 1002, dev)
-    if args.interp == "lut16_i32":
-        from recview_amd import phantom
-        sinos = phantom.to_int16(sinos)
+    odt = torch.int32 if args.interp == "lut16_i32" else torch.float32
+    if args.workload == "c2":
+        S, P, W = (args.slices or 64), args.proj, args.width
+        sinos, fdeg, c_det = make_stack(S, P, W, 1002, dev)
+        if args.interp == "lut16_i32":
+            from recview_amd import phantom
+            sinos = phantom.to_int16(sinos)
+        dtype = "i16" if args.interp == "lut16_i32" else "f32"
+        specs = [fbp.SliceSpec(W, fdeg, c_det, sino_dtype=dtype) for _ in range(S)]
+        sino_list = [sinos[s] for s in range(S)]
+        shapes = [(W, P)] * S
+        desc = (f"C2: {S}-slice stack, {P} proj x {W} det -> {W}^2, {args.interp} interpolation, "
+                f"{args.store} filtered sinogram")
+    else:
+        S = args.slices or 256
+        specs, sino_list, shapes = make_c3(S, 1003, dev, args.interp, args.store)
+        desc = (f"C3: {S} uneven slices (W 512..4096, P 450..3600, 1/4 offset-axis 360-deg scans "
+                f"stitched to 2W), per-shape tiles, {args.interp} {args.store}")
     torch.cuda.synchronize()
-    log(f"inputs: {S} x {P} x {W} generated in {time.time() - t0:.1f}s")
+    log(f"inputs generated in {time.time() - t0:.1f}s")
 
-    dtype = "i16" if args.interp == "lut16_i32" else "f32"
-    specs = [fbp.SliceSpec(W, fdeg, c_det, sino_dtype=dtype) for _ in range(S)]
     plan = fbp.FbpPlan(specs, interp=args.interp, store=args.store, persistent=bool(args.persistent),
                        tile=args.tile, angles_per_chunk=args.chunk, lds_half=bool(args.lds_half))
-    N = plan.out_n(0)
-    odt = torch.int32 if args.interp == "lut16_i32" else torch.float32
-    images = torch.empty(S, N, N, dtype=odt, device=dev)
-    sino_list = [sinos[s] for s in range(S)]
-    out_list = [images[s] for s in range(S)]
+    outs = [torch.empty(plan.out_n(s), plan.out_n(s), dtype=odt, device=dev) for s in range(S)]
     stream = torch.cuda.current_stream()
+    updates = float(sum(n * n * p for n, p in shapes))
 
     def step():
-        plan.execute(sino_list, out_list, stream=stream, graph=bool(args.graph))
+        plan.execute(sino_list, outs, stream=stream, graph=bool(args.graph))
 
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
 
-    # Stage timing with HIP events on the launch stream (eager, one pass).
+    # Stage timing with HIP events on the launch stream (eager passes).
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
     bp_ms = []
     for _ in range(max(1, min(3, args.steps))):
         ev[0].record(stream)
         plan.filter(sino_list, stream=stream)
         ev[1].record(stream)
-        plan.backproject(out_list, stream=stream)
+        plan.backproject(outs, stream=stream)
         ev[2].record(stream)
         torch.cuda.synchronize()
         bp_ms.append(ev[1].elapsed_time(ev[2]))
@@ -157,24 +200,32 @@ def main():
     elapsed = time.perf_counter() - t_start
     ms_per_step = 1e3 * elapsed / args.steps
     slices_per_s = S * args.steps / elapsed
-    updates = float(N) * N * P * S
     tau_ns = elapsed / args.steps / updates * 1e9
     achieved = FLOP_PER_UPDATE * updates / (bp_ms * 1e-3) / 1e12
-    if not torch.isfinite(images.float()).all():
-        raise RuntimeError("non-finite image values")
+    for o in outs:
+        if not torch.isfinite(o.float()).all():
+            raise RuntimeError("non-finite image values")
 
     cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and args.workload == "c2":
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        secs, isa = cpu_baseline(sinos[0] if args.interp != "lut16_i32" else sinos[0].float() / 3000.0,
-                                 fdeg, c_det, threads)
+        src = sino_list[0] if args.interp != "lut16_i32" else sino_list[0].float() / 3000.0
+        secs, isa = cpu_baseline(src, fdeg, c_det, threads)
+        N = W
         cpu = {"value": round(1.0 / secs, 4), "unit": "slices/s", "cores": threads, "kind": "port",
                "sample": f"1 slice {P}x{W} -> {N}^2, int16 input, reference LUT16 path "
                          f"({isa} restatement of projx64, row bands + per-thread filtering), "
                          f"tau {secs / (N * N * P) * 1e9:.4f} ns"}
 
-    traffic = read_traffic("bp_linear_c2_traffic.md") if (args.interp == "linear" and args.store == "f32"
-                                                         and S == 64 and P == 1800 and W == 2048) else None
+    traffic = None
+    if args.workload == "c2" and args.interp == "linear" and args.store == "f32" and S == 64 \
+            and args.proj == 1800 and args.width == 2048:
+        traffic = read_traffic("bp_linear_c2_traffic.md")
+    config = {"workload": desc, "slices": S, "interp": args.interp, "store": args.store,
+              "persistent": bool(args.persistent), "graph": bool(args.graph),
+              "lds_half": bool(args.lds_half), "updates": updates}
+    if args.workload == "c2":
+        config.update({"projections": P, "detector": W, "image": W, "tile": args.tile})
     result = {
         "metric": METRIC,
         "value": round(slices_per_s, 3),
@@ -185,19 +236,14 @@ def main():
         "ms_per_step": round(ms_per_step, 3),
         "higher_is_better": True,
         "scaling": "weak",
-        "vs_baseline": round(REF_TAU_NS / tau_ns, 3),
+        "vs_baseline": round(REF_TAU_NS / tau_ns, 3) if args.workload == "c2" else None,
         "dtype": "f32" if args.interp != "lut16_i32" else "int32",
         "data": "synthetic",
         "tau_ns": round(tau_ns, 7),
         "tau_bp_ns": round(bp_ms * 1e-3 / updates * 1e9, 7),
         "filter_ms": round(filt_ms, 3),
         "bp_ms": round(bp_ms, 3),
-        "config": {"workload": f"C2: {S}-slice stack, {P} proj x {W} det -> {N}^2, "
-                               f"{args.interp} interpolation, {args.store} filtered sinogram",
-                   "slices": S, "projections": P, "detector": W, "image": N,
-                   "interp": args.interp, "store": args.store, "tile": args.tile,
-                   "persistent": bool(args.persistent), "graph": bool(args.graph),
-                   "lds_half": bool(args.lds_half)},
+        "config": config,
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
                      "traffic": traffic},

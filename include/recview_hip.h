@@ -221,6 +221,16 @@ typedef struct {
     int out_n;
     double pix_zoom;
     double x0, y0;
+    /* Tile configuration of this slice: -1 = the plan's (RvOptions.tile);
+     * 0..3 = 64x64, 64x128, 32x32, 128x64 (chosen per shape by an autotuner).
+     * Slices with different tiles run as separate launches of one execution. */
+    int tile;
+    /* 1: the input is a 360-degree scan of P frames with the axis off centre
+     * (fdeg over 0..360, optional frame flags binc); the plan stitches it on the
+     * GPU to P' rows of 2W samples (RecviewOffsetCTStitch) and reconstructs
+     * those, with the centre remap of gazoDoc.cpp:3832-3837. */
+    int offset_ct;
+    const char* binc;
 } RvSliceSpec;
 
 typedef struct {
@@ -258,6 +268,8 @@ int rvPlanFilteredRows(rv_plan_t plan, int s, void** rows, float** scales, int* 
 int rvPlanSetFilteredRows(rv_plan_t plan, int s, const float* d_rows, rv_stream_t stream);
 /* Plan facts: image side, number of tiles, tile shape, workspace bytes. */
 int rvPlanInfo(rv_plan_t plan, int s, int* out_n, int* n_fft, int* tiles, int* tile_w, int* tile_h);
+/* Rows and width that slice s reconstructs (after offset-CT stitching). */
+int rvPlanRows(rv_plan_t plan, int s, int* nrows, int* width);
 void rvPlanDestroy(rv_plan_t plan);
 
 /* Offset CT (gazoDoc.cpp:3536-3665, 2967-2978).  RecviewOffsetCTPairs pairs

@@ -45,7 +45,8 @@ class RvSliceSpec(C.Structure):
         ("tilt", C.c_float), ("fdeg", C.POINTER(C.c_float)), ("filter", C.c_int),
         ("cutoff", C.c_double), ("order", C.c_double), ("sino_dtype", C.c_int),
         ("pad_mode", C.c_int), ("out_n", C.c_int), ("pix_zoom", C.c_double),
-        ("x0", C.c_double), ("y0", C.c_double),
+        ("x0", C.c_double), ("y0", C.c_double), ("tile", C.c_int), ("offset_ct", C.c_int),
+        ("binc", C.c_char_p),
     ]
 
 
@@ -68,7 +69,7 @@ EXPORTS = [
     "CUDA_MALLOC_HOST_INT", "CUDA_FREE_HOST", "blocksize", "RecviewSetDlgCtrl",
     "RecviewGetErrors", "RecviewClearErrors", "RecviewLastMessage", "RecviewSetCompatOptions",
     "rvPlanCreate", "rvPlanExecute", "rvPlanExecuteGraph", "rvPlanFilter", "rvPlanBackproject",
-    "rvPlanFilteredRows", "rvPlanSetFilteredRows", "rvPlanInfo", "rvPlanDestroy",
+    "rvPlanFilteredRows", "rvPlanSetFilteredRows", "rvPlanInfo", "rvPlanRows", "rvPlanDestroy",
     "RecviewSetFilter", "RecviewGeometry", "RecviewOffsetCTPairs", "RecviewOffsetCTStitch",
 ]
 
@@ -117,6 +118,7 @@ def _declare(L):
     L.rvPlanFilteredRows.argtypes = [vp, C.c_int, C.POINTER(vp), C.POINTER(vp), ip]
     L.rvPlanSetFilteredRows.argtypes = [vp, C.c_int, vp, vp]
     L.rvPlanInfo.argtypes = [vp, C.c_int, ip, ip, ip, ip, ip]
+    L.rvPlanRows.argtypes = [vp, C.c_int, ip, ip]
     L.rvPlanDestroy.argtypes = [vp]
     L.rvPlanDestroy.restype = None
     L.RecviewSetFilter.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, fp]

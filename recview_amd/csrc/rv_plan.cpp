@@ -2,18 +2,20 @@
 //
 // A plan fixes everything that does not depend on the sinogram values: the
 // geometry of every slice, its filter table, the FP64 angle table, the
-// filtered-sinogram workspace in HBM and the cost-sorted tile work list.
-// Execution is then: one filter launch per slice (LDS FFT, all rows), one
-// back-projection launch for the whole batch (tiles of every slice, in one
-// grid or through a persistent work counter), optionally captured once into a
-// hipGraph and replayed.  The reference has no counterpart: it reconstructs
-// one slice at a time (gazoDoc.cpp:1965-1991).
+// filtered-sinogram workspace in HBM, the offset-CT pairing and stitch buffers,
+// and the cost-sorted tile work lists.  Execution is then: one stitch launch per
+// offset-CT slice, one filter launch per slice (LDS FFT, all rows), and one
+// back-projection launch per tile configuration (tiles of every slice of that
+// configuration, in one grid or through a persistent work counter), optionally
+// captured once into a hipGraph and replayed.  The reference has no
+// counterpart: it reconstructs one slice at a time (gazoDoc.cpp:1965-1991).
 #include "recview_hip.h"
 #include "rv_common.h"
 
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -22,14 +24,31 @@ namespace {
 
 struct SliceState {
     RvSliceSpec spec;
-    std::vector<float> fdeg;
-    rv::Geom g;
+    std::vector<float> fdeg;     // angles of the reconstructed rows
+    rv::Geom g;                  // geometry of the reconstructed rows
     int out_n = 0;
     int nang = 0;
-    size_t rows_off = 0;    // element offset of row 0 in the workspace
-    size_t ang_off = 0;     // first entry in the angle table
-    size_t filt_off = 0;    // first entry in the filter tables
+    size_t rows_off = 0;         // element offset of row 0 in the filtered workspace
+    size_t ang_off = 0;          // first entry in the angle table
+    size_t filt_off = 0;         // first entry in the filter tables
     int tiles_x = 0, tiles_y = 0, tile_base = 0;
+    int group = 0;
+    int tile = 0;
+    // offset CT
+    bool offset = false;
+    int raw_W = 0;
+    size_t stitch_off = 0;       // element offset in the stitch buffer
+    size_t pair_off = 0;         // first entry in the pair table
+};
+
+struct Group {
+    int tile = 0, chunk = 0;
+    std::vector<int> slices;
+    std::vector<rv::BpSlice> h;
+    rv::BpSlice* d_slices = nullptr;
+    int* d_order = nullptr;
+    int total_items = 0;
+    int grid = 0;
 };
 
 }  // namespace
@@ -39,19 +58,16 @@ struct rv_plan_opaque {
     RvOptions opt{};
     int mode = rv::BP_LINEAR;
     int store = 0;
-    int tile = 0, chunk = 32;
     std::vector<SliceState> sl;
+    std::vector<Group> groups;
     void* d_rows = nullptr;
     float* d_scales = nullptr;
     float* d_filts = nullptr;
     rv::AngleEntry* d_ang = nullptr;
-    rv::BpSlice* d_slices = nullptr;
-    int* d_order = nullptr;
-    unsigned int* d_counter = nullptr;
-    std::vector<rv::BpSlice> h_slices;
+    unsigned int* d_counter = nullptr;   // one per group
+    void* d_stitch = nullptr;            // stitched offset-CT rows
+    int* d_pairs = nullptr;
     std::vector<void*> cur_out;
-    int total_items = 0;
-    int grid = 0;
     hipStream_t cap_stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -68,13 +84,17 @@ void plan_free(rv_plan_t p)
     if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
     if (p->graph) (void)hipGraphDestroy(p->graph);
     if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);
+    for (auto& G : p->groups) {
+        (void)hipFree(G.d_slices);
+        (void)hipFree(G.d_order);
+    }
     (void)hipFree(p->d_rows);
     (void)hipFree(p->d_scales);
     (void)hipFree(p->d_filts);
     (void)hipFree(p->d_ang);
-    (void)hipFree(p->d_slices);
-    (void)hipFree(p->d_order);
     (void)hipFree(p->d_counter);
+    (void)hipFree(p->d_stitch);
+    (void)hipFree(p->d_pairs);
     delete p;
 }
 
@@ -87,17 +107,21 @@ int fail(rv_plan_t p, const std::string& m)
 
 bool upload_slices(rv_plan_t p, void* const* d_out)
 {
-    bool changed = false;
-    for (size_t s = 0; s < p->sl.size(); s++) {
-        if (p->cur_out[s] != d_out[s]) {
-            p->cur_out[s] = d_out[s];
-            p->h_slices[s].out = d_out[s];
-            changed = true;
+    for (auto& G : p->groups) {
+        bool changed = false;
+        for (size_t q = 0; q < G.slices.size(); q++) {
+            const int s = G.slices[q];
+            if (p->cur_out[s] != d_out[s]) {
+                G.h[q].out = d_out[s];
+                changed = true;
+            }
         }
+        if (changed && !rv::hip_ok(hipMemcpy(G.d_slices, G.h.data(), sizeof(rv::BpSlice) * G.h.size(),
+                                             hipMemcpyHostToDevice), "slice table upload"))
+            return false;
     }
-    if (!changed) return true;
-    return rv::hip_ok(hipMemcpy(p->d_slices, p->h_slices.data(), sizeof(rv::BpSlice) * p->h_slices.size(),
-                                hipMemcpyHostToDevice), "slice table upload");
+    for (size_t s = 0; s < p->sl.size(); s++) p->cur_out[s] = d_out[s];
+    return true;
 }
 
 int launch_filters(rv_plan_t p, const void* const* d_sino, hipStream_t st)
@@ -106,13 +130,31 @@ int launch_filters(rv_plan_t p, const void* const* d_sino, hipStream_t st)
     for (size_t s = 0; s < p->sl.size(); s++) {
         const SliceState& S = p->sl[s];
         const rv::Geom& g = S.g;
+        const bool is_float = (S.spec.sino_dtype != RV_SINO_I16);
+        const void* src = d_sino[s];
+        if (S.offset) {
+            rv::StitchJob J{};
+            J.sino = d_sino[s];
+            J.W = S.raw_W;
+            J.bin = 1;
+            J.trim = 0;
+            J.center = S.spec.center;
+            J.deltac = 0.0;
+            J.pair = p->d_pairs + S.pair_off;
+            J.nout = S.nang;
+            J.out_width = g.W;
+            J.out = (unsigned char*)p->d_stitch + S.stitch_off * (is_float ? 4 : 2);
+            J.is_float = is_float ? 1 : 0;
+            rv::launch_stitch(J, st);
+            src = J.out;
+        }
         const float* tw = rv::device_twiddles(p->dev, g.ndimp);
         if (!tw) { rv::set_message("twiddle table allocation failed"); return -1; }
         rv::FilterJob J{};
-        J.sino = d_sino[s];
+        J.sino = src;
         J.in_stride = g.W;
         J.nrows = S.nang;
-        J.in_kind = (S.spec.sino_dtype == RV_SINO_I16) ? 0 : 1;
+        J.in_kind = is_float ? 1 : 0;
         J.W = g.W; J.z = g.z; J.ic = g.ic; J.n = g.n; J.np = g.ndimp; J.N = g.N;
         const bool zero = (S.spec.pad_mode == RV_PAD_ZERO);
         J.place = zero ? g.n / 2 : g.n / 2 - 1;
@@ -135,18 +177,21 @@ int launch_filters(rv_plan_t p, const void* const* d_sino, hipStream_t st)
 
 int launch_backprojection(rv_plan_t p, hipStream_t st)
 {
-    rv::BpLaunch L{};
-    L.slices = p->d_slices;
-    L.nslices = (int)p->sl.size();
-    L.total_items = p->total_items;
-    L.item_order = p->d_order;
-    L.counter = p->opt.persistent ? p->d_counter : nullptr;
-    L.mode = p->mode;
-    L.rule = p->opt.index_rule ? 1 : 0;
-    L.chunk = p->chunk;
-    L.tile = p->tile;
-    if (L.counter) rv::launch_zero_u32(p->d_counter, 1, st);
-    rv::launch_bp(L, p->grid, st);
+    for (size_t gi = 0; gi < p->groups.size(); gi++) {
+        const Group& G = p->groups[gi];
+        rv::BpLaunch L{};
+        L.slices = G.d_slices;
+        L.nslices = (int)G.slices.size();
+        L.total_items = G.total_items;
+        L.item_order = G.d_order;
+        L.counter = p->opt.persistent ? p->d_counter + gi : nullptr;
+        L.mode = p->mode;
+        L.rule = p->opt.index_rule ? 1 : 0;
+        L.chunk = G.chunk;
+        L.tile = G.tile;
+        if (L.counter) rv::launch_zero_u32(p->d_counter + gi, 1, st);
+        rv::launch_bp(L, G.grid, st);
+    }
     return rv::hip_ok(hipGetLastError(), "back-projection launch") ? 0 : -1;
 }
 
@@ -170,40 +215,63 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
     // kernel is LDS-bound at 8 B/update and VALU-bound at 4 B/update with the
     // extra fract); lds_half = 1 selects the FP16-pair variant.
     if (p->mode == rv::BP_LINEAR && p->store != 0 && opt->lds_half) p->mode = rv::BP_LINEAR_H;
-    p->tile = (opt->tile >= 0 && opt->tile <= 3) ? opt->tile : 0;
-    p->chunk = opt->angles_per_chunk > 0 ? opt->angles_per_chunk : rv::bp_default_chunk(p->mode, p->tile);
-    if (p->chunk > rv::bp_max_chunk(p->mode)) p->chunk = rv::bp_max_chunk(p->mode);
-    while (p->chunk > 1 && rv::bp_lds_bytes(p->mode, p->tile, p->chunk) > 160 * 1024) p->chunk /= 2;
+    const bool linear = (p->mode == rv::BP_LINEAR || p->mode == rv::BP_LINEAR_H);
+    const int plan_tile = (opt->tile >= 0 && opt->tile <= 3) ? opt->tile : 0;
 
-    const int tw = rv::bp_tile_side(p->tile), th = rv::bp_tile_h(p->tile);
-    size_t rows_elems = 0, nang_total = 0, filt_total = 0;
+    size_t rows_elems = 0, nang_total = 0, filt_total = 0, stitch_elems = 0, pair_total = 0;
+    std::vector<int> all_pairs;
     p->sl.resize(nslices);
     for (int s = 0; s < nslices; s++) {
         SliceState& S = p->sl[s];
         S.spec = specs[s];
         if (!specs[s].fdeg || specs[s].P <= 0) return fail(p, "slice without angles");
-        S.fdeg.assign(specs[s].fdeg, specs[s].fdeg + specs[s].P);
         S.spec.fdeg = nullptr;
-        S.g = rv::geometry(S.spec.W, S.spec.iInterpolation, S.spec.center, true);
-        if (S.spec.W < 4 || S.g.ndimp < 3 || S.g.n > 16384)
+        S.spec.binc = nullptr;
+        double center = S.spec.center;
+        int W = S.spec.W;
+        if (S.spec.offset_ct) {
+            // pair the 360-degree frames, stitch to 2W (gazoDoc.cpp:3536-3665)
+            if (S.spec.iInterpolation != 2) return fail(p, "offset CT needs iInterpolation = 2");
+            std::vector<int> pr(S.spec.P);
+            std::vector<char> binc(S.spec.P, (char)CGAZODOC_BINC_SAMPLE);
+            if (specs[s].binc) binc.assign(specs[s].binc, specs[s].binc + S.spec.P);
+            const int nout = RecviewOffsetCTPairs(specs[s].fdeg, binc.data(), S.spec.P, pr.data());
+            if (nout <= 0) return fail(p, "offset-CT pairing failed (no partner frames)");
+            S.offset = true;
+            S.raw_W = W;
+            S.fdeg.assign(specs[s].fdeg, specs[s].fdeg + nout);
+            S.pair_off = pair_total;
+            all_pairs.insert(all_pairs.end(), pr.begin(), pr.begin() + nout);
+            pair_total += nout;
+            // DeconvBackProj's centre remap on the user centre (gazoDoc.cpp:3832-3837)
+            if (center - 0.5 <= W / 2.0) center += W;
+            W = 2 * W;
+            S.stitch_off = stitch_elems;
+            stitch_elems += (size_t)nout * W;
+        } else {
+            S.fdeg.assign(specs[s].fdeg, specs[s].fdeg + specs[s].P);
+        }
+        S.g = rv::geometry(W, S.spec.iInterpolation, center, true);
+        if (W < 4 || S.g.ndimp < 3 || S.g.n > 16384)
             return fail(p, "detector width / zoom outside the supported FFT range (n <= 16384)");
         if (S.spec.out_n > 0) {
-            if (p->mode != rv::BP_LINEAR && p->mode != rv::BP_LINEAR_H) return fail(p, "custom output grids need the LINEAR mode");
+            if (!linear) return fail(p, "custom output grids need the LINEAR mode");
             if (S.g.z != 1) return fail(p, "custom output grids need iInterpolation = 2");
             if (!(S.spec.pix_zoom >= 1.0)) return fail(p, "pix_zoom must be >= 1");
             S.out_n = S.spec.out_n;
         } else {
             S.out_n = S.g.N;
         }
-        S.nang = S.spec.P;
+        S.nang = (int)S.fdeg.size();
         S.rows_off = rows_elems;
         S.ang_off = nang_total;
         S.filt_off = filt_total;
         rows_elems += (size_t)S.nang * S.g.N;
         nang_total += (size_t)S.nang;
         filt_total += (size_t)S.g.n;
-        S.tiles_x = (S.out_n + tw - 1) / tw;
-        S.tiles_y = (S.out_n + th - 1) / th;
+        S.tile = (S.spec.tile >= 0 && S.spec.tile <= 3) ? S.spec.tile : plan_tile;
+        S.tiles_x = (S.out_n + rv::bp_tile_side(S.tile) - 1) / rv::bp_tile_side(S.tile);
+        S.tiles_y = (S.out_n + rv::bp_tile_h(S.tile) - 1) / rv::bp_tile_h(S.tile);
     }
 
     // filter tables and angle tables
@@ -233,61 +301,94 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
         }
     }
 
-    // work items, cost-sorted (longest projection lists first)
-    std::vector<int> slice_order(nslices);
-    std::iota(slice_order.begin(), slice_order.end(), 0);
-    std::stable_sort(slice_order.begin(), slice_order.end(),
-                     [&](int a, int b) { return p->sl[a].nang > p->sl[b].nang; });
-    int base = 0;
+    // groups by tile configuration; work items cost-sorted within each group
+    std::map<int, int> gidx;
     for (int s = 0; s < nslices; s++) {
-        p->sl[s].tile_base = base;
-        base += p->sl[s].tiles_x * p->sl[s].tiles_y;
-    }
-    p->total_items = base;
-    std::vector<int> order;
-    order.reserve(base);
-    for (int s : slice_order) {
-        const SliceState& S = p->sl[s];
-        for (int t = 0; t < S.tiles_x * S.tiles_y; t++) order.push_back(S.tile_base + t);
+        auto it = gidx.find(p->sl[s].tile);
+        if (it == gidx.end()) {
+            gidx[p->sl[s].tile] = (int)p->groups.size();
+            Group G;
+            G.tile = p->sl[s].tile;
+            G.chunk = opt->angles_per_chunk > 0 ? opt->angles_per_chunk : rv::bp_default_chunk(p->mode, G.tile);
+            if (G.chunk > rv::bp_max_chunk(p->mode)) G.chunk = rv::bp_max_chunk(p->mode);
+            while (G.chunk > 1 && rv::bp_lds_bytes(p->mode, G.tile, G.chunk) > 160 * 1024) G.chunk /= 2;
+            p->groups.push_back(G);
+        }
+        p->sl[s].group = gidx[p->sl[s].tile];
+        p->groups[p->sl[s].group].slices.push_back(s);
     }
 
     const size_t esz = (p->store == 0) ? 4 : 2;
-    if (!rv::hip_ok(hipMalloc(&p->d_rows, esz * rows_elems), "filtered-sinogram workspace") ||
+    const size_t in_esz = 4;   // stitch buffer sized for float rows (int16 uses half)
+    if (!rv::hip_ok(hipMalloc(&p->d_rows, esz * std::max<size_t>(rows_elems, 1)), "filtered-sinogram workspace") ||
         !rv::hip_ok(hipMalloc(&p->d_scales, sizeof(float) * nang_total), "row scales") ||
         !rv::hip_ok(hipMalloc(&p->d_filts, sizeof(float) * filt_total), "filters") ||
         !rv::hip_ok(hipMalloc(&p->d_ang, sizeof(rv::AngleEntry) * nang_total), "angle table") ||
-        !rv::hip_ok(hipMalloc(&p->d_slices, sizeof(rv::BpSlice) * nslices), "slice table") ||
-        !rv::hip_ok(hipMalloc(&p->d_order, sizeof(int) * order.size()), "work list") ||
-        !rv::hip_ok(hipMalloc(&p->d_counter, 64), "work counter"))
+        !rv::hip_ok(hipMalloc(&p->d_counter, 64 * (p->groups.size() + 1)), "work counters"))
         return fail(p, std::string("plan allocation failed: ") + RecviewLastMessage());
-
-    p->h_slices.resize(nslices);
-    p->cur_out.assign(nslices, nullptr);
-    for (int s = 0; s < nslices; s++) {
-        const SliceState& S = p->sl[s];
-        rv::BpSlice& B = p->h_slices[s];
-        std::memset(&B, 0, sizeof(B));
-        B.rows = (const unsigned char*)p->d_rows + S.rows_off * esz;
-        B.scales = (p->store == 0) ? nullptr : p->d_scales + S.ang_off;
-        B.row_stride = S.g.N;
-        B.row_len = S.g.N;
-        B.nang = S.nang;
-        B.ang = p->d_ang + S.ang_off;
-        B.out = nullptr;
-        B.out_n = S.out_n;
-        B.tiles_x = S.tiles_x;
-        B.tiles_y = S.tiles_y;
-        B.tile_base = S.tile_base;
-        B.accumulate = 0;
-        B.lut_formula = opt->lut_formula ? 1 : 0;
-        B.store = p->store;
-        B.out_scale = 1.f;
+    if (stitch_elems) {
+        if (!rv::hip_ok(hipMalloc(&p->d_stitch, in_esz * stitch_elems), "stitch buffer") ||
+            !rv::hip_ok(hipMalloc(&p->d_pairs, sizeof(int) * pair_total), "pair table") ||
+            !rv::hip_ok(hipMemcpy(p->d_pairs, all_pairs.data(), sizeof(int) * pair_total, hipMemcpyHostToDevice),
+                        "pairs"))
+            return fail(p, std::string("plan allocation failed: ") + RecviewLastMessage());
     }
     if (!rv::hip_ok(hipMemcpy(p->d_filts, hf.data(), sizeof(float) * filt_total, hipMemcpyHostToDevice), "filters") ||
-        !rv::hip_ok(hipMemcpy(p->d_ang, ha.data(), sizeof(rv::AngleEntry) * nang_total, hipMemcpyHostToDevice), "angles") ||
-        !rv::hip_ok(hipMemcpy(p->d_order, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice), "order") ||
-        !rv::hip_ok(hipMemcpy(p->d_slices, p->h_slices.data(), sizeof(rv::BpSlice) * nslices, hipMemcpyHostToDevice), "slices"))
+        !rv::hip_ok(hipMemcpy(p->d_ang, ha.data(), sizeof(rv::AngleEntry) * nang_total, hipMemcpyHostToDevice), "angles"))
         return fail(p, std::string("plan upload failed: ") + RecviewLastMessage());
+
+    for (auto& G : p->groups) {
+        // slice order inside the group: longest projection lists first
+        std::vector<int> ord = G.slices;
+        std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return p->sl[a].nang > p->sl[b].nang; });
+        int base = 0;
+        for (int s : G.slices) {
+            p->sl[s].tile_base = base;
+            base += p->sl[s].tiles_x * p->sl[s].tiles_y;
+        }
+        G.total_items = base;
+        std::vector<int> order;
+        order.reserve(base);
+        for (int s : ord) {
+            const SliceState& S = p->sl[s];
+            for (int t = 0; t < S.tiles_x * S.tiles_y; t++) order.push_back(S.tile_base + t);
+        }
+        G.h.resize(G.slices.size());
+        for (size_t q = 0; q < G.slices.size(); q++) {
+            const SliceState& S = p->sl[G.slices[q]];
+            rv::BpSlice& B = G.h[q];
+            std::memset(&B, 0, sizeof(B));
+            B.rows = (const unsigned char*)p->d_rows + S.rows_off * esz;
+            B.scales = (p->store == 0) ? nullptr : p->d_scales + S.ang_off;
+            B.row_stride = S.g.N;
+            B.row_len = S.g.N;
+            B.nang = S.nang;
+            B.ang = p->d_ang + S.ang_off;
+            B.out = nullptr;
+            B.out_n = S.out_n;
+            B.tiles_x = S.tiles_x;
+            B.tiles_y = S.tiles_y;
+            B.tile_base = S.tile_base;
+            B.accumulate = 0;
+            B.lut_formula = opt->lut_formula ? 1 : 0;
+            B.store = p->store;
+            B.out_scale = 1.f;
+        }
+        if (!rv::hip_ok(hipMalloc(&G.d_slices, sizeof(rv::BpSlice) * G.h.size()), "slice table") ||
+            !rv::hip_ok(hipMalloc(&G.d_order, sizeof(int) * std::max<size_t>(order.size(), 1)), "work list") ||
+            !rv::hip_ok(hipMemcpy(G.d_order, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice), "order") ||
+            !rv::hip_ok(hipMemcpy(G.d_slices, G.h.data(), sizeof(rv::BpSlice) * G.h.size(), hipMemcpyHostToDevice),
+                        "slices"))
+            return fail(p, std::string("plan upload failed: ") + RecviewLastMessage());
+        (void)rv::bp_blocks_per_cu(p->mode, G.tile, G.chunk);     // sets LDS attributes outside capture
+        if (opt->persistent) {
+            const int per_cu = rv::bp_blocks_per_cu(p->mode, G.tile, G.chunk);
+            G.grid = std::max(1, std::min(per_cu * rv::device_cu_count(device), G.total_items));
+        } else {
+            G.grid = G.total_items;
+        }
+    }
+    p->cur_out.assign(nslices, nullptr);
 
     // Everything that allocates or sets kernel attributes happens here, so that
     // execution (and hipGraph capture of it) only launches kernels.
@@ -297,14 +398,6 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
         max_n = std::max(max_n, S.g.n);
     }
     rv::prepare_filter(max_n);
-    (void)rv::bp_blocks_per_cu(p->mode, p->tile, p->chunk);
-    if (opt->persistent) {
-        const int per_cu = rv::bp_blocks_per_cu(p->mode, p->tile, p->chunk);
-        const int g = per_cu * rv::device_cu_count(device);
-        p->grid = std::max(1, std::min(g, p->total_items));
-    } else {
-        p->grid = p->total_items;
-    }
     *out = p;
     return 0;
 }
@@ -365,9 +458,11 @@ int rvPlanExecuteGraph(rv_plan_t p, const void* const* d_sino, void* const* d_ou
 int rvPlanFilteredRows(rv_plan_t p, int s, void** rows, float** scales, int* row_len)
 {
     if (!p || s < 0 || s >= (int)p->sl.size()) return -1;
-    if (rows) *rows = (void*)p->h_slices[s].rows;
-    if (scales) *scales = (float*)p->h_slices[s].scales;
-    if (row_len) *row_len = p->sl[s].g.N;
+    const SliceState& S = p->sl[s];
+    const size_t esz = (p->store == 0) ? 4 : 2;
+    if (rows) *rows = (unsigned char*)p->d_rows + S.rows_off * esz;
+    if (scales) *scales = (p->store == 0) ? nullptr : p->d_scales + S.ang_off;
+    if (row_len) *row_len = S.g.N;
     return 0;
 }
 
@@ -388,8 +483,16 @@ int rvPlanInfo(rv_plan_t p, int s, int* out_n, int* n_fft, int* tiles, int* tile
     if (out_n) *out_n = S.out_n;
     if (n_fft) *n_fft = S.g.n;
     if (tiles) *tiles = S.tiles_x * S.tiles_y;
-    if (tile_w) *tile_w = rv::bp_tile_side(p->tile);
-    if (tile_h) *tile_h = rv::bp_tile_h(p->tile);
+    if (tile_w) *tile_w = rv::bp_tile_side(S.tile);
+    if (tile_h) *tile_h = rv::bp_tile_h(S.tile);
+    return 0;
+}
+
+int rvPlanRows(rv_plan_t p, int s, int* nrows, int* width)
+{
+    if (!p || s < 0 || s >= (int)p->sl.size()) return -1;
+    if (nrows) *nrows = p->sl[s].nang;
+    if (width) *width = p->sl[s].g.W;
     return 0;
 }
 

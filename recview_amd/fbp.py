@@ -37,7 +37,13 @@ class SliceSpec:
 
     def __init__(self, W, fdeg, center, iInterpolation=2, tilt=0.0, filter="hann", cutoff=0.5,
                  order=6.0, sino_dtype="f32", pad="replicate", out_n=0, pix_zoom=1.0, x0=0.0,
-                 y0=0.0):
+                 y0=0.0, tile=-1, offset_ct=False, binc=None):
+        """center follows RECONST_INFO.center (user centre + 0.5); offset_ct: the
+        rows are a 360-degree scan (angles over 0..360) with an off-centre axis,
+        stitched to double width on the GPU; tile = -1 uses the plan's tile."""
+        self.tile = int(tile)
+        self.offset_ct = bool(offset_ct)
+        self.binc = None if binc is None else np.ascontiguousarray(binc, np.int8)
         self.W = int(W)
         self.fdeg = np.ascontiguousarray(fdeg, np.float32)
         self.P = len(self.fdeg)
@@ -61,6 +67,8 @@ class SliceSpec:
         s.filter, s.cutoff, s.order = self.filter, self.cutoff, self.order
         s.sino_dtype, s.pad_mode = self.sino_dtype, self.pad
         s.out_n, s.pix_zoom, s.x0, s.y0 = self.out_n, self.pix_zoom, self.x0, self.y0
+        s.tile, s.offset_ct = self.tile, int(self.offset_ct)
+        s.binc = None if self.binc is None else self.binc.ctypes.data_as(C.c_char_p)
         return s
 
 
@@ -90,6 +98,12 @@ class FbpPlan:
     def out_n(self, s):
         return self.info(s)["out_n"]
 
+    def rows(self, s):
+        """(rows, width) slice s reconstructs, after offset-CT stitching."""
+        n, w = C.c_int(), C.c_int()
+        _lib.check(_lib.lib().rvPlanRows(self._h, s, C.byref(n), C.byref(w)), "rvPlanRows")
+        return n.value, w.value
+
     def execute(self, sinos, outs, stream=None, graph=False):
         fn = _lib.lib().rvPlanExecuteGraph if graph else _lib.lib().rvPlanExecute
         _lib.check(fn(self._h, _ptrs(sinos), _ptrs(outs), _stream_handle(stream)), "rvPlanExecute")
@@ -111,7 +125,7 @@ class FbpPlan:
         """Copy of slice s's filtered rows as a float32 [P, N] device tensor."""
         import torch
         rows, scales, n = self.filtered_rows(s)
-        P = self.slices[s].P
+        P = self.rows(s)[0]
         if self.store == "f32":
             src = _wrap_device(rows, (P, n), torch.float32)
             return src.clone()

@@ -141,6 +141,34 @@ def test_plan_batch_uneven_equals_single():
             assert np.array_equal(batch[i], one)
 
 
+def test_plan_offset_ct_and_mixed_tiles():
+    """An offset-CT slice inside a plan (GPU stitch -> filter -> BP) equals the
+    stitch followed by an ordinary plan on the stitched rows with the centre remap
+    of gazoDoc.cpp:3832-3837; slices with different tiles in one plan equal their
+    single-slice plans."""
+    from recview_amd import offset
+    W, P = 96, 60
+    for c_det in (0.8 * W + 0.3, 0.2 * W + 0.6):
+        fdeg = (np.arange(2 * P + 1) * (180.0 / P)).astype(np.float32)
+        ell = phantom.random_ellipses(5, count=6, axis_range=(0.1, 0.3))
+        s360 = phantom.sinogram(ell, W, fdeg, c_det, scale=1.2 * W).float()
+        _, (img_plan,) = _run([fbp.SliceSpec(W, fdeg, c_det, offset_ct=True)], [s360])
+        stitched, ang = offset.stitch(s360.cuda(), fdeg, c_det - 0.5)
+        c_user = offset.stitched_center(c_det - 0.5, W)
+        _, (img_ref,) = _run([fbp.SliceSpec(2 * W, ang, c_user + 0.5)], [stitched.cpu()])
+        assert img_plan.shape == (2 * W, 2 * W)
+        assert np.array_equal(img_plan, img_ref)
+    specs, sinos = [], []
+    for i, (Wi, Pi, tile) in enumerate([(64, 30, 0), (96, 45, 2), (128, 60, 3), (80, 40, 1)]):
+        s, fd, c = _sino(Wi, Pi, 90 + i)
+        specs.append(fbp.SliceSpec(Wi, fd, c, tile=tile))
+        sinos.append(torch.from_numpy(s))
+    _, batch = _run(specs, sinos, persistent=True)
+    for i in range(4):
+        _, (one,) = _run([specs[i]], [sinos[i]])
+        assert np.array_equal(batch[i], one)
+
+
 def test_plan_graph_replay():
     """hipGraph replays (kernel nodes only) equal the eager result every time,
     with eager work between replays."""

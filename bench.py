@@ -7,6 +7,10 @@ noise at 1e4 photons/ray), reconstructed to 2048 x 2048 float32 with the Hann
 filter and linear detector interpolation.  One step = filter + back-projection
 of the whole stack, sinograms already resident in HBM, images written to HBM.
 
+--workload c5: sparse-view restoration -- 16 slices reconstructed from 225 of
+1800 angles and restored by the BF16-MFMA U-Net, beside full-angle FBP of the
+same slices (rate ratio, PSNR / SSIM).
+
 --workload c3: 256 slices of uneven width {512..4096} and projection count
 {450..3600}, a quarter of them 360-degree offset-axis scans stitched on the GPU
 to double width, per-shape autotuned tiles, one hipGraph per step.
@@ -24,6 +28,8 @@ import os
 import re
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -114,13 +120,148 @@ def cpu_baseline(sino_f32, fdeg, c_det, threads):
     return secs, isa
 
 
+BF16_PEAK_TFLOPS = 2500.0   # MI355X published dense BF16 MFMA peak
+
+
+def run_c5(args):
+    """Sparse-view (every 8th of P angles) FBP + the restoration U-Net on a batch
+    of slices, against full-angle FBP of the same slices."""
+    import torch
+    from recview_amd import fbp, learned, phantom, unet
+
+    dev = torch.device("cuda:0")
+    S, P, W = (args.slices or 16), args.proj, args.width
+    weights = os.path.join(ROOT, "recview_amd", "weights", "unet_sparse8.npz")
+    trained = os.path.exists(weights)
+    net = unet.Unet.load(weights) if trained else unet.Unet(seed=0)
+    scale = net.scale if trained else W / 2.0
+    t0 = time.time()
+    seeds = list(range(3_000_001, 3_000_001 + S))
+    sinos = [learned.slice_sinogram(s, W, P, dev)[0].contiguous() for s in seeds]
+    sparse = [x[::learned.SPARSE].contiguous() for x in sinos]
+    fdeg = phantom.angles_deg(P)
+    c = W / 2.0
+    sp_plan = fbp.FbpPlan([fbp.SliceSpec(W, fdeg[::learned.SPARSE], c, filter=learned.FILTER)] * S,
+                          persistent=True, normalize=True)
+    full_plan = fbp.FbpPlan([fbp.SliceSpec(W, fdeg, c, filter=learned.FILTER)] * S, persistent=True,
+                            normalize=True)
+    N = sp_plan.out_n(0)
+    sp_img = torch.empty(S, N, N, device=dev)
+    restored = torch.empty_like(sp_img)
+    full_img = torch.empty_like(sp_img)
+    sp_views, full_views = list(sp_img.unbind(0)), list(full_img.unbind(0))
+    stream = torch.cuda.current_stream()
+    torch.cuda.synchronize()
+    log(f"inputs generated in {time.time() - t0:.1f}s (trained weights: {trained})")
+
+    def sparse_step():
+        sp_plan.execute(sparse, sp_views, stream=stream, graph=True)
+        net.forward(sp_img, scale, out=restored, stream=stream)
+
+    def full_step():
+        full_plan.execute(sinos, full_views, stream=stream, graph=True)
+
+    def timed(fn, steps):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / steps
+
+    for _ in range(args.warmup):
+        sparse_step()
+        full_step()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    fbp_ms, cnn_ms = [], []
+    for _ in range(max(1, min(3, args.steps))):
+        ev[0].record(stream)
+        sp_plan.execute(sparse, sp_views, stream=stream, graph=True)
+        ev[1].record(stream)
+        net.forward(sp_img, scale, out=restored, stream=stream)
+        ev[2].record(stream)
+        torch.cuda.synchronize()
+        fbp_ms.append(ev[0].elapsed_time(ev[1]))
+        cnn_ms.append(ev[1].elapsed_time(ev[2]))
+    fbp_ms, cnn_ms = sum(fbp_ms) / len(fbp_ms), sum(cnn_ms) / len(cnn_ms)
+    t_sparse = timed(sparse_step, args.steps)
+    t_full = timed(full_step, args.steps)
+
+    yy, xx = torch.meshgrid(torch.arange(N, device=dev), torch.arange(N, device=dev), indexing="ij")
+    mask = ((xx - N / 2 + 0.5) ** 2 + (yy - N / 2 + 0.5) ** 2) <= (0.9 * N / 2) ** 2
+    q = {"psnr_vs_full": {"sparse": [], "restored": []}, "ssim_vs_full": {"sparse": [], "restored": []},
+         "psnr_phantom": {"sparse": [], "restored": [], "full": []}}
+    for i in range(min(S, 4)):
+        truth = learned.slice_image(seeds[i], W, dev)
+        rf = float(full_img[i][mask].max() - full_img[i][mask].min())
+        rt = float(truth[mask].max() - truth[mask].min())
+        for k, img in (("sparse", sp_img[i]), ("restored", restored[i])):
+            q["psnr_vs_full"][k].append(learned.psnr(img, full_img[i], mask, rf))
+            q["ssim_vs_full"][k].append(learned.ssim(img, full_img[i], mask, rf))
+        for k, img in (("sparse", sp_img[i]), ("restored", restored[i]), ("full", full_img[i])):
+            q["psnr_phantom"][k].append(learned.psnr(img, truth, mask, rt))
+    quality = {m: {k: round(sum(v) / len(v), 3) for k, v in d.items()} for m, d in q.items()}
+
+    flops = unet.flops_per_pixel(net.base, net.levels) * S * N * N
+    achieved = flops / (cnn_ms * 1e-3) / 1e12
+    cpu = None
+    if not args.no_cpu:
+        from oracle import unet_ref
+        crop = 128
+        p = net.get_params().astype(np.float64)
+        x = sp_img[0, N // 2 - crop // 2:N // 2 + crop // 2, N // 2 - crop // 2:N // 2 + crop // 2]
+        x = x.double().cpu().numpy()[None]
+        ref = unet_ref.UnetRef(net.base, net.levels)
+        t = time.perf_counter()
+        ref.forward(p, x, scale, rnd=None)
+        secs = (time.perf_counter() - t) * (N * N) / (crop * crop)
+        cpu = {"value": round(1.0 / secs, 6), "unit": "slices/s", "cores": _blas_threads(), "kind": "port",
+               "sample": f"network only: FP64 numpy forward of one {crop}x{crop} crop, scaled to a "
+                         f"{N}x{N} slice"}
+    result = {
+        "metric": "sparse-view FBP + restoration U-Net slices/s at 2048^2 (225 of 1800 angles)",
+        "value": round(S / t_sparse, 3),
+        "unit": "slices/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_sparse * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16",
+        "data": "synthetic",
+        "full_fbp_slices_per_s": round(S / t_full, 3),
+        "rate_vs_full_fbp": round(t_full / t_sparse, 3),
+        "sparse_fbp_ms": round(fbp_ms, 3),
+        "cnn_ms": round(cnn_ms, 3),
+        "quality": quality,
+        "trained": trained,
+        "config": {"workload": f"C5: {S} slices, {P // learned.SPARSE} of {P} proj x {W} det -> {N}^2, "
+                               f"sparse FBP + U-Net (4 levels, 32 base ch, bf16 MFMA)",
+                   "slices": S, "projections": P // learned.SPARSE, "detector": W},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": BF16_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(achieved / BF16_PEAK_TFLOPS, 4), "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(result), flush=True)
+
+
+def _blas_threads():
+    try:
+        from threadpoolctl import threadpool_info
+        return max([i.get("num_threads", 1) for i in threadpool_info()] or [1])
+    except Exception:
+        return os.cpu_count() or 1
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3"])
-    ap.add_argument("--slices", type=int, default=0, help="default 64 (c2) / 256 (c3)")
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c5"])
+    ap.add_argument("--slices", type=int, default=0, help="default 64 (c2) / 256 (c3) / 16 (c5)")
     ap.add_argument("--proj", type=int, default=1800)
     ap.add_argument("--width", type=int, default=2048)
     ap.add_argument("--interp", default="linear", choices=["linear", "lut16_f32", "lut16_i32"])
@@ -138,6 +279,10 @@ def main():
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
               "(slices shard across GPUs with no communication; see the design notes)", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "c5":
+        run_c5(args)
+        return
 
     import torch
     from recview_amd import fbp

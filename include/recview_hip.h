@@ -242,6 +242,9 @@ typedef struct {
     int tile;            /* 0 = autotuned, else tile config id */
     int angles_per_chunk;/* 0 = default */
     int lds_half;        /* FP16/BF16 stores: 0 = FP32 pairs in LDS (default), 1 = FP16 pairs */
+    int normalize;       /* float modes: 1 multiplies each image by 2*pi / (projections used), i.e.
+                          * attenuation per detector sample (the caller-side scale of
+                          * gazoDoc.cpp:4000-4001 without the pixel-width factor); 0 = raw sums */
 } RvOptions;
 
 typedef struct rv_plan_opaque* rv_plan_t;
@@ -282,6 +285,48 @@ int RecviewOffsetCTPairs(const float* fdeg, const char* binc, int nframes, int* 
 int RecviewOffsetCTStitch(const void* d_sino, int is_float, int W, const int* d_pair, int nout,
                           double center, double deltac, int bin, int trim, int out_width,
                           void* d_out, rv_stream_t stream);
+
+/* ------------------------------------------------------------------------
+ * Part 3: learned restoration of sparse-view slices (no counterpart in the
+ * reference).  A residual U-Net (levels, base channels; 4 and 32 by default)
+ * maps an FBP slice reconstructed from 1/8 of the angles to an estimate of the
+ * full-angle FBP slice.  3x3 convolutions run on BF16 MFMA with FP32
+ * accumulation; parameters are one FP32 vector (FP32 master weights, Adam).
+ * Image pointers are device float32 [B][H][W]; H and W must be divisible by
+ * 2^(levels-1).  `scale` multiplies the inputs before the network (outputs are
+ * divided by it).  Entry points return 0 on success, -1 on error
+ * (RecviewLastMessage() says why).
+ * ------------------------------------------------------------------------ */
+typedef struct rv_unet_opaque* rv_unet_t;
+int rvUnetCreate(rv_unet_t* net, int device, int base_channels, int levels, unsigned long long seed);
+long long rvUnetParamCount(rv_unet_t net);
+int rvUnetGetParams(rv_unet_t net, float* host, long long n);
+int rvUnetSetParams(rv_unet_t net, const float* host, long long n);
+/* out = in + net(in * scale) / scale */
+int rvUnetForward(rv_unet_t net, const float* d_in, float* d_out, int B, int H, int W, float scale,
+                  rv_stream_t stream);
+/* Mean squared error of the normalised output against target * scale and its
+ * gradient w.r.t. every parameter (copied to host_grads when non-null; loss
+ * when non-null, which synchronises the stream). */
+int rvUnetLossGrad(rv_unet_t net, const float* d_in, const float* d_target, int B, int H, int W,
+                   float scale, float* loss, float* host_grads, rv_stream_t stream);
+/* rvUnetLossGrad followed by one Adam step (beta 0.9 / 0.999, eps 1e-8). */
+int rvUnetTrainStep(rv_unet_t net, const float* d_in, const float* d_target, int B, int H, int W,
+                    float scale, float lr, float* loss, rv_stream_t stream);
+void rvUnetDestroy(rv_unet_t net);
+/* Single 3x3 convolution on NHWC bf16 tensors (uint16 storage), FP32 weights
+ * W[Cout][3][3][Cin] packed to bf16 fragments per call.  data_gradient = 1
+ * computes dX (Cin channels, into d_out) from dY (Cout channels, d_in) with the
+ * same W; d_mask zeroes outputs whose mask value is not > 0. */
+int rvConv3x3Bf16(const unsigned short* d_in, int in_stride, int in_off, int Cin, const float* d_w,
+                  const float* d_bias, unsigned short* d_out, int out_stride, int out_off, int Cout,
+                  const unsigned short* d_mask, int mask_stride, int mask_off, int relu,
+                  int data_gradient, int B, int H, int W, rv_stream_t stream);
+/* Weight (FP32 [Cout][3][3][Cin]) and bias (may be null) gradients of that
+ * convolution from dY and X. */
+int rvConv3x3WgradBf16(const unsigned short* d_dy, int dy_stride, int dy_off, int Cout,
+                       const unsigned short* d_x, int x_stride, int x_off, int Cin, float* d_dw,
+                       float* d_db, int B, int H, int W, rv_stream_t stream);
 
 /* The filter design of gazoDoc.cpp:3747-3812 (plus 5 = Shepp-Logan, new). */
 void RecviewSetFilter(int ndim, int ifilter, double cutoff, double order, float* out);

@@ -54,7 +54,7 @@ class RvOptions(C.Structure):
     _fields_ = [
         ("interp", C.c_int), ("store", C.c_int), ("lut_formula", C.c_int),
         ("index_rule", C.c_int), ("persistent", C.c_int), ("tile", C.c_int),
-        ("angles_per_chunk", C.c_int), ("lds_half", C.c_int),
+        ("angles_per_chunk", C.c_int), ("lds_half", C.c_int), ("normalize", C.c_int),
     ]
 
 
@@ -71,6 +71,8 @@ EXPORTS = [
     "rvPlanCreate", "rvPlanExecute", "rvPlanExecuteGraph", "rvPlanFilter", "rvPlanBackproject",
     "rvPlanFilteredRows", "rvPlanSetFilteredRows", "rvPlanInfo", "rvPlanRows", "rvPlanDestroy",
     "RecviewSetFilter", "RecviewGeometry", "RecviewOffsetCTPairs", "RecviewOffsetCTStitch",
+    "rvUnetCreate", "rvUnetParamCount", "rvUnetGetParams", "rvUnetSetParams", "rvUnetForward",
+    "rvUnetLossGrad", "rvUnetTrainStep", "rvUnetDestroy", "rvConv3x3Bf16", "rvConv3x3WgradBf16",
 ]
 
 
@@ -127,6 +129,20 @@ def _declare(L):
     L.RecviewOffsetCTPairs.argtypes = [fp, C.c_char_p, C.c_int, ip]
     L.RecviewOffsetCTStitch.argtypes = [vp, C.c_int, C.c_int, vp, C.c_int, C.c_double, C.c_double,
                                         C.c_int, C.c_int, C.c_int, vp, vp]
+    L.rvUnetCreate.argtypes = [C.POINTER(vp), C.c_int, C.c_int, C.c_int, C.c_ulonglong]
+    L.rvUnetParamCount.argtypes = [vp]
+    L.rvUnetParamCount.restype = C.c_longlong
+    L.rvUnetGetParams.argtypes = [vp, fp, C.c_longlong]
+    L.rvUnetSetParams.argtypes = [vp, fp, C.c_longlong]
+    L.rvUnetForward.argtypes = [vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_float, vp]
+    L.rvUnetLossGrad.argtypes = [vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_float, fp, fp, vp]
+    L.rvUnetTrainStep.argtypes = [vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, fp, vp]
+    L.rvUnetDestroy.argtypes = [vp]
+    L.rvUnetDestroy.restype = None
+    L.rvConv3x3Bf16.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, vp, vp, C.c_int, C.c_int, C.c_int,
+                                vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, vp]
+    L.rvConv3x3WgradBf16.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, C.c_int, C.c_int, C.c_int, vp, vp,
+                                     C.c_int, C.c_int, C.c_int, vp]
 
 
 def lib():

@@ -8,8 +8,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "librecview_hip.so")
 
-HOST_SOURCES = ["rv_common.cpp", "rv_compat.cpp", "rv_plan.cpp", "rv_offset.cpp"]
-DEVICE_SOURCES = ["rv_filter.hip", "rv_bp.hip", "rv_stitch.hip"]
+HOST_SOURCES = ["rv_common.cpp", "rv_compat.cpp", "rv_plan.cpp", "rv_offset.cpp", "rv_unet.cpp"]
+DEVICE_SOURCES = ["rv_filter.hip", "rv_bp.hip", "rv_stitch.hip", "rv_unet.hip"]
 
 # -ffp-contract=off: the parity modes reproduce the reference's FP32 operation
 # order; FMAs are written explicitly where they are wanted.  Packed FP32
@@ -25,7 +25,7 @@ def build(verbose=False):
     os.makedirs(os.path.join(HERE, "_build"), exist_ok=True)
     jobs = []
     for src in HOST_SOURCES + DEVICE_SOURCES:
-        obj = os.path.join(HERE, "_build", os.path.splitext(src)[0] + ".o")
+        obj = os.path.join(HERE, "_build", src + ".o")
         cmd = [hipcc] + FLAGS + ["-c"]
         if src.endswith(".hip"):
             cmd += ["-x", "hip"]

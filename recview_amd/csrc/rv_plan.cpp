@@ -372,7 +372,7 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
             B.accumulate = 0;
             B.lut_formula = opt->lut_formula ? 1 : 0;
             B.store = p->store;
-            B.out_scale = 1.f;
+            B.out_scale = opt->normalize ? (float)(6.283185307179586 / S.nang) : 1.f;
         }
         if (!rv::hip_ok(hipMalloc(&G.d_slices, sizeof(rv::BpSlice) * G.h.size()), "slice table") ||
             !rv::hip_ok(hipMalloc(&G.d_order, sizeof(int) * std::max<size_t>(order.size(), 1)), "work list") ||

@@ -1,0 +1,710 @@
+// rv_unet.hip -- CDNA4 kernels of the learned restoration stage.
+//
+// 3x3 convolutions are implicit GEMMs on v_mfma_f32_16x16x32_bf16:
+//   forward / data gradient: M = 16 pixels of an image row, N = 16 output
+//   channels, K = 32 input channels of one tap.  A block owns an 8 x 16 pixel
+//   patch and 16*NF output channels; per 32-channel chunk it stages the 10 x 18
+//   halo (80-byte pixel pitch, conflict-free 16-byte reads) and the chunk's
+//   weights (already in fragment order) in LDS, then each wave runs
+//   9 taps x 2 rows x NF fragments of MFMA.
+//   weight gradient: M = 16 output channels, N = 16 input channels, K = 32
+//   pixels.  Pixels are the contraction, so dY and X are staged transposed
+//   (pixel-contiguous) and X in three column-shifted copies, which keeps every
+//   fragment read an aligned 16-byte LDS load for all nine taps.
+// Operand layouts (checked on gfx950): A[i][k] in lane i + 16*(k/8), element
+// k%8; B[k][j] in lane j + 16*(k/8), element k%8; D[i][j] in lane j + 16*(i/4),
+// element i%4.
+#include "rv_unet.h"
+
+#include <algorithm>
+#include <cmath>
+
+namespace rv {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+static __device__ __forceinline__ uint16_t f2bf(float f) {
+    return __builtin_bit_cast(uint16_t, (__bf16)f);
+}
+static __device__ __forceinline__ float bf2f(uint16_t u) {
+    return __uint_as_float((uint32_t)u << 16);
+}
+static __device__ __forceinline__ uint4 ld16(const uint16_t* p) { return *(const uint4*)p; }
+static __device__ __forceinline__ void st16(uint16_t* p, uint4 v) { *(uint4*)p = v; }
+
+// ------------------------------------------------------------ conv forward
+constexpr int HALO_PITCH = 40;   // bf16 per staged pixel: 32 channels + 8 pad
+
+template <int NF>
+__global__ __launch_bounds__(256) void conv3x3_kernel(ConvArgs a) {
+    __shared__ __attribute__((aligned(16))) uint16_t in_s[180 * HALO_PITCH];
+    __shared__ __attribute__((aligned(16))) uint16_t w_s[NF * 576 * 8];
+    const int tiles_x = (a.W + 15) >> 4;
+    const int x0 = (blockIdx.x % tiles_x) * 16, y0 = (blockIdx.x / tiles_x) * 8;
+    const int bb = blockIdx.z, f0 = blockIdx.y * NF;
+    const int t = threadIdx.x, l = t & 63, w = t >> 6;
+    const int nch = a.Cin >> 5;
+    const uint16_t* in = a.in + (long long)bb * a.H * a.W * a.in_stride + a.in_off;
+    floatx4 acc[2][NF];
+#pragma unroll
+    for (int m = 0; m < 2; m++)
+#pragma unroll
+        for (int f = 0; f < NF; f++) acc[m][f] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+    for (int cc = 0; cc < nch; cc++) {
+        for (int i = t; i < 720; i += 256) {
+            const int hp = i >> 2, g = i & 3;
+            const int hy = hp / 18, hx = hp - hy * 18;
+            const int y = y0 - 1 + hy, x = x0 - 1 + hx;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (y >= 0 && y < a.H && x >= 0 && x < a.W)
+                v = ld16(in + ((long long)y * a.W + x) * a.in_stride + cc * 32 + g * 8);
+            st16(in_s + hp * HALO_PITCH + g * 8, v);
+        }
+        for (int i = t; i < NF * 576; i += 256) {
+            const int f = i / 576, r = i - f * 576;
+            st16(w_s + i * 8, ld16(a.wfrag + ((long long)((f0 + f) * nch + cc) * 576 + r) * 8));
+        }
+        __syncthreads();
+#pragma unroll
+        for (int tap = 0; tap < 9; tap++) {
+            const int ky = tap / 3, kx = tap % 3;
+            bf16x8 av[2];
+#pragma unroll
+            for (int m = 0; m < 2; m++)
+                av[m] = *(const bf16x8*)(in_s + ((2 * w + m + ky) * 18 + (l & 15) + kx) * HALO_PITCH +
+                                         8 * (l >> 4));
+#pragma unroll
+            for (int f = 0; f < NF; f++) {
+                const bf16x8 bv = *(const bf16x8*)(w_s + ((f * 9 + tap) * 64 + l) * 8);
+#pragma unroll
+                for (int m = 0; m < 2; m++)
+                    acc[m][f] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[m], bv, acc[m][f], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int m = 0; m < 2; m++) {
+        const int y = y0 + 2 * w + m;
+        if (y >= a.H) continue;
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+            const int co = (f0 + f) * 16 + (l & 15);
+            const float bias = a.bias ? a.bias[co] : 0.f;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int x = x0 + 4 * (l >> 4) + i;
+                if (x >= a.W) continue;
+                const long long pix = ((long long)bb * a.H + y) * a.W + x;
+                float v = acc[m][f][i] + bias;
+                if (a.relu) v = fmaxf(v, 0.f);
+                if (a.mask && !(bf2f(a.mask[pix * a.mask_stride + a.mask_off + co]) > 0.f)) v = 0.f;
+                a.out[pix * a.out_stride + a.out_off + co] = f2bf(v);
+            }
+        }
+    }
+}
+
+hipError_t launch_conv3x3(const ConvArgs& a, hipStream_t s) {
+    if (a.Cin % 32 || a.Cout % 32 || a.in_stride % 8 || a.in_off % 8 || a.B <= 0 || a.H <= 0 ||
+        a.W <= 0 || a.in_off + a.Cin > a.in_stride || a.out_off + a.Cout > a.out_stride ||
+        (a.mask && a.mask_off + a.Cout > a.mask_stride))
+        return hipErrorInvalidValue;
+    const int tiles = ((a.W + 15) / 16) * ((a.H + 7) / 8);
+    if (a.Cout % 64 == 0)
+        hipLaunchKernelGGL(conv3x3_kernel<4>, dim3(tiles, a.Cout / 64, a.B), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL(conv3x3_kernel<2>, dim3(tiles, a.Cout / 32, a.B), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------ weight gradient
+constexpr int DY_PITCH = 136;   // bf16 per staged output channel: 128 pixels + 8 pad
+constexpr int XT_PITCH = 168;   // bf16 per staged (shift, channel): 10 x 16 pixels + 8 pad
+
+template <int NFW>
+__global__ __launch_bounds__(256) void conv3x3_wgrad_kernel(WgradArgs a) {
+    constexpr int TCO = 16 * NFW;
+    constexpr int CF = NFW == 4 ? 2 : 1;
+    __shared__ __attribute__((aligned(16))) uint16_t dy_s[TCO * DY_PITCH];
+    __shared__ __attribute__((aligned(16))) uint16_t x_s[3 * 32 * XT_PITCH];
+    const int t = threadIdx.x, l = t & 63, w = t >> 6;
+    const int co0 = blockIdx.x * TCO, cc = blockIdx.y, split = blockIdx.z;
+    const int cof = w % NFW, cib = (w / NFW) * CF;
+    const int tiles_x = (a.W + 15) >> 4, tiles_y = (a.H + 7) >> 3;
+    const int per_img = tiles_x * tiles_y, npatch = a.B * per_img;
+    floatx4 acc[9][CF];
+#pragma unroll
+    for (int k = 0; k < 9; k++)
+#pragma unroll
+        for (int c = 0; c < CF; c++) acc[k][c] = floatx4{0.f, 0.f, 0.f, 0.f};
+    float bsum = 0.f;
+    for (int patch = split; patch < npatch; patch += a.nsplit) {
+        const int bb = patch / per_img, r = patch - bb * per_img;
+        const int y0 = (r / tiles_x) * 8, x0 = (r % tiles_x) * 16;
+        for (int i = t; i < 128 * (TCO / 8); i += 256) {
+            const int px = i / (TCO / 8), g = i - px * (TCO / 8);
+            const int y = y0 + (px >> 4), x = x0 + (px & 15);
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (y < a.H && x < a.W)
+                v = ld16(a.dy + (((long long)bb * a.H + y) * a.W + x) * a.dy_stride + a.dy_off + co0 +
+                         8 * g);
+            const uint16_t* e8 = (const uint16_t*)&v;
+#pragma unroll
+            for (int e = 0; e < 8; e++) dy_s[(8 * g + e) * DY_PITCH + px] = e8[e];
+        }
+        for (int i = t; i < 720; i += 256) {
+            const int hp = i >> 2, g = i & 3;
+            const int hy = hp / 18, hx = hp - hy * 18;
+            const int y = y0 - 1 + hy, x = x0 - 1 + hx;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (y >= 0 && y < a.H && x >= 0 && x < a.W)
+                v = ld16(a.x + (((long long)bb * a.H + y) * a.W + x) * a.x_stride + a.x_off + cc * 32 +
+                         8 * g);
+            const uint16_t* e8 = (const uint16_t*)&v;
+#pragma unroll
+            for (int kx = 0; kx < 3; kx++) {
+                const int c = hx - kx;
+                if (c < 0 || c >= 16) continue;
+#pragma unroll
+                for (int e = 0; e < 8; e++) x_s[(kx * 32 + 8 * g + e) * XT_PITCH + hy * 16 + c] = e8[e];
+            }
+        }
+        __syncthreads();
+        if (cc == 0 && t < TCO) {
+            for (int px = 0; px < 128; px += 8) {
+                const uint4 v = ld16(dy_s + t * DY_PITCH + px);
+                const uint16_t* e8 = (const uint16_t*)&v;
+#pragma unroll
+                for (int e = 0; e < 8; e++) bsum += bf2f(e8[e]);
+            }
+        }
+        const int q = l >> 4;
+#pragma unroll
+        for (int kb = 0; kb < 4; kb++) {
+            const bf16x8 av = *(const bf16x8*)(dy_s + (cof * 16 + (l & 15)) * DY_PITCH + kb * 32 + 8 * q);
+            const int rr = 2 * kb + (q >> 1), c0 = 8 * (q & 1);
+#pragma unroll
+            for (int tap = 0; tap < 9; tap++) {
+                const int ky = tap / 3, kx = tap % 3;
+#pragma unroll
+                for (int cf = 0; cf < CF; cf++) {
+                    const int ci = (cib + cf) * 16 + (l & 15);
+                    const bf16x8 bv =
+                        *(const bf16x8*)(x_s + (kx * 32 + ci) * XT_PITCH + (rr + ky) * 16 + c0);
+                    acc[tap][cf] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[tap][cf], 0, 0, 0);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    float* P = a.part + (long long)split * a.Cout * 9 * a.Cin;
+#pragma unroll
+    for (int tap = 0; tap < 9; tap++)
+#pragma unroll
+        for (int cf = 0; cf < CF; cf++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int co = co0 + cof * 16 + 4 * (l >> 4) + e;
+                const int ci = cc * 32 + (cib + cf) * 16 + (l & 15);
+                P[((long long)co * 9 + tap) * a.Cin + ci] = acc[tap][cf][e];
+            }
+    if (cc == 0 && t < TCO) a.partb[(long long)split * a.Cout + co0 + t] = bsum;
+}
+
+int wgrad_splits(int Cin, int Cout, int B, int H, int W) {
+    const int nfw = Cout % 64 == 0 ? 4 : 2;
+    const int tiles = (Cout / (16 * nfw)) * (Cin / 32);
+    const long long npatch = (long long)B * ((H + 7) / 8) * ((W + 15) / 16);
+    long long ns = (1024 + tiles - 1) / tiles;
+    return (int)std::max(1LL, std::min(ns, npatch));
+}
+
+hipError_t launch_conv3x3_wgrad(const WgradArgs& a, hipStream_t s) {
+    if (a.Cin % 32 || a.Cout % 32 || a.dy_stride % 8 || a.dy_off % 8 || a.x_stride % 8 ||
+        a.x_off % 8 || a.nsplit <= 0 || a.dy_off + a.Cout > a.dy_stride || a.x_off + a.Cin > a.x_stride ||
+        a.B <= 0 || a.H <= 0 || a.W <= 0)
+        return hipErrorInvalidValue;
+    if (a.Cout % 64 == 0)
+        hipLaunchKernelGGL(conv3x3_wgrad_kernel<4>, dim3(a.Cout / 64, a.Cin / 32, a.nsplit), dim3(256),
+                           0, s, a);
+    else
+        hipLaunchKernelGGL(conv3x3_wgrad_kernel<2>, dim3(a.Cout / 32, a.Cin / 32, a.nsplit), dim3(256),
+                           0, s, a);
+    return hipGetLastError();
+}
+
+__global__ void sum_splits_kernel(const float* part, int nsplit, int n, float* out) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    float s = 0.f;
+    for (int k = 0; k < nsplit; k++) s += part[(long long)k * n + i];
+    out[i] = s;
+}
+
+hipError_t launch_sum_splits(const float* part, int nsplit, int n, float* out, hipStream_t s) {
+    hipLaunchKernelGGL(sum_splits_kernel, dim3((n + 255) / 256), dim3(256), 0, s, part, nsplit, n, out);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------ weight packing
+__global__ void pack_kernel(const float* w, int Cin, int Cout, int transpose, uint16_t* out,
+                            long long total) {
+    const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= total) return;
+    const int CinP = transpose ? Cout : Cin;   // contraction channels of the launch
+    const int nch = CinP / 32;
+    const int e = idx & 7, l = (idx >> 3) & 63;
+    const long long r = idx >> 9;              // (f*nch + cc)*9 + tap
+    const int tap = (int)(r % 9);
+    const long long fc = r / 9;
+    const int cc = (int)(fc % nch), f = (int)(fc / nch);
+    const int o = f * 16 + (l & 15);           // output channel of the launch
+    const int c = cc * 32 + 8 * (l >> 4) + e;  // contraction channel of the launch
+    const float v = transpose ? w[((long long)c * 9 + (8 - tap)) * Cin + o]
+                              : w[((long long)o * 9 + tap) * Cin + c];
+    out[idx] = f2bf(v);
+}
+
+hipError_t launch_pack_conv_weights(const float* w, int Cin, int Cout, int transpose,
+                                    uint16_t* wfrag, hipStream_t s) {
+    if (Cin % 32 || Cout % 32) return hipErrorInvalidValue;
+    const long long total = (long long)Cin * Cout * 9;
+    hipLaunchKernelGGL(pack_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, w, Cin,
+                       Cout, transpose, wfrag, total);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------ edge layers
+static __device__ __forceinline__ void pix_coords(long long p, int H, int W, int& b, int& y, int& x) {
+    x = (int)(p % W);
+    const long long r = p / W;
+    y = (int)(r % H);
+    b = (int)(r / H);
+}
+
+template <int C>
+__global__ void edge_in_kernel(const float* x, const float* w, const float* b, uint16_t* out,
+                               int os, int oo, int B, int H, int W) {
+    const long long p = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (p >= (long long)B * H * W) return;
+    int bb, y, xx;
+    pix_coords(p, H, W, bb, y, xx);
+    float v[9];
+#pragma unroll
+    for (int tap = 0; tap < 9; tap++) {
+        const int yy = y + tap / 3 - 1, x2 = xx + tap % 3 - 1;
+        v[tap] = (yy >= 0 && yy < H && x2 >= 0 && x2 < W) ? x[((long long)bb * H + yy) * W + x2] : 0.f;
+    }
+    uint16_t* o = out + p * os + oo;
+#pragma unroll
+    for (int g = 0; g < C / 8; g++) {
+        uint4 r;
+        uint16_t* r8 = (uint16_t*)&r;
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const int c = 8 * g + e;
+            float s = b[c];
+#pragma unroll
+            for (int tap = 0; tap < 9; tap++) s = fmaf(w[c * 9 + tap], v[tap], s);
+            r8[e] = f2bf(fmaxf(s, 0.f));
+        }
+        st16(o + 8 * g, r);
+    }
+}
+
+hipError_t launch_edge_in(const float* x, const float* w, const float* b, int C, uint16_t* out,
+                          int out_stride, int out_off, int B, int H, int W, hipStream_t s) {
+    const long long n = (long long)B * H * W;
+    const dim3 g((unsigned)((n + 255) / 256));
+    if (out_stride % 8 || out_off % 8) return hipErrorInvalidValue;
+    if (C == 32)
+        hipLaunchKernelGGL(edge_in_kernel<32>, g, dim3(256), 0, s, x, w, b, out, out_stride, out_off, B, H, W);
+    else if (C == 64)
+        hipLaunchKernelGGL(edge_in_kernel<64>, g, dim3(256), 0, s, x, w, b, out, out_stride, out_off, B, H, W);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+int edge_blocks(int B, int H, int W) {
+    const long long n = (long long)B * H * W;
+    return (int)std::max(1LL, std::min(1024LL, (n + 4095) / 4096));
+}
+
+// Pixels of block k: [k*ppb, min(n, (k+1)*ppb)).  Thread t handles channel
+// group t % (C/8) for every (256 / (C/8))-th pixel.  part[blk] = weights
+// [c][tap] (9*C) then biases [c] (C), the layout of the parameters.
+template <int C>
+__global__ void edge_in_wgrad_kernel(const uint16_t* dy, const float* x, float* part, int B, int H,
+                                     int W) {
+    constexpr int G = C / 8, LANES = 256 / G;
+    __shared__ float red[C * 10];
+    const long long n = (long long)B * H * W;
+    const long long ppb = (n + gridDim.x - 1) / gridDim.x;
+    const long long p0 = blockIdx.x * ppb, p1 = p0 + ppb < n ? p0 + ppb : n;
+    const int t = threadIdx.x, g = t % G;
+    float acc[8][10];
+#pragma unroll
+    for (int e = 0; e < 8; e++)
+#pragma unroll
+        for (int k = 0; k < 10; k++) acc[e][k] = 0.f;
+    for (int i = t; i < C * 10; i += 256) red[i] = 0.f;
+    for (long long p = p0 + t / G; p < p1; p += LANES) {
+        int bb, y, xx;
+        pix_coords(p, H, W, bb, y, xx);
+        const uint4 d = ld16(dy + p * C + 8 * g);
+        const uint16_t* d8 = (const uint16_t*)&d;
+        float v[9];
+#pragma unroll
+        for (int tap = 0; tap < 9; tap++) {
+            const int yy = y + tap / 3 - 1, x2 = xx + tap % 3 - 1;
+            v[tap] = (yy >= 0 && yy < H && x2 >= 0 && x2 < W) ? x[((long long)bb * H + yy) * W + x2] : 0.f;
+        }
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const float de = bf2f(d8[e]);
+#pragma unroll
+            for (int tap = 0; tap < 9; tap++) acc[e][tap] = fmaf(de, v[tap], acc[e][tap]);
+            acc[e][9] += de;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 8; e++)
+#pragma unroll
+        for (int k = 0; k < 10; k++)
+            atomicAdd(&red[k < 9 ? (8 * g + e) * 9 + k : 9 * C + 8 * g + e], acc[e][k]);
+    __syncthreads();
+    for (int i = t; i < C * 10; i += 256) part[(long long)blockIdx.x * C * 10 + i] = red[i];
+}
+
+hipError_t launch_edge_in_wgrad(const uint16_t* dy, int C, const float* x, float* part, int nblk,
+                                int B, int H, int W, hipStream_t s) {
+    if (C == 32)
+        hipLaunchKernelGGL(edge_in_wgrad_kernel<32>, dim3(nblk), dim3(256), 0, s, dy, x, part, B, H, W);
+    else if (C == 64)
+        hipLaunchKernelGGL(edge_in_wgrad_kernel<64>, dim3(nblk), dim3(256), 0, s, dy, x, part, B, H, W);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+template <int C>
+__global__ void edge_out_kernel(const uint16_t* x, const float* w, const float* b, const float* xres,
+                                float* y, int B, int H, int W) {
+    const long long p = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (p >= (long long)B * H * W) return;
+    int bb, yy, xx;
+    pix_coords(p, H, W, bb, yy, xx);
+    float s = b[0];
+#pragma unroll
+    for (int tap = 0; tap < 9; tap++) {
+        const int y2 = yy + tap / 3 - 1, x2 = xx + tap % 3 - 1;
+        if (y2 < 0 || y2 >= H || x2 < 0 || x2 >= W) continue;
+        const uint16_t* q = x + (((long long)bb * H + y2) * W + x2) * C;
+#pragma unroll
+        for (int g = 0; g < C / 8; g++) {
+            const uint4 v = ld16(q + 8 * g);
+            const uint16_t* v8 = (const uint16_t*)&v;
+#pragma unroll
+            for (int e = 0; e < 8; e++) s = fmaf(w[tap * C + 8 * g + e], bf2f(v8[e]), s);
+        }
+    }
+    y[p] = xres[p] + s;
+}
+
+hipError_t launch_edge_out(const uint16_t* x, int C, const float* w, const float* b,
+                           const float* xres, float* y, int B, int H, int W, hipStream_t s) {
+    const long long n = (long long)B * H * W;
+    const dim3 g((unsigned)((n + 255) / 256));
+    if (C == 32)
+        hipLaunchKernelGGL(edge_out_kernel<32>, g, dim3(256), 0, s, x, w, b, xres, y, B, H, W);
+    else if (C == 64)
+        hipLaunchKernelGGL(edge_out_kernel<64>, g, dim3(256), 0, s, x, w, b, xres, y, B, H, W);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+template <int C>
+__global__ void edge_out_bwd_kernel(const float* dy, const float* w, const uint16_t* x, uint16_t* dx,
+                                    int B, int H, int W) {
+    const long long p = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (p >= (long long)B * H * W) return;
+    int bb, yy, xx;
+    pix_coords(p, H, W, bb, yy, xx);
+    float d[9];
+#pragma unroll
+    for (int tap = 0; tap < 9; tap++) {
+        const int y2 = yy - (tap / 3 - 1), x2 = xx - (tap % 3 - 1);
+        d[tap] = (y2 >= 0 && y2 < H && x2 >= 0 && x2 < W) ? dy[((long long)bb * H + y2) * W + x2] : 0.f;
+    }
+#pragma unroll
+    for (int g = 0; g < C / 8; g++) {
+        const uint4 xv = ld16(x + p * C + 8 * g);
+        const uint16_t* x8 = (const uint16_t*)&xv;
+        uint4 r;
+        uint16_t* r8 = (uint16_t*)&r;
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const int c = 8 * g + e;
+            float s = 0.f;
+#pragma unroll
+            for (int tap = 0; tap < 9; tap++) s = fmaf(w[tap * C + c], d[tap], s);
+            r8[e] = f2bf(bf2f(x8[e]) > 0.f ? s : 0.f);
+        }
+        st16(dx + p * C + 8 * g, r);
+    }
+}
+
+hipError_t launch_edge_out_bwd(const float* dy, const float* w, const uint16_t* x, int C,
+                               uint16_t* dx, int B, int H, int W, hipStream_t s) {
+    const long long n = (long long)B * H * W;
+    const dim3 g((unsigned)((n + 255) / 256));
+    if (C == 32)
+        hipLaunchKernelGGL(edge_out_bwd_kernel<32>, g, dim3(256), 0, s, dy, w, x, dx, B, H, W);
+    else if (C == 64)
+        hipLaunchKernelGGL(edge_out_bwd_kernel<64>, g, dim3(256), 0, s, dy, w, x, dx, B, H, W);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+template <int C>
+__global__ void edge_out_wgrad_kernel(const float* dy, const uint16_t* x, float* part, int B, int H,
+                                      int W) {
+    constexpr int G = C / 8, LANES = 256 / G;
+    __shared__ float red[9 * C + 1];
+    const long long n = (long long)B * H * W;
+    const long long ppb = (n + gridDim.x - 1) / gridDim.x;
+    const long long p0 = blockIdx.x * ppb, p1 = p0 + ppb < n ? p0 + ppb : n;
+    const int t = threadIdx.x, g = t % G;
+    float acc[9][8];
+    float bs = 0.f;
+#pragma unroll
+    for (int k = 0; k < 9; k++)
+#pragma unroll
+        for (int e = 0; e < 8; e++) acc[k][e] = 0.f;
+    for (int i = t; i < 9 * C + 1; i += 256) red[i] = 0.f;
+    for (long long p = p0 + t / G; p < p1; p += LANES) {
+        int bb, yy, xx;
+        pix_coords(p, H, W, bb, yy, xx);
+        const float d = dy[p];
+        if (g == 0) bs += d;
+#pragma unroll
+        for (int tap = 0; tap < 9; tap++) {
+            const int y2 = yy + tap / 3 - 1, x2 = xx + tap % 3 - 1;
+            if (y2 < 0 || y2 >= H || x2 < 0 || x2 >= W) continue;
+            const uint4 v = ld16(x + (((long long)bb * H + y2) * W + x2) * C + 8 * g);
+            const uint16_t* v8 = (const uint16_t*)&v;
+#pragma unroll
+            for (int e = 0; e < 8; e++) acc[tap][e] = fmaf(d, bf2f(v8[e]), acc[tap][e]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 9; k++)
+#pragma unroll
+        for (int e = 0; e < 8; e++) atomicAdd(&red[k * C + 8 * g + e], acc[k][e]);
+    if (g == 0) atomicAdd(&red[9 * C], bs);
+    __syncthreads();
+    for (int i = t; i < 9 * C + 1; i += 256) part[(long long)blockIdx.x * (9 * C + 1) + i] = red[i];
+}
+
+hipError_t launch_edge_out_wgrad(const float* dy, const uint16_t* x, int C, float* part, int nblk,
+                                 int B, int H, int W, hipStream_t s) {
+    if (C == 32)
+        hipLaunchKernelGGL(edge_out_wgrad_kernel<32>, dim3(nblk), dim3(256), 0, s, dy, x, part, B, H, W);
+    else if (C == 64)
+        hipLaunchKernelGGL(edge_out_wgrad_kernel<64>, dim3(nblk), dim3(256), 0, s, dy, x, part, B, H, W);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------ pooling / upsampling
+__global__ void avgpool2_kernel(const uint16_t* in, int is, int io, int C, uint16_t* out, int B,
+                                int H, int W) {
+    const int G = C / 8, Ho = H / 2, Wo = W / 2;
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (long long)B * Ho * Wo * G) return;
+    const int g = (int)(i % G);
+    const long long q = i / G;
+    int bb, y, x;
+    pix_coords(q, Ho, Wo, bb, y, x);
+    float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const long long p = ((long long)bb * H + 2 * y + (k >> 1)) * W + 2 * x + (k & 1);
+        const uint4 v = ld16(in + p * is + io + 8 * g);
+        const uint16_t* v8 = (const uint16_t*)&v;
+#pragma unroll
+        for (int e = 0; e < 8; e++) s[e] += bf2f(v8[e]);
+    }
+    uint4 r;
+    uint16_t* r8 = (uint16_t*)&r;
+#pragma unroll
+    for (int e = 0; e < 8; e++) r8[e] = f2bf(0.25f * s[e]);
+    st16(out + q * C + 8 * g, r);
+}
+
+hipError_t launch_avgpool2(const uint16_t* in, int in_stride, int in_off, int C, uint16_t* out,
+                           int B, int H, int W, hipStream_t s) {
+    if (C % 8 || in_stride % 8 || in_off % 8 || H % 2 || W % 2) return hipErrorInvalidValue;
+    const long long n = (long long)B * (H / 2) * (W / 2) * (C / 8);
+    hipLaunchKernelGGL(avgpool2_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, in,
+                       in_stride, in_off, C, out, B, H, W);
+    return hipGetLastError();
+}
+
+__global__ void upsample2_kernel(const uint16_t* in, int C, uint16_t* out, int os, int oo, int B,
+                                 int H, int W) {
+    const int G = C / 8;
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (long long)B * H * W * G) return;
+    const int g = (int)(i % G);
+    const long long p = i / G;
+    int bb, y, x;
+    pix_coords(p, H, W, bb, y, x);
+    const long long q = ((long long)bb * (H / 2) + y / 2) * (W / 2) + x / 2;
+    st16(out + p * os + oo + 8 * g, ld16(in + q * C + 8 * g));
+}
+
+hipError_t launch_upsample2(const uint16_t* in, int C, uint16_t* out, int out_stride, int out_off,
+                            int B, int H, int W, hipStream_t s) {
+    if (C % 8 || out_stride % 8 || out_off % 8 || H % 2 || W % 2) return hipErrorInvalidValue;
+    const long long n = (long long)B * H * W * (C / 8);
+    hipLaunchKernelGGL(upsample2_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, in, C,
+                       out, out_stride, out_off, B, H, W);
+    return hipGetLastError();
+}
+
+// H, W: full-resolution size of g; out is H/2 x W/2 with C channels.
+__global__ void upsample2_bwd_kernel(const uint16_t* g, int gs, int go, int C, uint16_t* out, int B,
+                                     int H, int W) {
+    const int G = C / 8, Ho = H / 2, Wo = W / 2;
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (long long)B * Ho * Wo * G) return;
+    const int gg = (int)(i % G);
+    const long long q = i / G;
+    int bb, y, x;
+    pix_coords(q, Ho, Wo, bb, y, x);
+    float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const long long p = ((long long)bb * H + 2 * y + (k >> 1)) * W + 2 * x + (k & 1);
+        const uint4 v = ld16(g + p * gs + go + 8 * gg);
+        const uint16_t* v8 = (const uint16_t*)&v;
+#pragma unroll
+        for (int e = 0; e < 8; e++) s[e] += bf2f(v8[e]);
+    }
+    uint4 r;
+    uint16_t* r8 = (uint16_t*)&r;
+#pragma unroll
+    for (int e = 0; e < 8; e++) r8[e] = f2bf(s[e]);
+    st16(out + q * C + 8 * gg, r);
+}
+
+hipError_t launch_upsample2_bwd(const uint16_t* g, int g_stride, int g_off, int C, uint16_t* out,
+                                int B, int H, int W, hipStream_t s) {
+    if (C % 8 || g_stride % 8 || g_off % 8 || H % 2 || W % 2) return hipErrorInvalidValue;
+    const long long n = (long long)B * (H / 2) * (W / 2) * (C / 8);
+    hipLaunchKernelGGL(upsample2_bwd_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, g,
+                       g_stride, g_off, C, out, B, H, W);
+    return hipGetLastError();
+}
+
+__global__ void pool_bwd_add_kernel(const uint16_t* skip, int ss, int so, const uint16_t* dp,
+                                    const uint16_t* e, int es, int eo, int C, uint16_t* out, int B,
+                                    int H, int W) {
+    const int G = C / 8;
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (long long)B * H * W * G) return;
+    const int g = (int)(i % G);
+    const long long p = i / G;
+    int bb, y, x;
+    pix_coords(p, H, W, bb, y, x);
+    const long long q = ((long long)bb * (H / 2) + y / 2) * (W / 2) + x / 2;
+    const uint4 sv = ld16(skip + p * ss + so + 8 * g), dv = ld16(dp + q * C + 8 * g),
+                ev = ld16(e + p * es + eo + 8 * g);
+    const uint16_t *s8 = (const uint16_t*)&sv, *d8 = (const uint16_t*)&dv, *e8 = (const uint16_t*)&ev;
+    uint4 r;
+    uint16_t* r8 = (uint16_t*)&r;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const float v = fmaf(0.25f, bf2f(d8[k]), bf2f(s8[k]));
+        r8[k] = f2bf(bf2f(e8[k]) > 0.f ? v : 0.f);
+    }
+    st16(out + p * C + 8 * g, r);
+}
+
+hipError_t launch_pool_bwd_add(const uint16_t* skip, int skip_stride, int skip_off,
+                               const uint16_t* dp, const uint16_t* e, int e_stride, int e_off,
+                               int C, uint16_t* out, int B, int H, int W, hipStream_t s) {
+    if (C % 8 || skip_stride % 8 || skip_off % 8 || e_stride % 8 || e_off % 8 || H % 2 || W % 2)
+        return hipErrorInvalidValue;
+    const long long n = (long long)B * H * W * (C / 8);
+    hipLaunchKernelGGL(pool_bwd_add_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, skip,
+                       skip_stride, skip_off, dp, e, e_stride, e_off, C, out, B, H, W);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------ loss / optimiser
+__global__ void scale_kernel(const float* x, float scale, float* y, long long n) {
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) y[i] = x[i] * scale;
+}
+
+hipError_t launch_scale(const float* x, float scale, float* y, long long n, hipStream_t s) {
+    hipLaunchKernelGGL(scale_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, scale, y, n);
+    return hipGetLastError();
+}
+
+int loss_blocks(long long n) { return (int)std::max(1LL, std::min(1024LL, (n + 2047) / 2048)); }
+
+__global__ void mse_kernel(const float* y, const float* t, float* dy, float* part, long long n) {
+    __shared__ float red[4];
+    float s = 0.f;
+    const float k = 2.f / (float)n;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+        const float d = y[i] - t[i];
+        dy[i] = k * d;
+        s = fmaf(d, d, s);
+    }
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+hipError_t launch_mse(const float* y, const float* t, float* dy, float* part, long long n,
+                      hipStream_t s) {
+    hipLaunchKernelGGL(mse_kernel, dim3(loss_blocks(n)), dim3(256), 0, s, y, t, dy, part, n);
+    return hipGetLastError();
+}
+
+__global__ void adam_kernel(float* p, const float* g, float* m, float* v, long long n, float lr,
+                            float b1, float b2, float eps, float c1, float c2) {
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = fmaf(b1, m[i], (1.f - b1) * gi);
+    const float vi = fmaf(b2, v[i], (1.f - b2) * gi * gi);
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+}
+
+hipError_t launch_adam(float* p, const float* g, float* m, float* v, long long n, float lr,
+                       float b1, float b2, float eps, int step, hipStream_t s) {
+    const float c1 = (float)(1.0 / (1.0 - std::pow((double)b1, step)));
+    const float c2 = (float)(1.0 / (1.0 - std::pow((double)b2, step)));
+    hipLaunchKernelGGL(adam_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, g, m, v, n,
+                       lr, b1, b2, eps, c1, c2);
+    return hipGetLastError();
+}
+
+}  // namespace rv

@@ -141,6 +141,25 @@ def test_plan_batch_uneven_equals_single():
             assert np.array_equal(batch[i], one)
 
 
+def test_plan_normalize_gives_attenuation():
+    """normalize = 1 multiplies each image by 2 pi / P (per slice, also in a
+    batch of different P), so the output is attenuation per detector sample."""
+    cases = [(96, 90), (96, 180)]
+    specs, sinos = [], []
+    for W, P in cases:
+        s, fdeg, c = _sino(W, P, 60)          # one phantom, two projection counts
+        specs.append(fbp.SliceSpec(W, fdeg, c, filter="ramp"))
+        sinos.append(torch.from_numpy(s))
+    _, raw = _run(specs, sinos, persistent=True)
+    _, norm = _run(specs, sinos, persistent=True, normalize=True)
+    for (W, P), r, n in zip(cases, raw, norm):
+        np.testing.assert_allclose(n, r * np.float32(2 * np.pi / P), rtol=1e-6, atol=1e-7 * np.abs(r).max())
+    # the two projection counts agree in physical units (inner disc, offset-free)
+    m = _circle(96, 0.35)
+    a, b = norm[0][m], norm[1][m]
+    assert np.abs(a - b).mean() < 0.1 * np.abs(b).mean()
+
+
 def test_plan_offset_ct_and_mixed_tiles():
     """An offset-CT slice inside a plan (GPU stitch -> filter -> BP) equals the
     stitch followed by an ordinary plan on the stitched rows with the centre remap

@@ -2,11 +2,11 @@
 //
 // 3x3 convolutions are implicit GEMMs on v_mfma_f32_16x16x32_bf16:
 //   forward / data gradient: M = 16 pixels of an image row, N = 16 output
-//   channels, K = 32 input channels of one tap.  A block owns an 8 x 16 pixel
-//   patch and 16*NF output channels; per 32-channel chunk it stages the 10 x 18
+//   channels, K = 32 input channels of one tap.  A block owns a 16 x 16 pixel
+//   patch and 16*NF output channels; per 32-channel chunk it stages the 18 x 18
 //   halo (80-byte pixel pitch, conflict-free 16-byte reads) and the chunk's
 //   weights (already in fragment order) in LDS, then each wave runs
-//   9 taps x 2 rows x NF fragments of MFMA.
+//   9 taps x 4 rows x NF fragments of MFMA.
 //   weight gradient: M = 16 output channels, N = 16 input channels, K = 32
 //   pixels.  Pixels are the contraction, so dY and X are staged transposed
 //   (pixel-contiguous) and X in three column-shifted copies, which keeps every
@@ -34,85 +34,138 @@ static __device__ __forceinline__ uint4 ld16(const uint16_t* p) { return *(const
 static __device__ __forceinline__ void st16(uint16_t* p, uint4 v) { *(uint4*)p = v; }
 
 // ------------------------------------------------------------ conv forward
-constexpr int HALO_PITCH = 40;   // bf16 per staged pixel: 32 channels + 8 pad
+constexpr int HALO_PITCH = 40;          // bf16 per staged pixel: 32 channels + 8 pad
+constexpr int CONV_ROWS = 16;           // patch rows (4 per wave)
+constexpr int HALO_ROWS = CONV_ROWS + 2;
+constexpr int HALO_PIX = HALO_ROWS * 18;
+constexpr int HALO_ITEMS = HALO_PIX * 4;          // 16-byte loads per chunk
+constexpr int HALO_PER = (HALO_ITEMS + 255) / 256;
 
+// One block: a 16 x 16 pixel patch of one image and 16*NF output channels.
+// Wave w owns patch rows 4w..4w+3 (four M fragments) and all NF N fragments,
+// so per tap it reads 4 A and NF B fragments for 4*NF MFMAs.  The next
+// 32-channel chunk is fetched into registers while the current one is
+// multiplied; the epilogue goes through LDS so every store is 16 bytes.
 template <int NF>
-__global__ __launch_bounds__(256) void conv3x3_kernel(ConvArgs a) {
-    __shared__ __attribute__((aligned(16))) uint16_t in_s[180 * HALO_PITCH];
-    __shared__ __attribute__((aligned(16))) uint16_t w_s[NF * 576 * 8];
+__global__ __launch_bounds__(256, 2) void conv3x3_kernel(ConvArgs a) {
+    constexpr int HSZ = HALO_PIX * HALO_PITCH;
+    constexpr int WSZ = NF * 576 * 8;
+    constexpr int TC = 16 * NF;
+    constexpr int OPITCH = TC + 8;
+    constexpr int OSZ = 256 * OPITCH;
+    constexpr int SMEM = HSZ + WSZ > OSZ ? HSZ + WSZ : OSZ;
+    constexpr int W_ITEMS = NF * 576;
+    constexpr int W_PER = (W_ITEMS + 255) / 256;
+    __shared__ __attribute__((aligned(16))) uint16_t smem[SMEM];
+    uint16_t* in_s = smem;
+    uint16_t* w_s = smem + HSZ;
     const int tiles_x = (a.W + 15) >> 4;
-    const int x0 = (blockIdx.x % tiles_x) * 16, y0 = (blockIdx.x / tiles_x) * 8;
+    const int x0 = (blockIdx.x % tiles_x) * 16, y0 = (blockIdx.x / tiles_x) * CONV_ROWS;
     const int bb = blockIdx.z, f0 = blockIdx.y * NF;
     const int t = threadIdx.x, l = t & 63, w = t >> 6;
     const int nch = a.Cin >> 5;
     const uint16_t* in = a.in + (long long)bb * a.H * a.W * a.in_stride + a.in_off;
-    floatx4 acc[2][NF];
+    floatx4 acc[4][NF];
 #pragma unroll
-    for (int m = 0; m < 2; m++)
+    for (int m = 0; m < 4; m++)
 #pragma unroll
         for (int f = 0; f < NF; f++) acc[m][f] = floatx4{0.f, 0.f, 0.f, 0.f};
 
-    for (int cc = 0; cc < nch; cc++) {
-        for (int i = t; i < 720; i += 256) {
+    uint4 hreg[HALO_PER], wreg[W_PER];
+    auto fetch = [&](int cc) {
+#pragma unroll
+        for (int k = 0; k < HALO_PER; k++) {
+            const int i = t + 256 * k;
             const int hp = i >> 2, g = i & 3;
             const int hy = hp / 18, hx = hp - hy * 18;
             const int y = y0 - 1 + hy, x = x0 - 1 + hx;
-            uint4 v = make_uint4(0, 0, 0, 0);
-            if (y >= 0 && y < a.H && x >= 0 && x < a.W)
-                v = ld16(in + ((long long)y * a.W + x) * a.in_stride + cc * 32 + g * 8);
-            st16(in_s + hp * HALO_PITCH + g * 8, v);
+            hreg[k] = make_uint4(0, 0, 0, 0);
+            if (i < HALO_ITEMS && y >= 0 && y < a.H && x >= 0 && x < a.W)
+                hreg[k] = ld16(in + ((long long)y * a.W + x) * a.in_stride + cc * 32 + g * 8);
         }
-        for (int i = t; i < NF * 576; i += 256) {
-            const int f = i / 576, r = i - f * 576;
-            st16(w_s + i * 8, ld16(a.wfrag + ((long long)((f0 + f) * nch + cc) * 576 + r) * 8));
+#pragma unroll
+        for (int k = 0; k < W_PER; k++) {
+            const int i = t + 256 * k;
+            if (i < W_ITEMS) {
+                const int f = i / 576, r = i - f * 576;
+                wreg[k] = ld16(a.wfrag + ((long long)((f0 + f) * nch + cc) * 576 + r) * 8);
+            }
+        }
+    };
+    fetch(0);
+    for (int cc = 0; cc < nch; cc++) {
+#pragma unroll
+        for (int k = 0; k < HALO_PER; k++) {
+            const int i = t + 256 * k;
+            if (i < HALO_ITEMS) st16(in_s + (i >> 2) * HALO_PITCH + (i & 3) * 8, hreg[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < W_PER; k++) {
+            const int i = t + 256 * k;
+            if (i < W_ITEMS) st16(w_s + i * 8, wreg[k]);
         }
         __syncthreads();
+        if (cc + 1 < nch) fetch(cc + 1);
 #pragma unroll
         for (int tap = 0; tap < 9; tap++) {
             const int ky = tap / 3, kx = tap % 3;
-            bf16x8 av[2];
+            bf16x8 av[4];
 #pragma unroll
-            for (int m = 0; m < 2; m++)
-                av[m] = *(const bf16x8*)(in_s + ((2 * w + m + ky) * 18 + (l & 15) + kx) * HALO_PITCH +
+            for (int m = 0; m < 4; m++)
+                av[m] = *(const bf16x8*)(in_s + ((4 * w + m + ky) * 18 + (l & 15) + kx) * HALO_PITCH +
                                          8 * (l >> 4));
 #pragma unroll
             for (int f = 0; f < NF; f++) {
                 const bf16x8 bv = *(const bf16x8*)(w_s + ((f * 9 + tap) * 64 + l) * 8);
 #pragma unroll
-                for (int m = 0; m < 2; m++)
+                for (int m = 0; m < 4; m++)
                     acc[m][f] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[m], bv, acc[m][f], 0, 0, 0);
             }
         }
         __syncthreads();
     }
+    // epilogue: bias, ReLU, bf16 into an LDS tile [256 pixels][TC], then 16-byte stores
+    uint16_t* o_s = smem;
 #pragma unroll
-    for (int m = 0; m < 2; m++) {
-        const int y = y0 + 2 * w + m;
-        if (y >= a.H) continue;
+    for (int f = 0; f < NF; f++) {
+        const int co = f * 16 + (l & 15);
+        const float bias = a.bias ? a.bias[(f0 * 16) + co] : 0.f;
 #pragma unroll
-        for (int f = 0; f < NF; f++) {
-            const int co = (f0 + f) * 16 + (l & 15);
-            const float bias = a.bias ? a.bias[co] : 0.f;
+        for (int m = 0; m < 4; m++)
 #pragma unroll
             for (int i = 0; i < 4; i++) {
-                const int x = x0 + 4 * (l >> 4) + i;
-                if (x >= a.W) continue;
-                const long long pix = ((long long)bb * a.H + y) * a.W + x;
                 float v = acc[m][f][i] + bias;
                 if (a.relu) v = fmaxf(v, 0.f);
-                if (a.mask && !(bf2f(a.mask[pix * a.mask_stride + a.mask_off + co]) > 0.f)) v = 0.f;
-                a.out[pix * a.out_stride + a.out_off + co] = f2bf(v);
+                o_s[((4 * w + m) * 16 + 4 * (l >> 4) + i) * OPITCH + co] = f2bf(v);
             }
+    }
+    __syncthreads();
+    for (int it = t; it < 256 * (TC / 8); it += 256) {
+        const int px = it / (TC / 8), g = it - px * (TC / 8);
+        const int y = y0 + (px >> 4), x = x0 + (px & 15);
+        if (y >= a.H || x >= a.W) continue;
+        const long long pix = ((long long)bb * a.H + y) * a.W + x;
+        const int c = f0 * 16 + 8 * g;
+        uint4 v = ld16(o_s + px * OPITCH + 8 * g);
+        if (a.mask) {
+            const uint4 mv = ld16(a.mask + pix * a.mask_stride + a.mask_off + c);
+            uint16_t* v8 = (uint16_t*)&v;
+            const uint16_t* m8 = (const uint16_t*)&mv;
+#pragma unroll
+            for (int e = 0; e < 8; e++)
+                if (!(bf2f(m8[e]) > 0.f)) v8[e] = 0;
         }
+        st16(a.out + pix * a.out_stride + a.out_off + c, v);
     }
 }
 
 hipError_t launch_conv3x3(const ConvArgs& a, hipStream_t s) {
-    if (a.Cin % 32 || a.Cout % 32 || a.in_stride % 8 || a.in_off % 8 || a.B <= 0 || a.H <= 0 ||
-        a.W <= 0 || a.in_off + a.Cin > a.in_stride || a.out_off + a.Cout > a.out_stride ||
-        (a.mask && a.mask_off + a.Cout > a.mask_stride))
+    if (a.Cin % 32 || a.Cout % 32 || a.in_stride % 8 || a.in_off % 8 || a.out_stride % 8 ||
+        a.out_off % 8 || a.B <= 0 || a.H <= 0 || a.W <= 0 || a.in_off + a.Cin > a.in_stride ||
+        a.out_off + a.Cout > a.out_stride ||
+        (a.mask && (a.mask_stride % 8 || a.mask_off % 8 || a.mask_off + a.Cout > a.mask_stride)))
         return hipErrorInvalidValue;
-    const int tiles = ((a.W + 15) / 16) * ((a.H + 7) / 8);
+    const int tiles = ((a.W + 15) / 16) * ((a.H + CONV_ROWS - 1) / CONV_ROWS);
     if (a.Cout % 64 == 0)
         hipLaunchKernelGGL(conv3x3_kernel<4>, dim3(tiles, a.Cout / 64, a.B), dim3(256), 0, s, a);
     else

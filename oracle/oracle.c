@@ -566,6 +566,34 @@ void orc_bp_rows_linear_f64(const float* px, int P, const float* fdeg, float til
     free(row);
 }
 
+void orc_bp_points_linear_f64(const double* px, int P, const float* fdeg, float tilt,
+                              const orc_geom* g, int iCenterOffset, const double* xs,
+                              const double* ys, int npts, double* out)
+{
+    const int N = g->N;
+    const double h = N / 2;
+    const double off = (double)g->fcenter16 / 16.0 - h + iCenterOffset;
+    int q;
+    #pragma omp parallel for schedule(static)
+    for (q = 0; q < npts; q++) {
+        double acc = 0.0;
+        int i;
+        for (i = 0; i < P; i++) {
+            const double th = (fdeg[i] + tilt) * RV_DEG_TO_RAD;
+            const double u = (xs[q] - h) * cos(th) + (ys[q] - h) * -sin(th) + h + off;
+            const double* row = px + (size_t)i * N;
+            double k;
+            int ik;
+            if (u < 0.0 || u >= N) continue;
+            k = floor(u);
+            ik = (int)k;
+            if (ik <= N - 2) acc += row[ik] + (u - k) * (row[ik + 1] - row[ik]);
+            else acc += row[N - 1];
+        }
+        out[q] = acc;
+    }
+}
+
 void orc_output_stage(int32_t* img, int N, int z, double center, int bin, int* pimin, int* pimax)
 {
     /* gazoDoc.cpp:3993-3999, 4007-4021 */

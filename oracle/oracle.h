@@ -103,6 +103,15 @@ void orc_recon_of64(const float* sino, int P, const float* fdeg, float tilt,
 void orc_bp_rows_linear_f64(const float* px, int P, const float* fdeg, float tilt,
                             const orc_geom* g, int iCenterOffset, double* img);
 
+/* FP64 linear-interpolation back-projection of FP64 rows px[P][N] at arbitrary
+ * points (x, y) of the N x N reconstruction grid (fractional for zoomed
+ * regions): out[q] = sum_i lerp(px_i, u_i), u_i = (x - h) cos th_i - (y - h) sin th_i
+ * + fcenter16/16 + iCenterOffset, the same detector coordinate and edge rules
+ * as orc_bp_linear_f64 (which it equals on integer points). */
+void orc_bp_points_linear_f64(const double* px, int P, const float* fdeg, float tilt,
+                              const orc_geom* g, int iCenterOffset, const double* xs,
+                              const double* ys, int npts, double* out);
+
 /* Caller-side output stage, gazoDoc.cpp:3993-4021: min/max over the square,
  * normalise to 0..65534 inside the circle of radius min(z*c, N-z*c). */
 void orc_output_stage(int32_t* img, int N, int z, double center_user_plus_half, int bin,

@@ -63,6 +63,7 @@ def _declare(L):
     L.orc_filter_rows_f64.argtypes = [fp, C.c_int, C.c_int, gp, fp, dp]
     L.orc_recon_of64.argtypes = [fp, C.c_int, fp, C.c_float, gp, fp, dp]
     L.orc_bp_rows_linear_f64.argtypes = [fp, C.c_int, fp, C.c_float, gp, C.c_int, dp]
+    L.orc_bp_points_linear_f64.argtypes = [dp, C.c_int, fp, C.c_float, gp, C.c_int, dp, dp, C.c_int, dp]
     L.orc_output_stage.argtypes = [ip, C.c_int, C.c_int, C.c_double, C.c_int, ip, ip]
     L.orc_offset_stitch.argtypes = [sp, C.c_int, C.c_int, fp, C.c_char_p, C.c_double, C.c_double,
                                     C.c_int, C.c_int, C.c_int, sp]
@@ -226,6 +227,19 @@ def bp_rows_linear_f64(px, fdeg, g, icoff=0, tilt=0.0):
     lib().orc_bp_rows_linear_f64(_ptr(px, C.c_float), px.shape[0], _ptr(fdeg, C.c_float), tilt,
                                  C.byref(g), int(icoff), _ptr(img, C.c_double))
     return img
+
+
+def bp_points_linear_f64(px, fdeg, g, xs, ys, icoff=0, tilt=0.0):
+    """FP64 linear back-projection of FP64 rows px[P][N] at grid points (xs, ys)."""
+    px = np.ascontiguousarray(px, np.float64)
+    fdeg = _f32(fdeg)
+    xs = np.ascontiguousarray(xs, np.float64).ravel()
+    ys = np.ascontiguousarray(ys, np.float64).ravel()
+    out = np.empty(xs.size, np.float64)
+    lib().orc_bp_points_linear_f64(_ptr(px, C.c_double), px.shape[0], _ptr(fdeg, C.c_float), tilt,
+                                   C.byref(g), int(icoff), _ptr(xs, C.c_double), _ptr(ys, C.c_double),
+                                   xs.size, _ptr(out, C.c_double))
+    return out
 
 
 def fft_f64(x, inverse=False):

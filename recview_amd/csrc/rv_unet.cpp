@@ -199,7 +199,8 @@ oom:
 }
 
 static bool conv_fwd(Unet* u, int li, const uint16_t* in, int in_stride, int in_off, uint16_t* out,
-                     int out_stride, int out_off, int lev, hipStream_t s) {
+                     int out_stride, int out_off, int lev, hipStream_t s, uint16_t* pool_out = nullptr,
+                     const uint16_t* up = nullptr, int Cup = 0) {
     const ULayer& L = u->layers[li];
     ConvArgs a{};
     a.in = in; a.in_stride = in_stride; a.in_off = in_off; a.Cin = L.cin;
@@ -208,6 +209,7 @@ static bool conv_fwd(Unet* u, int li, const uint16_t* in, int in_stride, int in_
     a.mask = nullptr; a.mask_stride = 0; a.mask_off = 0;
     a.relu = 1;
     a.B = u->B; a.H = u->H >> lev; a.W = u->W >> lev;
+    a.up = up; a.Cup = Cup; a.pool_out = pool_out;
     return ok(launch_conv3x3(a, s), "conv3x3");
 }
 
@@ -241,7 +243,10 @@ static bool conv_wgrad(Unet* u, int li, const uint16_t* g, const uint16_t* x, in
            ok(launch_sum_splits(a.partb, a.nsplit, L.cout, u->G + L.b, s), "sum");
 }
 
-static bool forward(Unet* u, const float* in, float scale, hipStream_t s) {
+// materialize: write the upsampled half of every concat (the backward pass
+// reads it); inference lets the decoder convolution read the half-resolution
+// tensor directly instead.
+static bool forward(Unet* u, const float* in, float scale, hipStream_t s, bool materialize) {
     const int Lv = u->levels;
     const long long n0 = (long long)u->B * u->H * u->W;
     if (!ok(launch_scale(in, scale, u->x, n0, s), "scale")) return false;
@@ -253,21 +258,28 @@ static bool forward(Unet* u, const float* in, float scale, hipStream_t s) {
     auto e_ptr = [&](int l) { return l < Lv - 1 ? u->cat[l] : u->bott; };
     auto e_stride = [&](int l) { return l < Lv - 1 ? u->cat_c(l) : u->C(l); };
     auto e_off = [&](int l) { return l < Lv - 1 ? u->C(l + 1) : 0; };
-    if (!conv_fwd(u, u->li_enc(0, 1), u->ea[0], u->C(0), 0, e_ptr(0), e_stride(0), e_off(0), 0, s))
+    // e_l and, below the bottleneck, its 2x2 average pool come out of one launch
+    if (!conv_fwd(u, u->li_enc(0, 1), u->ea[0], u->C(0), 0, e_ptr(0), e_stride(0), e_off(0), 0, s,
+                  Lv > 1 ? u->pool[0] : nullptr))
         return false;
     for (int l = 1; l < Lv; l++) {
-        if (!ok(launch_avgpool2(e_ptr(l - 1), e_stride(l - 1), e_off(l - 1), u->C(l - 1), u->pool[l - 1],
-                                u->B, u->H >> (l - 1), u->W >> (l - 1), s), "pool") ||
-            !conv_fwd(u, u->li_enc(l, 0), u->pool[l - 1], u->C(l - 1), 0, u->ea[l], u->C(l), 0, l, s) ||
-            !conv_fwd(u, u->li_enc(l, 1), u->ea[l], u->C(l), 0, e_ptr(l), e_stride(l), e_off(l), l, s))
+        if (!conv_fwd(u, u->li_enc(l, 0), u->pool[l - 1], u->C(l - 1), 0, u->ea[l], u->C(l), 0, l, s) ||
+            !conv_fwd(u, u->li_enc(l, 1), u->ea[l], u->C(l), 0, e_ptr(l), e_stride(l), e_off(l), l, s,
+                      l < Lv - 1 ? u->pool[l] : nullptr))
             return false;
     }
     for (int l = Lv - 2; l >= 0; l--) {
         const uint16_t* src = l == Lv - 2 ? u->bott : u->d[l + 1];
-        if (!ok(launch_upsample2(src, u->C(l + 1), u->cat[l], u->cat_c(l), 0, u->B, u->H >> l, u->W >> l,
-                                 s), "upsample") ||
-            !conv_fwd(u, u->li_dec(l, 0), u->cat[l], u->cat_c(l), 0, u->da[l], u->C(l), 0, l, s) ||
-            !conv_fwd(u, u->li_dec(l, 1), u->da[l], u->C(l), 0, u->d[l], u->C(l), 0, l, s))
+        if (materialize) {
+            if (!ok(launch_upsample2(src, u->C(l + 1), u->cat[l], u->cat_c(l), 0, u->B, u->H >> l,
+                                     u->W >> l, s), "upsample") ||
+                !conv_fwd(u, u->li_dec(l, 0), u->cat[l], u->cat_c(l), 0, u->da[l], u->C(l), 0, l, s))
+                return false;
+        } else if (!conv_fwd(u, u->li_dec(l, 0), u->cat[l], u->cat_c(l), 0, u->da[l], u->C(l), 0, l, s,
+                             nullptr, src, u->C(l + 1))) {
+            return false;
+        }
+        if (!conv_fwd(u, u->li_dec(l, 1), u->da[l], u->C(l), 0, u->d[l], u->C(l), 0, l, s))
             return false;
     }
     const ULayer& Lo = u->layers[u->li_out()];
@@ -333,7 +345,7 @@ static bool backward(Unet* u, hipStream_t s) {
 static bool loss_and_grad(Unet* u, const float* in, const float* target, float scale, float* loss,
                           hipStream_t s) {
     const long long n0 = (long long)u->B * u->H * u->W;
-    if (!forward(u, in, scale, s) || !ok(launch_scale(target, scale, u->t, n0, s), "scale") ||
+    if (!forward(u, in, scale, s, true) || !ok(launch_scale(target, scale, u->t, n0, s), "scale") ||
         !ok(launch_mse(u->y, u->t, u->dy, u->lpart, n0, s), "loss") || !backward(u, s))
         return false;
     if (loss) {
@@ -442,7 +454,7 @@ int rvUnetForward(rv_unet_t u, const float* d_in, float* d_out, int B, int H, in
     if (!ok(hipSetDevice(u->device), "hipSetDevice") || !ensure_shape(u, B, H, W, u->have_grads && u->B == B &&
                                                                                  u->H == H && u->W == W))
         return -1;
-    return forward(u, d_in, scale, s) &&
+    return forward(u, d_in, scale, s, false) &&
                    ok(launch_scale(u->y, 1.f / scale, d_out, (long long)B * H * W, s), "scale")
                ? 0
                : -1;

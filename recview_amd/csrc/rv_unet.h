@@ -23,6 +23,14 @@ struct ConvArgs {
     const uint16_t* mask; int mask_stride, mask_off;   // nullptr: no mask
     int relu;
     int B, H, W;
+    // Optional fusions (nullptr: off).
+    // up: input channels [0, Cup) are read from this H/2 x W/2 tensor (Cup
+    //     channels, contiguous) at pixel (y/2, x/2) -- nearest 2x upsampling of
+    //     the concat's first part; channels [Cup, Cin) come from `in` as usual.
+    const uint16_t* up; int Cup;
+    // pool_out: also writes the 2x2 average of the (bf16) outputs, H/2 x W/2 x
+    //     Cout contiguous (H and W even).
+    uint16_t* pool_out;
 };
 hipError_t launch_conv3x3(const ConvArgs& a, hipStream_t s);
 

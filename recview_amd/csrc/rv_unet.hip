@@ -65,6 +65,7 @@ __global__ __launch_bounds__(256, 2) void conv3x3_kernel(ConvArgs a) {
     const int t = threadIdx.x, l = t & 63, w = t >> 6;
     const int nch = a.Cin >> 5;
     const uint16_t* in = a.in + (long long)bb * a.H * a.W * a.in_stride + a.in_off;
+    const uint16_t* up = a.up ? a.up + (long long)bb * (a.H >> 1) * (a.W >> 1) * a.Cup : nullptr;
     floatx4 acc[4][NF];
 #pragma unroll
     for (int m = 0; m < 4; m++)
@@ -80,8 +81,13 @@ __global__ __launch_bounds__(256, 2) void conv3x3_kernel(ConvArgs a) {
             const int hy = hp / 18, hx = hp - hy * 18;
             const int y = y0 - 1 + hy, x = x0 - 1 + hx;
             hreg[k] = make_uint4(0, 0, 0, 0);
-            if (i < HALO_ITEMS && y >= 0 && y < a.H && x >= 0 && x < a.W)
-                hreg[k] = ld16(in + ((long long)y * a.W + x) * a.in_stride + cc * 32 + g * 8);
+            if (i < HALO_ITEMS && y >= 0 && y < a.H && x >= 0 && x < a.W) {
+                if (cc * 32 < a.Cup)
+                    hreg[k] = ld16(up + ((long long)(y >> 1) * (a.W >> 1) + (x >> 1)) * a.Cup + cc * 32 +
+                                   g * 8);
+                else
+                    hreg[k] = ld16(in + ((long long)y * a.W + x) * a.in_stride + cc * 32 + g * 8);
+            }
         }
 #pragma unroll
         for (int k = 0; k < W_PER; k++) {
@@ -140,6 +146,30 @@ __global__ __launch_bounds__(256, 2) void conv3x3_kernel(ConvArgs a) {
             }
     }
     __syncthreads();
+    if (a.pool_out) {
+        // 8 x 8 pooled pixels of this patch, from the rounded outputs
+        const int Ho = a.H >> 1, Wo = a.W >> 1;
+        for (int it = t; it < 64 * (TC / 8); it += 256) {
+            const int q = it / (TC / 8), g = it - q * (TC / 8);
+            const int py = q >> 3, pxx = q & 7;
+            const int yo = (y0 >> 1) + py, xo = (x0 >> 1) + pxx;
+            if (yo >= Ho || xo >= Wo) continue;
+            float sum[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int px = (2 * py + (k >> 1)) * 16 + 2 * pxx + (k & 1);
+                const uint4 v = ld16(o_s + px * OPITCH + 8 * g);
+                const uint16_t* v8 = (const uint16_t*)&v;
+#pragma unroll
+                for (int e = 0; e < 8; e++) sum[e] += bf2f(v8[e]);
+            }
+            uint4 r;
+            uint16_t* r8 = (uint16_t*)&r;
+#pragma unroll
+            for (int e = 0; e < 8; e++) r8[e] = f2bf(0.25f * sum[e]);
+            st16(a.pool_out + (((long long)bb * Ho + yo) * Wo + xo) * a.Cout + f0 * 16 + 8 * g, r);
+        }
+    }
     for (int it = t; it < 256 * (TC / 8); it += 256) {
         const int px = it / (TC / 8), g = it - px * (TC / 8);
         const int y = y0 + (px >> 4), x = x0 + (px & 15);
@@ -163,7 +193,9 @@ hipError_t launch_conv3x3(const ConvArgs& a, hipStream_t s) {
     if (a.Cin % 32 || a.Cout % 32 || a.in_stride % 8 || a.in_off % 8 || a.out_stride % 8 ||
         a.out_off % 8 || a.B <= 0 || a.H <= 0 || a.W <= 0 || a.in_off + a.Cin > a.in_stride ||
         a.out_off + a.Cout > a.out_stride ||
-        (a.mask && (a.mask_stride % 8 || a.mask_off % 8 || a.mask_off + a.Cout > a.mask_stride)))
+        (a.mask && (a.mask_stride % 8 || a.mask_off % 8 || a.mask_off + a.Cout > a.mask_stride)) ||
+        (a.up && (a.Cup % 32 || a.Cup > a.Cin || a.H % 2 || a.W % 2)) || (!a.up && a.Cup) ||
+        (a.pool_out && (a.H % 2 || a.W % 2)))
         return hipErrorInvalidValue;
     const int tiles = ((a.W + 15) / 16) * ((a.H + CONV_ROWS - 1) / CONV_ROWS);
     if (a.Cout % 64 == 0)
@@ -338,33 +370,44 @@ static __device__ __forceinline__ void pix_coords(long long p, int H, int W, int
     b = (int)(r / H);
 }
 
+// One block = 256 consecutive pixels; outputs are staged in LDS so the stores
+// are contiguous 16-byte chunks.
 template <int C>
-__global__ void edge_in_kernel(const float* x, const float* w, const float* b, uint16_t* out,
-                               int os, int oo, int B, int H, int W) {
-    const long long p = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (p >= (long long)B * H * W) return;
-    int bb, y, xx;
-    pix_coords(p, H, W, bb, y, xx);
-    float v[9];
+__global__ __launch_bounds__(256) void edge_in_kernel(const float* x, const float* w, const float* b,
+                                                      uint16_t* out, int os, int oo, int B, int H, int W) {
+    constexpr int PITCH = C + 8;
+    __shared__ __attribute__((aligned(16))) uint16_t o_s[256 * PITCH];
+    const long long n = (long long)B * H * W;
+    const long long p0 = (long long)blockIdx.x * 256;
+    const long long p = p0 + threadIdx.x;
+    if (p < n) {
+        int bb, y, xx;
+        pix_coords(p, H, W, bb, y, xx);
+        float v[9];
 #pragma unroll
-    for (int tap = 0; tap < 9; tap++) {
-        const int yy = y + tap / 3 - 1, x2 = xx + tap % 3 - 1;
-        v[tap] = (yy >= 0 && yy < H && x2 >= 0 && x2 < W) ? x[((long long)bb * H + yy) * W + x2] : 0.f;
-    }
-    uint16_t* o = out + p * os + oo;
-#pragma unroll
-    for (int g = 0; g < C / 8; g++) {
-        uint4 r;
-        uint16_t* r8 = (uint16_t*)&r;
-#pragma unroll
-        for (int e = 0; e < 8; e++) {
-            const int c = 8 * g + e;
-            float s = b[c];
-#pragma unroll
-            for (int tap = 0; tap < 9; tap++) s = fmaf(w[c * 9 + tap], v[tap], s);
-            r8[e] = f2bf(fmaxf(s, 0.f));
+        for (int tap = 0; tap < 9; tap++) {
+            const int yy = y + tap / 3 - 1, x2 = xx + tap % 3 - 1;
+            v[tap] = (yy >= 0 && yy < H && x2 >= 0 && x2 < W) ? x[((long long)bb * H + yy) * W + x2] : 0.f;
         }
-        st16(o + 8 * g, r);
+#pragma unroll
+        for (int g = 0; g < C / 8; g++) {
+            uint4 r;
+            uint16_t* r8 = (uint16_t*)&r;
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                const int c = 8 * g + e;
+                float s = b[c];
+#pragma unroll
+                for (int tap = 0; tap < 9; tap++) s = fmaf(w[c * 9 + tap], v[tap], s);
+                r8[e] = f2bf(fmaxf(s, 0.f));
+            }
+            st16(o_s + threadIdx.x * PITCH + 8 * g, r);
+        }
+    }
+    __syncthreads();
+    for (int it = threadIdx.x; it < 256 * (C / 8); it += 256) {
+        const int px = it / (C / 8), g = it - px * (C / 8);
+        if (p0 + px < n) st16(out + (p0 + px) * os + oo + 8 * g, ld16(o_s + px * PITCH + 8 * g));
     }
 }
 
@@ -445,34 +488,48 @@ hipError_t launch_edge_in_wgrad(const uint16_t* dy, int C, const float* x, float
     return hipGetLastError();
 }
 
+// One block = a 16 x 16 pixel patch; the 18 x 18 halo of X is staged in LDS
+// with coalesced 16-byte loads, then each thread reduces 9 taps x C channels.
 template <int C>
-__global__ void edge_out_kernel(const uint16_t* x, const float* w, const float* b, const float* xres,
-                                float* y, int B, int H, int W) {
-    const long long p = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (p >= (long long)B * H * W) return;
-    int bb, yy, xx;
-    pix_coords(p, H, W, bb, yy, xx);
+__global__ __launch_bounds__(256) void edge_out_kernel(const uint16_t* x, const float* w, const float* b,
+                                                       const float* xres, float* y, int B, int H, int W) {
+    constexpr int PITCH = C + 8;
+    constexpr int G = C / 8;
+    __shared__ __attribute__((aligned(16))) uint16_t h_s[324 * PITCH];
+    const int tiles_x = (W + 15) >> 4;
+    const int x0 = (blockIdx.x % tiles_x) * 16, y0 = (blockIdx.x / tiles_x) * 16;
+    const int bb = blockIdx.y;
+    for (int i = threadIdx.x; i < 324 * G; i += 256) {
+        const int hp = i / G, g = i - hp * G;
+        const int hy = hp / 18, hx = hp - hy * 18;
+        const int yy = y0 - 1 + hy, xx = x0 - 1 + hx;
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (yy >= 0 && yy < H && xx >= 0 && xx < W) v = ld16(x + (((long long)bb * H + yy) * W + xx) * C + 8 * g);
+        st16(h_s + hp * PITCH + 8 * g, v);
+    }
+    __syncthreads();
+    const int py = threadIdx.x >> 4, pxl = threadIdx.x & 15;
+    const int yy = y0 + py, xx = x0 + pxl;
+    if (yy >= H || xx >= W) return;
     float s = b[0];
 #pragma unroll
     for (int tap = 0; tap < 9; tap++) {
-        const int y2 = yy + tap / 3 - 1, x2 = xx + tap % 3 - 1;
-        if (y2 < 0 || y2 >= H || x2 < 0 || x2 >= W) continue;
-        const uint16_t* q = x + (((long long)bb * H + y2) * W + x2) * C;
+        const uint16_t* q = h_s + ((py + tap / 3) * 18 + pxl + tap % 3) * PITCH;
 #pragma unroll
-        for (int g = 0; g < C / 8; g++) {
+        for (int g = 0; g < G; g++) {
             const uint4 v = ld16(q + 8 * g);
             const uint16_t* v8 = (const uint16_t*)&v;
 #pragma unroll
             for (int e = 0; e < 8; e++) s = fmaf(w[tap * C + 8 * g + e], bf2f(v8[e]), s);
         }
     }
+    const long long p = ((long long)bb * H + yy) * W + xx;
     y[p] = xres[p] + s;
 }
 
 hipError_t launch_edge_out(const uint16_t* x, int C, const float* w, const float* b,
                            const float* xres, float* y, int B, int H, int W, hipStream_t s) {
-    const long long n = (long long)B * H * W;
-    const dim3 g((unsigned)((n + 255) / 256));
+    const dim3 g((unsigned)(((W + 15) / 16) * ((H + 15) / 16)), (unsigned)B);
     if (C == 32)
         hipLaunchKernelGGL(edge_out_kernel<32>, g, dim3(256), 0, s, x, w, b, xres, y, B, H, W);
     else if (C == 64)

@@ -48,10 +48,16 @@ typedef int RV_BOOL;               /* BOOL of reconstinfo.h:41 */
 #define RECONST_INFO_BUSY 1
 #define RECONST_INFO_ERROR 2
 /* DlgQueue.h:23-26, StdAfx.h:29 */
+#define RQFLAGS_ZERNIKE 2          /* DlgQueue.h:20-22 */
+#define RQFLAGS_DRIFTLIST 4
+#define RQFLAGS_DRIFTPARAMS 8
 #define RQFLAGS_USEONLYEVENFRAMES 16
 #define RQFLAGS_USEONLYODDFRAMES 32
 #define RQFLAGS_SINOGRAMKEPT 128
 #define CGAZODOC_BINC_SAMPLE 1
+#define CGAZODOC_BINC_NOUSE 2      /* StdAfx.h:30-31 */
+#define CGAZODOC_BINC_SKIP 4
+#define READTIF8bit 2              /* general.h:45, RECONST_INFO.iFlag */
 
 /* RECONST_INFO, reconstinfo.h:21-101: same field names and order, so that the
  * caller's assignments compile unchanged.  Device buffers are owned by the
@@ -142,8 +148,13 @@ int GetCudaDeviceName(int iDevice, char* pcName, int iszcName);
 int GetCudaDeviceComputingCapability(int iDevice, int* piMajor, int* piMinor);
 int GetCudaNumberOfCores(int iDevice, int* piCores, int* piProcessors);
 int GetCudaClockRate(int iDevice, int* piClockRate, int* piMemRate);
-/* cudaReconst.h:28: sinogram generation is not provided by this engine; the
- * call logs error 28901 and sets ri->iStatus = RECONST_INFO_ERROR. */
+/* cudaReconst.h:28: sinogram generation on device idev, in place on
+ * ri->iSinogr (maxSinogrLen frames x iMultiplex rows of ixdim uint16 samples,
+ * dark frame last), with the production semantics of GenerateSinogramThread
+ * (general.cpp:2377-2532): incident interpolated in exposure time (fexp), bInc
+ * SAMPLE / NOUSE / SKIP, iStartSino / iStepSino, drift parameters and list,
+ * Zernike mode, 8- / 16-bit pixel floors.  Sets ri->iStatus to IDLE, or to
+ * ERROR where the reference does (no incident frame). */
 void CudaSinogramHost(RECONST_INFO* ri, int idev, bool bReport);
 
 /* ------------------------------------------ Part 1: kernel wrappers -- */

@@ -594,6 +594,92 @@ void orc_bp_points_linear_f64(const double* px, int P, const float* fdeg, float 
     }
 }
 
+int orc_sinogram(uint16_t* frames, int nframes, int mult, int W, const char* binc,
+                 const float* fexp, const float* fdeg, unsigned flags, int drStart, int drEnd,
+                 double drX, double drY, int drOmit, const int* piDrift, int is8bit, int start,
+                 int step, int binning)
+{
+    /* general.cpp:2394-2520; constants.h:39-48, StdAfx.h:29-31, DlgQueue.h:20-22 */
+    const int pixmin = is8bit ? 3 : 20;
+    double* inc = (double*)malloc(sizeof(double) * (size_t)W);
+    int16_t* tmp = (int16_t*)malloc(sizeof(int16_t) * (size_t)W);
+    int i0 = -1, i1 = -1, i, j, k, status = 0;
+    for (i = 0; i < nframes - 1; i++) {
+        if (binc[i] & 2) continue;
+        if (!(binc[i] & 1)) { i0 = i; break; }
+    }
+    for (i = 0; i < nframes - 1; i++) {
+        float t0;
+        double cth, sth;
+        int idrift = 0, sh;
+        if ((binc[i] & 2) && !(binc[i] & 1)) continue;
+        if (!(binc[i] & 1)) { i0 = i; i1 = -1; continue; }
+        if (i1 < 0) {
+            for (j = i; j < nframes - 1; j++) {
+                if (binc[j] & 1) continue;
+                i1 = j;
+                break;
+            }
+            if (i1 < 0) {
+                if (i0 >= 0) i1 = i0;
+                else { status = -1; break; }
+            }
+        }
+        if (i0 < 0 || i1 < 0) { status = -1; break; }
+        if (i % step != start) continue;
+        t0 = fexp[i0];
+        {
+            const float t1 = fexp[i1], ti = fexp[i];
+            if (t0 == t1) t0 = 0;
+            else t0 = (ti - t0) / (t1 - t0);
+        }
+        cth = cos(fdeg[i] * RV_DEG_TO_RAD);
+        sth = sin(fdeg[i] * RV_DEG_TO_RAD);
+        if (drEnd > drStart && (flags & 8)) {
+            if (drStart < i && i < drEnd && drOmit) {
+                for (k = 0; k < mult; k++) memset(frames + ((size_t)i * mult + k) * W, 0, sizeof(uint16_t) * W);
+                continue;
+            }
+            if (i >= drEnd) idrift = (int)(drX * cth - drY * sth);
+            else if (i > drStart) idrift = (int)((drX * cth - drY * sth) * (i - drStart) / (drEnd - drStart));
+        }
+        if (binc[i] & 4) {
+            for (k = 0; k < mult; k++) memset(frames + ((size_t)i * mult + k) * W, 0, sizeof(uint16_t) * W);
+            continue;
+        }
+        if ((flags & 4) && piDrift) idrift -= piDrift[i];
+        sh = idrift / binning;
+        for (k = 0; k < mult; k++) {
+            const uint16_t* dark = frames + ((size_t)(nframes - 1) * mult + k) * W;
+            const uint16_t* a0 = frames + ((size_t)i0 * mult + k) * W;
+            const uint16_t* a1 = frames + ((size_t)i1 * mult + k) * W;
+            uint16_t* row = frames + ((size_t)i * mult + k) * W;
+            for (j = 0; j < W; j++) {
+                /* unsigned short + float -> float arithmetic, stored as double */
+                const float fa = (float)((int)a1[j] - (int)a0[j]) * t0;
+                const float fb = (float)a0[j] + fa;
+                inc[j] = (double)(fb - (float)dark[j]);
+            }
+            for (j = 0; j < W; j++) {
+                int isample;
+                if (inc[j] <= 0) { tmp[j] = 0; continue; }
+                isample = (int)row[j] - (int)dark[j];
+                if (isample < pixmin) { tmp[j] = 0; continue; }
+                if (flags & 2) tmp[j] = (int16_t)(int)(inc[j] * 1000 / isample + 0.5);
+                else tmp[j] = (int16_t)(int)(log(inc[j] / isample) * 3000 + 0.5);
+            }
+            /* drift shift, general.cpp:2501-2513 */
+            for (j = 0; j < W; j++) {
+                const int src = j - sh;
+                row[j] = (uint16_t)((src >= 0 && src < W) ? tmp[src] : 0);
+            }
+        }
+    }
+    free(inc);
+    free(tmp);
+    return status;
+}
+
 void orc_output_stage(int32_t* img, int N, int z, double center, int bin, int* pimin, int* pimax)
 {
     /* gazoDoc.cpp:3993-3999, 4007-4021 */

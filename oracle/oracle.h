@@ -112,6 +112,18 @@ void orc_bp_points_linear_f64(const double* px, int P, const float* fdeg, float 
                               const orc_geom* g, int iCenterOffset, const double* xs,
                               const double* ys, int npts, double* out);
 
+/* Sinogram generation, the production body of GenerateSinogramThread
+ * (general.cpp:2377-2532): frames are uint16 rows [nframes][mult][W] with the
+ * dark frame last; every sample frame i (< nframes-1) taken by
+ * i % step == start becomes int16 -ln(I/I0)*3000 (or the Zernike ratio), with
+ * I0 interpolated in exposure time between the enclosing incident frames, the
+ * pixel-count floor, SKIP / drift-omit zeroing and the drift shift, in place.
+ * Returns 0, or -1 where the reference sets RECONST_INFO_ERROR. */
+int orc_sinogram(uint16_t* frames, int nframes, int mult, int W, const char* binc,
+                 const float* fexp, const float* fdeg, unsigned flags, int drStart, int drEnd,
+                 double drX, double drY, int drOmit, const int* piDrift, int is8bit, int start,
+                 int step, int binning);
+
 /* Caller-side output stage, gazoDoc.cpp:3993-4021: min/max over the square,
  * normalise to 0..65534 inside the circle of radius min(z*c, N-z*c). */
 void orc_output_stage(int32_t* img, int N, int z, double center_user_plus_half, int bin,

@@ -63,6 +63,9 @@ def _declare(L):
     L.orc_filter_rows_f64.argtypes = [fp, C.c_int, C.c_int, gp, fp, dp]
     L.orc_recon_of64.argtypes = [fp, C.c_int, fp, C.c_float, gp, fp, dp]
     L.orc_bp_rows_linear_f64.argtypes = [fp, C.c_int, fp, C.c_float, gp, C.c_int, dp]
+    L.orc_sinogram.argtypes = [_p(C.c_uint16), C.c_int, C.c_int, C.c_int, C.c_char_p, fp, fp, C.c_uint,
+                               C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, ip, C.c_int, C.c_int,
+                               C.c_int, C.c_int]
     L.orc_bp_points_linear_f64.argtypes = [dp, C.c_int, fp, C.c_float, gp, C.c_int, dp, dp, C.c_int, dp]
     L.orc_output_stage.argtypes = [ip, C.c_int, C.c_int, C.c_double, C.c_int, ip, ip]
     L.orc_offset_stitch.argtypes = [sp, C.c_int, C.c_int, fp, C.c_char_p, C.c_double, C.c_double,
@@ -240,6 +243,24 @@ def bp_points_linear_f64(px, fdeg, g, xs, ys, icoff=0, tilt=0.0):
                                    C.byref(g), int(icoff), _ptr(xs, C.c_double), _ptr(ys, C.c_double),
                                    xs.size, _ptr(out, C.c_double))
     return out
+
+
+def sinogram(frames, binc, fexp, fdeg, flags=0, drift=(0, 0, 0.0, 0.0, 0), pidrift=None, is8bit=False,
+             start=0, step=1, binning=1):
+    """Sinogram generation (general.cpp:2377-2532) on uint16 frames [nframes][mult][W]
+    (dark frame last).  Returns (int16 frames after the in-place conversion, status)."""
+    f = np.ascontiguousarray(frames, np.uint16).copy()
+    nframes, mult, W = f.shape
+    binc = np.ascontiguousarray(binc, np.int8)
+    fexp, fdeg = _f32(fexp), _f32(fdeg)
+    drs, dre, dx, dy, om = drift
+    pd = None if pidrift is None else np.ascontiguousarray(pidrift, np.int32)
+    st = lib().orc_sinogram(_ptr(f, C.c_uint16), nframes, mult, W, binc.ctypes.data_as(C.c_char_p),
+                            _ptr(fexp, C.c_float), _ptr(fdeg, C.c_float), int(flags), int(drs),
+                            int(dre), float(dx), float(dy), int(om),
+                            None if pd is None else _ptr(pd, C.c_int32), int(is8bit), int(start), int(step),
+                            int(binning))
+    return f.view(np.int16), st
 
 
 def fft_f64(x, inverse=False):

@@ -9,7 +9,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "librecview_hip.so")
 
 HOST_SOURCES = ["rv_common.cpp", "rv_compat.cpp", "rv_plan.cpp", "rv_offset.cpp", "rv_unet.cpp"]
-DEVICE_SOURCES = ["rv_filter.hip", "rv_bp.hip", "rv_stitch.hip", "rv_unet.hip"]
+DEVICE_SOURCES = ["rv_filter.hip", "rv_bp.hip", "rv_stitch.hip", "rv_unet.hip", "rv_sinogram.hip"]
 
 # -ffp-contract=off: the parity modes reproduce the reference's FP32 operation
 # order; FMAs are written explicitly where they are wanted.  Packed FP32

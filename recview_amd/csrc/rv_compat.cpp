@@ -321,11 +321,113 @@ rv_error_t CUDA_MALLOC_HOST_INT(int** ptr, size_t size)
 
 rv_error_t CUDA_FREE_HOST(void* ptr) { return (rv_error_t)hipHostFree(ptr); }
 
-void CudaSinogramHost(RECONST_INFO* ri, int, bool)
+void CudaSinogramHost(RECONST_INFO* ri, int idev, bool)
 {
-    rv::set_message("sinogram generation is not provided by this engine");
-    rv::log_code(28901);
-    if (ri) ri->iStatus = RECONST_INFO_ERROR;
+    // Production semantics of GenerateSinogramThread (general.cpp:2377-2532): the
+    // frame walk below (incident frames i0 / i1, exposure weight, drift, SKIP)
+    // is sequential and runs on the host; the per-sample work runs in one kernel
+    // launch over every converted row.  Rows are converted in place.
+    if (!ri || !ri->iSinogr || !ri->bInc || !ri->fexp || !ri->fdeg || ri->ixdim <= 0 ||
+        ri->iMultiplex <= 0 || ri->maxSinogrLen < 2 || ri->iStepSino <= 0) {
+        rv::set_message("CudaSinogramHost: incomplete RECONST_INFO");
+        rv::log_code(28901);
+        if (ri) ri->iStatus = RECONST_INFO_ERROR;
+        return;
+    }
+    const int isino = ri->maxSinogrLen, W = ri->ixdim, M = ri->iMultiplex;
+    const int binning = ri->iInterpolation == 0 ? 4 : (ri->iInterpolation == 1 ? 2 : 1);
+    const char* binc = ri->bInc;
+    const float* fexp = ri->fexp;
+    std::vector<rv::SinoJob> jobs;
+    bool error = false;
+    int i0 = -1, i1 = -1;
+    for (int i = 0; i < isino - 1; i++) {
+        if (binc[i] & CGAZODOC_BINC_NOUSE) continue;
+        if (!(binc[i] & CGAZODOC_BINC_SAMPLE)) { i0 = i; break; }
+    }
+    for (int i = 0; i < isino - 1; i++) {
+        if ((binc[i] & CGAZODOC_BINC_NOUSE) && !(binc[i] & CGAZODOC_BINC_SAMPLE)) continue;
+        if (!(binc[i] & CGAZODOC_BINC_SAMPLE)) { i0 = i; i1 = -1; continue; }
+        if (i1 < 0) {
+            for (int j = i; j < isino - 1; j++) {
+                if (binc[j] & CGAZODOC_BINC_SAMPLE) continue;
+                i1 = j;
+                break;
+            }
+            if (i1 < 0) {
+                if (i0 >= 0) i1 = i0;
+                else { error = true; break; }
+            }
+        }
+        if (i0 < 0 || i1 < 0) { error = true; break; }
+        if (i % ri->iStepSino != ri->iStartSino) continue;
+        float t0 = fexp[i0];
+        const float t1 = fexp[i1], ti = fexp[i];
+        if (t0 == t1) t0 = 0;
+        else t0 = (ti - t0) / (t1 - t0);
+        const double th = rv::theta_rad(ri->fdeg[i], 0.f), cth = std::cos(th), sth = std::sin(th);
+        int idrift = 0, action = 1;
+        if (ri->drEnd > ri->drStart && (ri->dReconFlags & RQFLAGS_DRIFTPARAMS)) {
+            if (ri->drStart < i && i < ri->drEnd && ri->drOmit) action = 2;
+            else if (i >= ri->drEnd) idrift = (int)(ri->drX * cth - ri->drY * sth);
+            else if (i > ri->drStart)
+                idrift = (int)((ri->drX * cth - ri->drY * sth) * (i - ri->drStart) / (ri->drEnd - ri->drStart));
+        }
+        if (action == 1 && (binc[i] & CGAZODOC_BINC_SKIP)) action = 2;
+        if (action == 1 && (ri->dReconFlags & RQFLAGS_DRIFTLIST) && ri->piDrift) idrift -= ri->piDrift[i];
+        for (int k = 0; k < M; k++) {
+            rv::SinoJob J{};
+            J.frame = i; J.k = k; J.i0 = i0; J.i1 = i1; J.t0 = t0;
+            J.shift = action == 1 ? idrift / binning : 0;
+            J.action = action;
+            jobs.push_back(J);
+        }
+    }
+    if (!jobs.empty()) {
+        if (!rv::hip_ok(hipSetDevice(idev), "hipSetDevice")) { ri->iStatus = RECONST_INFO_ERROR; return; }
+        const size_t nrow = (size_t)isino * M, frame_bytes = nrow * W * sizeof(uint16_t);
+        const size_t out_bytes = jobs.size() * W * sizeof(int16_t);
+        uint16_t *h_frames = nullptr, *d_frames = nullptr;
+        int16_t *h_out = nullptr, *d_out = nullptr;
+        rv::SinoJob* d_jobs = nullptr;
+        hipStream_t st = nullptr;
+        bool good = rv::hip_ok(hipHostMalloc((void**)&h_frames, frame_bytes, hipHostMallocDefault), "pinned") &&
+                    rv::hip_ok(hipHostMalloc((void**)&h_out, out_bytes, hipHostMallocDefault), "pinned") &&
+                    rv::hip_ok(hipMalloc(&d_frames, frame_bytes), "hipMalloc") &&
+                    rv::hip_ok(hipMalloc(&d_out, out_bytes), "hipMalloc") &&
+                    rv::hip_ok(hipMalloc(&d_jobs, jobs.size() * sizeof(rv::SinoJob)), "hipMalloc") &&
+                    rv::hip_ok(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "stream");
+        if (good) {
+            for (size_t r = 0; r < nrow; r++) std::memcpy(h_frames + r * W, ri->iSinogr[r], W * sizeof(uint16_t));
+            const int pixmin = (ri->iFlag & READTIF8bit) ? 3 : 20;   // SINOGRAM_PIXEL_MIN8bit / 16bit
+            good = rv::hip_ok(hipMemcpyAsync(d_frames, h_frames, frame_bytes, hipMemcpyHostToDevice, st), "H2D") &&
+                   rv::hip_ok(hipMemcpyAsync(d_jobs, jobs.data(), jobs.size() * sizeof(rv::SinoJob),
+                                             hipMemcpyHostToDevice, st), "H2D");
+            if (good) {
+                rv::launch_sinogram(d_frames, isino, M, W, d_jobs, (int)jobs.size(), d_out, pixmin,
+                                    (ri->dReconFlags & RQFLAGS_ZERNIKE) ? 1 : 0, st);
+                good = rv::hip_ok(hipGetLastError(), "sinogram kernel") &&
+                       rv::hip_ok(hipMemcpyAsync(h_out, d_out, out_bytes, hipMemcpyDeviceToHost, st), "D2H") &&
+                       rv::hip_ok(hipStreamSynchronize(st), "sync");
+            }
+            if (good)
+                for (size_t q = 0; q < jobs.size(); q++)
+                    std::memcpy(ri->iSinogr[(size_t)jobs[q].frame * M + jobs[q].k], h_out + q * W,
+                                W * sizeof(int16_t));
+        }
+        if (st) (void)hipStreamDestroy(st);
+        if (d_jobs) (void)hipFree(d_jobs);
+        if (d_out) (void)hipFree(d_out);
+        if (d_frames) (void)hipFree(d_frames);
+        if (h_out) (void)hipHostFree(h_out);
+        if (h_frames) (void)hipHostFree(h_frames);
+        if (!good) {
+            rv::log_code(28901);
+            ri->iStatus = RECONST_INFO_ERROR;
+            return;
+        }
+    }
+    ri->iStatus = error ? RECONST_INFO_ERROR : RECONST_INFO_IDLE;
 }
 
 void CudaDeconv(int ixdim, int iIntpDim, int ndim, float center, float* d_filt, short* d_strip,

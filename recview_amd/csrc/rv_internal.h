@@ -41,6 +41,17 @@ void launch_px2igp(const float* d_px, int* d_igp, int N, int formula, float scal
 void launch_bp_lut_global(int* d_ifp, const int* d_igp, int N, float fcos, float fsin,
                           float foffset, int rule, int block, hipStream_t stream);
 
+// ------------------------------------------------------- sinogram generation
+// One converted row: frame `frame`, multiplex line k, incident frames i0 / i1
+// with weight t0, drift shift (samples), action 1 = convert, 2 = zero.
+struct SinoJob {
+    int frame, k, i0, i1;
+    float t0;
+    int shift, action, pad;
+};
+void launch_sinogram(const uint16_t* frames, int nframes, int mult, int W, const SinoJob* jobs,
+                     int njobs, int16_t* out, int pixmin, int zernike, hipStream_t stream);
+
 // ------------------------------------------------------- back-projection
 enum BpMode { BP_LINEAR = 0, BP_LUT_F32 = 1, BP_LUT_I32 = 2, BP_LINEAR_H = 3 };
 

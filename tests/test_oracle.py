@@ -353,3 +353,94 @@ def test_bp_points_equals_grid_and_reproduces_linear_rows():
     ok = (u >= 0) & (u < g.N - 1)
     ref = ((a[:, None] + b[:, None] * u) * ok).sum(axis=0)
     np.testing.assert_allclose(got, ref, rtol=1e-10, atol=1e-10)
+
+
+# ---------------------------------------------------------------- sinogram generation
+def _frames(W, values, dark=100):
+    """uint16 frames [n][1][W]: given rows plus a dark frame of constant `dark`."""
+    f = [np.asarray(v, np.float64) for v in values] + [np.full(W, dark)]
+    return np.round(np.stack(f)[:, None, :]).astype(np.uint16)
+
+
+def test_sinogram_recovers_line_integrals():
+    """sample = dark + I0 exp(-p) gives round(3000 p) (count rounding <= 1 unit)."""
+    W, dark, I0 = 64, 100, 60000.0
+    p = np.linspace(0.0, 2.9, W)
+    fr = _frames(W, [np.full(W, I0 + dark), I0 * np.exp(-p) + dark, np.full(W, I0 + dark)], dark)
+    out, st = O.sinogram(fr, [0, 1, 0], [0, 1, 2], [0, 0, 0])
+    assert st == 0
+    assert np.abs(out[1, 0] - 3000 * p).max() <= 1.0
+    assert np.array_equal(out[0], fr[0].view(np.int16)) and np.array_equal(out[2], fr[2].view(np.int16))
+
+
+def test_sinogram_incident_interpolated_in_exposure_time():
+    """Incident at t=0 (1000 counts) and t=2 (3000): a sample at t=1 sees 2000."""
+    W, dark = 8, 50
+    fr = _frames(W, [np.full(W, 1000 + dark), np.full(W, 2000 * np.exp(-0.5) + dark), np.full(W, 3000 + dark)], dark)
+    out, _ = O.sinogram(fr, [0, 1, 0], [0.0, 1.0, 2.0], [0, 0, 0])
+    assert np.all(np.abs(out[1, 0] - 1500) <= 1)
+    # equal exposure times: t0 = 0 -> the earlier incident frame alone
+    out2, _ = O.sinogram(fr, [0, 1, 0], [5.0, 1.0, 5.0], [0, 0, 0])
+    assert np.all(np.abs(out2[1, 0] - np.round(3000 * np.log(1000 / (2000 * np.exp(-0.5))))) <= 1)
+
+
+def test_sinogram_floors_zero_cases_and_zernike():
+    W, dark = 6, 100
+    inc = np.array([5000, 5000, 5000, 40, 0, 5000]) + dark        # incident - dark = 0 at j=4
+    smp = np.array([2000, dark + 19, dark + 20, 30, 3000, 1000]) + 0
+    smp[0] += dark
+    smp[5] += dark
+    fr = _frames(W, [inc, smp, inc], dark)
+    out, _ = O.sinogram(fr, [0, 1, 0], [0, 1, 2], [0, 0, 0])
+    row = out[1, 0]
+    assert row[1] == 0                      # sample - dark = 19 < 20
+    assert row[2] == int(np.log(5000 / 20) * 3000 + 0.5)
+    assert row[3] == 0                      # sample below dark
+    assert row[4] == 0                      # incident - dark <= 0
+    out8, _ = O.sinogram(fr, [0, 1, 0], [0, 1, 2], [0, 0, 0], is8bit=True)
+    assert out8[1, 0, 1] == int(np.log(5000 / 19) * 3000 + 0.5)   # 8-bit floor is 3
+    z, _ = O.sinogram(fr, [0, 1, 0], [0, 1, 2], [0, 0, 0], flags=2)
+    assert z[1, 0, 0] == int(5000 * 1000 / 2000 + 0.5) and z[1, 0, 5] == int(5000 * 1000 / 1000 + 0.5)
+
+
+def test_sinogram_frame_selection_skip_drift():
+    W, dark = 16, 10
+    base = np.full(W, 4000.0 + dark)
+    smp = 4000 * np.exp(-np.linspace(0.1, 1.5, W)) + dark
+    fr = _frames(W, [base, smp, smp, smp, smp, base], dark)
+    binc = [0, 1, 1 | 4, 1, 1, 0]
+    ref, _ = O.sinogram(fr, [0, 1, 1, 1, 1, 0], np.arange(6), np.zeros(6))
+    out, _ = O.sinogram(fr, binc, np.arange(6), np.zeros(6))
+    assert (out[2] == 0).all()                              # SKIP
+    assert np.array_equal(out[1], ref[1])
+    # start/step: only i % 2 == 1 converted, the others untouched
+    sel, _ = O.sinogram(fr, [0, 1, 1, 1, 1, 0], np.arange(6), np.zeros(6), start=1, step=2)
+    assert np.array_equal(sel[1], ref[1]) and np.array_equal(sel[3], ref[3])
+    assert np.array_equal(sel[2], fr[2].view(np.int16)) and np.array_equal(sel[4], fr[4].view(np.int16))
+    # drift list: shift by -(piDrift[i]) / binning, truncated toward zero, zero fill
+    pd = np.array([0, 3, -3, 5, -5, 0])
+    dr, _ = O.sinogram(fr, [0, 1, 1, 1, 1, 0], np.arange(6), np.zeros(6), flags=4, pidrift=pd, binning=2)
+    for i, s in ((1, -1), (3, -2), (4, 2)):                 # -3/2 -> -1, -5/2 -> -2, 5/2 -> 2
+        exp = np.zeros(W, np.int16)
+        if s >= 0:
+            exp[s:] = ref[i, 0, :W - s]
+        else:
+            exp[:W + s] = ref[i, 0, -s:]
+        assert np.array_equal(dr[i, 0], exp), i
+    # drift parameters: frames strictly inside (drStart, drEnd) are zeroed with drOmit
+    om, _ = O.sinogram(fr, [0, 1, 1, 1, 1, 0], np.arange(6), np.zeros(6), flags=8, drift=(1, 4, 0.0, 0.0, 1))
+    assert (om[2] == 0).all() and (om[3] == 0).all() and np.array_equal(om[1], ref[1])
+
+
+def test_sinogram_incident_search_quirks():
+    """i0 skips NOUSE incident frames; the forward search for i1 does not."""
+    W, dark = 4, 0
+    a, b, c = np.full(W, 1000.0), np.full(W, 4000.0), np.full(W, 9000.0)
+    smp = np.full(W, 500.0)
+    fr = _frames(W, [a, smp, b, c], dark)
+    # frame 2 (b) is an incident frame marked NOUSE: i1 for frame 1 is still b
+    out, _ = O.sinogram(fr, [0, 1, 2, 0], [0.0, 1.0, 2.0, 3.0], np.zeros(4))
+    assert out[1, 0, 0] == int(np.log(2500 / 500) * 3000 + 0.5)
+    # no incident frame at all -> error status
+    _, st = O.sinogram(_frames(W, [smp, smp], dark), [1, 1], [0, 1], [0, 0])
+    assert st == -1

@@ -339,6 +339,29 @@ int rvConv3x3WgradBf16(const unsigned short* d_dy, int dy_stride, int dy_off, in
                        const unsigned short* d_x, int x_stride, int x_off, int Cin, float* d_dw,
                        float* d_db, int B, int H, int W, rv_stream_t stream);
 
+/* ------------------------------------------------------------------------
+ * Output side of a slice (gazoDoc.cpp:3993-4021, 1986-1989; general.cpp:1313-1470).
+ * ------------------------------------------------------------------------ */
+/* Caller-side normalisation on the device: min / max over the N x N image
+ * (int32, or float with is_float = 1; the maximum starts at 0 as in the
+ * reference), then (int)((float)(v - min) * 65534 / (max - min)) inside the
+ * circle of radius min(z*c/bin, N - z*c/bin) and 0 outside, written as uint16.
+ * center is the value the caller passes at gazoDoc.cpp:4008; zoom and binning
+ * follow iInterpolation.  The minimum and maximum are returned (max bumped by
+ * one when equal to min). */
+int RecviewOutputStage(const void* d_img, int is_float, int N, double center, int iInterpolation,
+                       unsigned short* d_out, double* vmin, double* vmax, rv_stream_t stream);
+/* The ImageDescription string RecView stores in every slice file
+ * (gazoDoc.cpp:1986-1988): sinogram count, pixel width, pixelDiv, pixelBase,
+ * centre, filter id + 1 and two constants.  Returns the length (snprintf). */
+int RecviewImageDescription(char* buf, int bufsize, int numSinogr, double pixelWidth, double pixelDiv,
+                            double pixelBase, double center, int ifilter);
+/* WriteTifMonochrome16 (general.cpp:1313-1470): the same bytes, from a host
+ * int32 image (negative values written as 0).  desc / artist may be NULL (the
+ * reference's defaults).  Returns 0, or 18100 / 18102 as the reference. */
+int RecviewWriteTif16(const char* path, const int* buffer, int height, int width, const char* desc,
+                      const char* artist);
+
 /* The filter design of gazoDoc.cpp:3747-3812 (plus 5 = Shepp-Logan, new). */
 void RecviewSetFilter(int ndim, int ifilter, double cutoff, double order, float* out);
 /* Geometry of cudaReconstHost.cpp:635-658: returns n (ndim) and writes N, zoom. */

@@ -73,6 +73,7 @@ EXPORTS = [
     "RecviewSetFilter", "RecviewGeometry", "RecviewOffsetCTPairs", "RecviewOffsetCTStitch",
     "rvUnetCreate", "rvUnetParamCount", "rvUnetGetParams", "rvUnetSetParams", "rvUnetForward",
     "rvUnetLossGrad", "rvUnetTrainStep", "rvUnetDestroy", "rvConv3x3Bf16", "rvConv3x3WgradBf16",
+    "RecviewOutputStage", "RecviewImageDescription", "RecviewWriteTif16",
 ]
 
 
@@ -129,6 +130,11 @@ def _declare(L):
     L.RecviewOffsetCTPairs.argtypes = [fp, C.c_char_p, C.c_int, ip]
     L.RecviewOffsetCTStitch.argtypes = [vp, C.c_int, C.c_int, vp, C.c_int, C.c_double, C.c_double,
                                         C.c_int, C.c_int, C.c_int, vp, vp]
+    L.RecviewOutputStage.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_int, vp, C.POINTER(C.c_double),
+                                     C.POINTER(C.c_double), vp]
+    L.RecviewImageDescription.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                          C.c_double, C.c_int]
+    L.RecviewWriteTif16.argtypes = [C.c_char_p, ip, C.c_int, C.c_int, C.c_char_p, C.c_char_p]
     L.rvUnetCreate.argtypes = [C.POINTER(vp), C.c_int, C.c_int, C.c_int, C.c_ulonglong]
     L.rvUnetParamCount.argtypes = [vp]
     L.rvUnetParamCount.restype = C.c_longlong

@@ -8,8 +8,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "librecview_hip.so")
 
-HOST_SOURCES = ["rv_common.cpp", "rv_compat.cpp", "rv_plan.cpp", "rv_offset.cpp", "rv_unet.cpp"]
-DEVICE_SOURCES = ["rv_filter.hip", "rv_bp.hip", "rv_stitch.hip", "rv_unet.hip", "rv_sinogram.hip"]
+HOST_SOURCES = ["rv_common.cpp", "rv_compat.cpp", "rv_plan.cpp", "rv_offset.cpp", "rv_unet.cpp", "rv_output.cpp"]
+DEVICE_SOURCES = ["rv_filter.hip", "rv_bp.hip", "rv_stitch.hip", "rv_unet.hip", "rv_sinogram.hip", "rv_output.hip"]
 
 # -ffp-contract=off: the parity modes reproduce the reference's FP32 operation
 # order; FMAs are written explicitly where they are wanted.  Packed FP32

@@ -52,6 +52,13 @@ struct SinoJob {
 void launch_sinogram(const uint16_t* frames, int nframes, int mult, int W, const SinoJob* jobs,
                      int njobs, int16_t* out, int pixmin, int zernike, hipStream_t stream);
 
+// ------------------------------------------------------- output stage
+// Min/max + normalisation to 0..65534 inside the circle of squared radius
+// irad2 (gazoDoc.cpp:3993-4021).  scratch: >= 2050 elements of the image type;
+// its last two hold (min, max) afterwards.
+void launch_output_stage(const void* img, int is_float, int N, int irad2, uint16_t* out, void* scratch,
+                         hipStream_t s);
+
 // ------------------------------------------------------- back-projection
 enum BpMode { BP_LINEAR = 0, BP_LUT_F32 = 1, BP_LUT_I32 = 2, BP_LINEAR_H = 3 };
 

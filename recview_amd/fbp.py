@@ -186,3 +186,24 @@ def reconstruct(sino, fdeg, center, out=None, stream=None, **kw):
     torch.cuda.synchronize()
     plan.close()
     return out
+
+
+def centre_sweep(sino, fdeg, centres, stream=None, **kw):
+    """Axis-scan images: one sinogram reconstructed at every centre in `centres`
+    (RECONST_INFO.center convention) in one persistent launch, the batched form of
+    the reference's kept-sinogram centre sweep (gazoDoc.cpp:1994-2025,
+    cudaReconstHost.cpp:775-777).  Returns a [K, N, N] float32 device tensor."""
+    import torch
+    dtype = "i16" if sino.dtype == torch.int16 else "f32"
+    plan_kw = {k: kw.pop(k) for k in list(kw) if k in ("interp", "store", "persistent", "tile",
+                                                       "angles_per_chunk", "lds_half", "normalize")}
+    plan_kw.setdefault("persistent", True)
+    specs = [SliceSpec(sino.shape[1], fdeg, float(c), sino_dtype=dtype, **kw) for c in centres]
+    plan = FbpPlan(specs, **plan_kw)
+    n = plan.out_n(0)
+    out = torch.empty(len(specs), n, n, device=sino.device)
+    src = sino.contiguous()
+    plan.execute([src] * len(specs), list(out.unbind(0)), stream=stream)
+    torch.cuda.synchronize()
+    plan.close()
+    return out

@@ -255,3 +255,20 @@ def test_plan_zoomed_roi():
     _, (roi2,) = _run([fbp.SliceSpec(W, fdeg, c, out_n=64, pix_zoom=2.0, x0=x0, y0=y0)],
                       [torch.from_numpy(s)])
     assert np.abs(roi2[::2, ::2] - full[y0:y0 + 32, x0:x0 + 32]).max() <= 1e-5 * rng
+
+
+def test_centre_sweep_matches_single_and_finds_axis():
+    """Batched axis scan equals one plan per centre; the total variation of
+    the image is smallest at the true rotation axis (the usual criterion)."""
+    W, P = 192, 180
+    s, fdeg, c = _sino(W, P, 321, c_off=2.6)
+    sino = torch.from_numpy(s).cuda()
+    centres = [c + d for d in (-2.0, -1.0, -0.5, 0.0, 0.5, 1.0, 2.0)]
+    sweep = fbp.centre_sweep(sino, fdeg, centres).cpu().numpy()
+    for k in (0, 3, 6):
+        _, (one,) = _run([fbp.SliceSpec(W, fdeg, centres[k])], [torch.from_numpy(s)], persistent=True)
+        assert np.array_equal(sweep[k], one)
+    m = _circle(W, 0.4)
+    tv = [np.abs(np.diff(img, axis=0))[m[1:]].sum() + np.abs(np.diff(img, axis=1))[m[:, 1:]].sum()
+          for img in sweep]
+    assert int(np.argmin(tv)) == 3

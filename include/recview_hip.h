@@ -256,6 +256,9 @@ typedef struct {
     int normalize;       /* float modes: 1 multiplies each image by 2*pi / (projections used), i.e.
                           * attenuation per detector sample (the caller-side scale of
                           * gazoDoc.cpp:4000-4001 without the pixel-width factor); 0 = raw sums */
+    int exact_filter;    /* LINEAR modes: 1 keeps CFft's operation order (filtered rows bit-identical to
+                          * the reference CPU filter); 0 (default) filters two rows per transform with
+                          * the symmetrised filter, the same operator at half the cost */
 } RvOptions;
 
 typedef struct rv_plan_opaque* rv_plan_t;

@@ -130,21 +130,66 @@ __device__ __forceinline__ void fft_first_stage(float2* buf, int half, int T, in
     __syncthreads();
 }
 
-template <int IN_KIND>
+// Per-row power-of-two scale so that |v| / scale <= 2^14, then the 16-bit store
+// of the N cropped values held in outf (LDS).
+__device__ void store_lp_row(const FilterJob& J, const float* outf, int rout, float* red)
+{
+    const int T = blockDim.x, tid = threadIdx.x;
+    __syncthreads();
+    float mx = 0.f;
+    for (int c = tid; c < J.N; c += T) mx = fmaxf(mx, fabsf(outf[c]));
+    for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
+    if ((tid & 63) == 0) red[tid >> 6] = mx;
+    __syncthreads();
+    if (tid == 0) {
+        float m2 = 0.f;
+        for (int w = 0; w < (T + 63) / 64; w++) m2 = fmaxf(m2, red[w]);
+        int e = 0;
+        if (m2 > 0.f) { frexpf(m2 / 16384.f, &e); }
+        red[16] = ldexpf(1.f, e);
+    }
+    __syncthreads();
+    const float sc = red[16];
+    const float inv = 1.f / sc;            // power of two: exact
+    if (tid == 0) J.out_scale[rout] = sc;
+    if (J.store == 1) {
+        __half* o = (__half*)J.out_lp + (long long)rout * J.out_stride;
+        for (int c = tid; c < J.N; c += T) o[c] = __float2half_rn(outf[c] * inv);
+    } else {
+        __hip_bfloat16* o = (__hip_bfloat16*)J.out_lp + (long long)rout * J.out_stride;
+        for (int c = tid; c < J.N; c += T) o[c] = __float2bfloat16(outf[c] * inv);
+    }
+    __syncthreads();
+}
+
+// PAIR: two real rows share one complex transform (row a in the real part,
+// row b in the imaginary part).  J.filt then holds the Hermitian-symmetrised
+// filter (H[k] + H[n-k]) / 2, which is real and even, so the transform keeps
+// the two rows apart and applies the same operator as the one-sided filter
+// followed by taking the real part: half the work, CFft's network but not its
+// rounding (used where parity is to tolerance, not bit-exact).
+template <int IN_KIND, bool PAIR>
 __global__ void __launch_bounds__(1024) filter_rows_kernel(FilterJob J)
 {
     extern __shared__ float2 buf[];
-    const int r = blockIdx.x;
-    const int rin = J.row_in ? J.row_in[r] : r;
-    const int rout = J.row_out ? J.row_out[r] : r;
+    const int ra = PAIR ? 2 * (int)blockIdx.x : (int)blockIdx.x;
+    const bool hasb = PAIR && (ra + 1 < J.nrows);
+    const int rin = J.row_in ? J.row_in[ra] : ra;
+    const int rout = J.row_out ? J.row_out[ra] : ra;
+    const int rin_b = hasb ? (J.row_in ? J.row_in[ra + 1] : ra + 1) : 0;
+    const int rout_b = hasb ? (J.row_out ? J.row_out[ra + 1] : ra + 1) : 0;
     const int n = J.n, half = n >> 1, np = J.np, T = blockDim.x, tid = threadIdx.x;
     const float2* tw = (const float2*)J.tw;
     const void* row = (IN_KIND == 0)
         ? (const void*)((const int16_t*)J.sino + (long long)rin * J.in_stride)
         : (const void*)((const float*)J.sino + (long long)rin * J.in_stride);
+    const void* row_b = (IN_KIND == 0)
+        ? (const void*)((const int16_t*)J.sino + (long long)rin_b * J.in_stride)
+        : (const void*)((const float*)J.sino + (long long)rin_b * J.in_stride);
 
     // Row preparation straight into LDS.
-    for (int m = tid; m < n; m += T) buf[m] = make_float2(prep_value<IN_KIND>(J, row, m), 0.f);
+    for (int m = tid; m < n; m += T)
+        buf[m] = make_float2(prep_value<IN_KIND>(J, row, m), hasb ? prep_value<IN_KIND>(J, row_b, m) : 0.f);
     __syncthreads();
 
     // ---- FFT1Rev (cfft.cpp:431-434 -> FFTrev with nmin < 0)
@@ -189,7 +234,8 @@ __global__ void __launch_bounds__(1024) filter_rows_kernel(FilterJob J)
         }
         __syncthreads();
         const float a = 1.f / (float)n;
-        float* outf = (float*)buf;                 // reuse LDS as the cropped real row
+        float* outf = (float*)buf;                 // reuse LDS as the cropped real row(s)
+        float* outf_b = outf + J.N;
         const bool lp = (J.store != 0);
 #pragma unroll
         for (int i = 0; i < kMaxBfly; i++) {
@@ -201,45 +247,33 @@ __global__ void __launch_bounds__(1024) filter_rows_kernel(FilterJob J)
                 // the caller keeps p[c + crop].re (cudaReconstHost.cpp:756).
                 const float v0 = (A[i].x - s.x) * a;   // f[k]
                 const float v1 = (A[i].x + s.x) * a;   // f[k + half]
+                const float w0 = (A[i].y - s.y) * a;   // imaginary parts: row b (PAIR)
+                const float w1 = (A[i].y + s.y) * a;
                 const int c0 = n - 1 - k - J.crop;
                 const int c1 = n - 1 - (k + half) - J.crop;
                 if (!lp) {
                     float* o = J.out + (long long)rout * J.out_stride;
                     if (c0 >= 0 && c0 < J.N) o[c0] = v0;
                     if (c1 >= 0 && c1 < J.N) o[c1] = v1;
+                    if (hasb) {
+                        float* ob = J.out + (long long)rout_b * J.out_stride;
+                        if (c0 >= 0 && c0 < J.N) ob[c0] = w0;
+                        if (c1 >= 0 && c1 < J.N) ob[c1] = w1;
+                    }
                 } else {
                     if (c0 >= 0 && c0 < J.N) outf[c0] = v0;
                     if (c1 >= 0 && c1 < J.N) outf[c1] = v1;
+                    if (hasb) {
+                        if (c0 >= 0 && c0 < J.N) outf_b[c0] = w0;
+                        if (c1 >= 0 && c1 < J.N) outf_b[c1] = w1;
+                    }
                 }
             }
         }
         if (lp) {
-            // Per-row power-of-two scale so that |v| / scale <= 2^14, then 16-bit store.
             __shared__ float red[32];
-            __syncthreads();
-            float mx = 0.f;
-            for (int c = tid; c < J.N; c += T) mx = fmaxf(mx, fabsf(outf[c]));
-            for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
-            if ((tid & 63) == 0) red[tid >> 6] = mx;
-            __syncthreads();
-            if (tid == 0) {
-                float m2 = 0.f;
-                for (int w = 0; w < (T + 63) / 64; w++) m2 = fmaxf(m2, red[w]);
-                int e = 0;
-                if (m2 > 0.f) { frexpf(m2 / 16384.f, &e); }
-                red[0] = ldexpf(1.f, e);
-            }
-            __syncthreads();
-            const float sc = red[0];
-            const float inv = 1.f / sc;            // power of two: exact
-            if (tid == 0) J.out_scale[rout] = sc;
-            if (J.store == 1) {
-                __half* o = (__half*)J.out_lp + (long long)rout * J.out_stride;
-                for (int c = tid; c < J.N; c += T) o[c] = __float2half_rn(outf[c] * inv);
-            } else {
-                __hip_bfloat16* o = (__hip_bfloat16*)J.out_lp + (long long)rout * J.out_stride;
-                for (int c = tid; c < J.N; c += T) o[c] = __float2bfloat16(outf[c] * inv);
-            }
+            store_lp_row(J, outf, rout, red);
+            if (hasb) store_lp_row(J, outf_b, rout_b, red);
         }
     }
 }
@@ -251,9 +285,13 @@ void prepare_filter(int n)
     static int done = 0;
     const size_t lds = sizeof(float2) * (size_t)n;
     if (lds <= 65536 || n <= done) return;
-    (void)hipFuncSetAttribute((const void*)&filter_rows_kernel<0>,
+    (void)hipFuncSetAttribute((const void*)&filter_rows_kernel<0, false>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    (void)hipFuncSetAttribute((const void*)&filter_rows_kernel<1>,
+    (void)hipFuncSetAttribute((const void*)&filter_rows_kernel<1, false>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    (void)hipFuncSetAttribute((const void*)&filter_rows_kernel<0, true>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    (void)hipFuncSetAttribute((const void*)&filter_rows_kernel<1, true>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     done = n;
 }
@@ -267,10 +305,17 @@ void launch_filter(const FilterJob& job, hipStream_t stream)
     const size_t lds = sizeof(float2) * (size_t)job.n;
     prepare_filter(job.n);
     if (job.nrows <= 0) return;
-    if (job.in_kind == 0)
-        hipLaunchKernelGGL(filter_rows_kernel<0>, dim3(job.nrows), dim3(T), lds, stream, job);
-    else
-        hipLaunchKernelGGL(filter_rows_kernel<1>, dim3(job.nrows), dim3(T), lds, stream, job);
+    if (job.pair) {
+        const dim3 g((job.nrows + 1) / 2);
+        if (job.in_kind == 0)
+            hipLaunchKernelGGL((filter_rows_kernel<0, true>), g, dim3(T), lds, stream, job);
+        else
+            hipLaunchKernelGGL((filter_rows_kernel<1, true>), g, dim3(T), lds, stream, job);
+    } else if (job.in_kind == 0) {
+        hipLaunchKernelGGL((filter_rows_kernel<0, false>), dim3(job.nrows), dim3(T), lds, stream, job);
+    } else {
+        hipLaunchKernelGGL((filter_rows_kernel<1, false>), dim3(job.nrows), dim3(T), lds, stream, job);
+    }
 }
 
 // ------------------------------------------------ compatibility kernels

@@ -29,6 +29,7 @@ struct FilterJob {
     int store;             // 0 f32, 1 f16, 2 bf16
     void* out_lp;          // [rows][out_stride] 16-bit rows
     float* out_scale;      // [rows] per-row scale (power of two)
+    int pair;              // 1: two rows per transform, filt = symmetrised filter
 };
 
 void launch_filter(const FilterJob& job, hipStream_t stream);

@@ -62,7 +62,9 @@ struct rv_plan_opaque {
     std::vector<Group> groups;
     void* d_rows = nullptr;
     float* d_scales = nullptr;
-    float* d_filts = nullptr;
+    float* d_filts = nullptr;            // [2][filter entries]: one-sided, symmetrised
+    size_t filt_total = 0;
+    bool pair_filter = false;            // LINEAR modes without exact_filter
     rv::AngleEntry* d_ang = nullptr;
     unsigned int* d_counter = nullptr;   // one per group
     void* d_stitch = nullptr;            // stitched offset-CT rows
@@ -161,6 +163,10 @@ int launch_filters(rv_plan_t p, const void* const* d_sino, hipStream_t st)
         J.pad_mode = zero ? 1 : 0;
         J.crop = zero ? g.n / 2 - g.h : g.ihoffset;
         J.filt = p->d_filts + S.filt_off;
+        if (p->pair_filter) {
+            J.pair = 1;
+            J.filt = p->d_filts + p->filt_total + S.filt_off;
+        }
         J.tw = tw;
         J.out_stride = g.N;
         J.store = p->store;
@@ -216,6 +222,7 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
     // extra fract); lds_half = 1 selects the FP16-pair variant.
     if (p->mode == rv::BP_LINEAR && p->store != 0 && opt->lds_half) p->mode = rv::BP_LINEAR_H;
     const bool linear = (p->mode == rv::BP_LINEAR || p->mode == rv::BP_LINEAR_H);
+    p->pair_filter = linear && !opt->exact_filter;
     const int plan_tile = (opt->tile >= 0 && opt->tile <= 3) ? opt->tile : 0;
 
     size_t rows_elems = 0, nang_total = 0, filt_total = 0, stitch_elems = 0, pair_total = 0;
@@ -274,11 +281,16 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
         S.tiles_y = (S.out_n + rv::bp_tile_h(S.tile) - 1) / rv::bp_tile_h(S.tile);
     }
 
-    // filter tables and angle tables
-    std::vector<float> hf(filt_total);
+    p->filt_total = filt_total;
+    // filter tables (one-sided, then the symmetrised copies used when two rows
+    // share a transform) and angle tables
+    std::vector<float> hf(2 * filt_total);
     std::vector<rv::AngleEntry> ha(nang_total);
     for (auto& S : p->sl) {
-        rv::set_filter(S.g.n, S.spec.filter, S.spec.cutoff, S.spec.order, hf.data() + S.filt_off);
+        float* f = hf.data() + S.filt_off;
+        rv::set_filter(S.g.n, S.spec.filter, S.spec.cutoff, S.spec.order, f);
+        float* fs = hf.data() + filt_total + S.filt_off;
+        for (int k = 0; k < S.g.n; k++) fs[k] = 0.5f * (f[k] + f[(S.g.n - k) % S.g.n]);
         const double fc = (double)S.g.fcenter16 / 16.0;
         const double h = S.g.h;
         for (int a = 0; a < S.nang; a++) {
@@ -322,7 +334,7 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
     const size_t in_esz = 4;   // stitch buffer sized for float rows (int16 uses half)
     if (!rv::hip_ok(hipMalloc(&p->d_rows, esz * std::max<size_t>(rows_elems, 1)), "filtered-sinogram workspace") ||
         !rv::hip_ok(hipMalloc(&p->d_scales, sizeof(float) * nang_total), "row scales") ||
-        !rv::hip_ok(hipMalloc(&p->d_filts, sizeof(float) * filt_total), "filters") ||
+        !rv::hip_ok(hipMalloc(&p->d_filts, sizeof(float) * 2 * filt_total), "filters") ||
         !rv::hip_ok(hipMalloc(&p->d_ang, sizeof(rv::AngleEntry) * nang_total), "angle table") ||
         !rv::hip_ok(hipMalloc(&p->d_counter, 64 * (p->groups.size() + 1)), "work counters"))
         return fail(p, std::string("plan allocation failed: ") + RecviewLastMessage());
@@ -333,7 +345,7 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
                         "pairs"))
             return fail(p, std::string("plan allocation failed: ") + RecviewLastMessage());
     }
-    if (!rv::hip_ok(hipMemcpy(p->d_filts, hf.data(), sizeof(float) * filt_total, hipMemcpyHostToDevice), "filters") ||
+    if (!rv::hip_ok(hipMemcpy(p->d_filts, hf.data(), sizeof(float) * 2 * filt_total, hipMemcpyHostToDevice), "filters") ||
         !rv::hip_ok(hipMemcpy(p->d_ang, ha.data(), sizeof(rv::AngleEntry) * nang_total, hipMemcpyHostToDevice), "angles"))
         return fail(p, std::string("plan upload failed: ") + RecviewLastMessage());
 

@@ -77,15 +77,16 @@ class FbpPlan:
 
     def __init__(self, slices, interp="linear", store="f32", lut_formula=1, index_rule=1,
                  persistent=False, tile=0, angles_per_chunk=0, lds_half=False, normalize=False,
-                 device=0):
+                 exact_filter=False, device=0):
         """normalize: images in attenuation per detector sample (x 2 pi / projections)
-        instead of raw back-projected sums."""
+        instead of raw back-projected sums.  exact_filter: LINEAR modes keep CFft's
+        operation order (bit-identical filtered rows) instead of two rows per transform."""
         self.slices = list(slices)
         L = _lib.lib()
         specs = (_lib.RvSliceSpec * len(self.slices))(*[s.to_c() for s in self.slices])
         opt = _lib.RvOptions(INTERP[interp], STORE[store], int(lut_formula), int(index_rule),
                              int(persistent), int(tile), int(angles_per_chunk), int(lds_half),
-                             int(normalize))
+                             int(normalize), int(exact_filter))
         self.interp = interp
         self.store = store
         h = C.c_void_p()
@@ -175,7 +176,7 @@ def reconstruct(sino, fdeg, center, out=None, stream=None, **kw):
     dtype = "i16" if sino.dtype == torch.int16 else "f32"
     plan_kw = {k: kw.pop(k) for k in list(kw) if k in ("interp", "store", "lut_formula", "index_rule",
                                                        "persistent", "tile", "angles_per_chunk",
-                                                       "lds_half", "normalize")}
+                                                       "lds_half", "normalize", "exact_filter")}
     spec = SliceSpec(sino.shape[1], fdeg, center, sino_dtype=dtype, **kw)
     plan = FbpPlan([spec], **plan_kw)
     n = plan.out_n(0)
@@ -196,7 +197,8 @@ def centre_sweep(sino, fdeg, centres, stream=None, **kw):
     import torch
     dtype = "i16" if sino.dtype == torch.int16 else "f32"
     plan_kw = {k: kw.pop(k) for k in list(kw) if k in ("interp", "store", "persistent", "tile",
-                                                       "angles_per_chunk", "lds_half", "normalize")}
+                                                       "angles_per_chunk", "lds_half", "normalize",
+                                                       "exact_filter")}
     plan_kw.setdefault("persistent", True)
     specs = [SliceSpec(sino.shape[1], fdeg, float(c), sino_dtype=dtype, **kw) for c in centres]
     plan = FbpPlan(specs, **plan_kw)

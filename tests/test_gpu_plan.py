@@ -81,7 +81,7 @@ def test_plan_filtered_rows_bit_exact():
     for W, P, interp in ((128, 20, 2), (300, 10, 2), (64, 8, 4)):
         s, fdeg, c = _sino(W, P, 700 + W)
         spec = fbp.SliceSpec(W, fdeg, c, iInterpolation=interp, filter="hamming")
-        plan = fbp.FbpPlan([spec])
+        plan = fbp.FbpPlan([spec], exact_filter=True)
         plan.filter([torch.from_numpy(s).cuda()])
         torch.cuda.synchronize()
         got = plan.filtered_rows_tensor(0).cpu().numpy()
@@ -89,6 +89,19 @@ def test_plan_filtered_rows_bit_exact():
         ref = O.filter_rows(s, g, O.set_filter(g.ndim, 1), kind=1)
         plan.close()
         assert np.array_equal(got, ref), f"W={W}: max {np.abs(got - ref).max()}"
+        # default LINEAR plans filter two rows per transform with the symmetrised
+        # filter: the same operator, so equal to the FP64 filter to FP32 rounding
+        for store in ("f32", "f16"):
+            plan = fbp.FbpPlan([spec], store=store)
+            plan.filter([torch.from_numpy(s).cuda()])
+            torch.cuda.synchronize()
+            pair = plan.filtered_rows_tensor(0).cpu().numpy()
+            plan.close()
+            ref64 = O.filter_rows_f64(s, g, O.set_filter(g.ndim, 1))
+            scale = np.abs(ref64).max()
+            tol = 5e-6 if store == "f32" else 2e-3   # two rows share the rounding of one transform
+            assert np.abs(pair - ref64).max() <= tol * scale, (W, store, np.abs(pair - ref64).max() / scale)
+            assert np.abs(got - ref64).max() <= 5e-6 * scale      # CFft order itself: FP32 rounding
 
 
 def test_plan_linear_bp_vs_fp64():

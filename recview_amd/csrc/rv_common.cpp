@@ -140,6 +140,29 @@ const float* device_twiddles(int dev, int np)
     return d;
 }
 
+const float* device_twiddles_full(int dev, int np)
+{
+    std::lock_guard<std::mutex> lk(g_tw_mu);
+    auto key = std::make_pair(dev, np | 0x100);
+    auto it = g_tw.find(key);
+    if (it != g_tw.end()) return it->second;
+    const int n = 1 << np;
+    std::vector<float> h(2 * (size_t)n);
+    for (int k = 0; k < n; k++) {
+        const double a = -6.283185307179586476925 * k / n;
+        h[2 * k] = (float)std::cos(a);
+        h[2 * k + 1] = (float)std::sin(a);
+    }
+    float* d = nullptr;
+    if (hipMalloc(&d, sizeof(float) * 2 * n) != hipSuccess) return nullptr;
+    if (hipMemcpy(d, h.data(), sizeof(float) * 2 * n, hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(d);
+        return nullptr;
+    }
+    g_tw[key] = d;
+    return d;
+}
+
 void drop_device_caches(int dev)
 {
     std::lock_guard<std::mutex> lk(g_tw_mu);

@@ -25,6 +25,9 @@ double theta_rad(float fdeg, float tilt);
 
 // CFft twiddles (cfft.cpp:403-409) for 2^np points, cached per device.
 const float* device_twiddles(int dev, int np);
+// exp(-2 pi i k / n) for k < n (interleaved re, im; computed in double) for the
+// standard-convention transforms of the paired filter, cached per device.
+const float* device_twiddles_full(int dev, int np);
 void drop_device_caches(int dev);
 
 // Error reporting: the ring of reference error codes and a thread-local message.

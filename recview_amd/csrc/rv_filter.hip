@@ -37,6 +37,13 @@ __device__ float prep_value(const FilterJob& J, const void* row, int m)
 {
     const int z = J.z, W = J.W, ic = J.ic;
     const int rel = m - J.place;
+    if (z == 1) {
+        // no zoom: sample ic + rel, edge-replicated (pad_mode 0) or zero outside
+        const int k = ic + rel;
+        if (k >= 0 && k < W) return load_in<IN_KIND>(row, k);
+        if (J.pad_mode != 0) return 0.f;
+        return load_in<IN_KIND>(row, k < 0 ? 0 : W - 1);
+    }
     const int kk = (rel >= 0) ? rel / z : -((-rel + z - 1) / z);
     const int j = rel - kk * z;
     const int k = ic + kk;
@@ -162,34 +169,21 @@ __device__ void store_lp_row(const FilterJob& J, const float* outf, int rout, fl
     __syncthreads();
 }
 
-// PAIR: two real rows share one complex transform (row a in the real part,
-// row b in the imaginary part).  J.filt then holds the Hermitian-symmetrised
-// filter (H[k] + H[n-k]) / 2, which is real and even, so the transform keeps
-// the two rows apart and applies the same operator as the one-sided filter
-// followed by taking the real part: half the work, CFft's network but not its
-// rounding (used where parity is to tolerance, not bit-exact).
-template <int IN_KIND, bool PAIR>
+template <int IN_KIND>
 __global__ void __launch_bounds__(1024) filter_rows_kernel(FilterJob J)
 {
     extern __shared__ float2 buf[];
-    const int ra = PAIR ? 2 * (int)blockIdx.x : (int)blockIdx.x;
-    const bool hasb = PAIR && (ra + 1 < J.nrows);
-    const int rin = J.row_in ? J.row_in[ra] : ra;
-    const int rout = J.row_out ? J.row_out[ra] : ra;
-    const int rin_b = hasb ? (J.row_in ? J.row_in[ra + 1] : ra + 1) : 0;
-    const int rout_b = hasb ? (J.row_out ? J.row_out[ra + 1] : ra + 1) : 0;
+    const int r = blockIdx.x;
+    const int rin = J.row_in ? J.row_in[r] : r;
+    const int rout = J.row_out ? J.row_out[r] : r;
     const int n = J.n, half = n >> 1, np = J.np, T = blockDim.x, tid = threadIdx.x;
     const float2* tw = (const float2*)J.tw;
     const void* row = (IN_KIND == 0)
         ? (const void*)((const int16_t*)J.sino + (long long)rin * J.in_stride)
         : (const void*)((const float*)J.sino + (long long)rin * J.in_stride);
-    const void* row_b = (IN_KIND == 0)
-        ? (const void*)((const int16_t*)J.sino + (long long)rin_b * J.in_stride)
-        : (const void*)((const float*)J.sino + (long long)rin_b * J.in_stride);
 
     // Row preparation straight into LDS.
-    for (int m = tid; m < n; m += T)
-        buf[m] = make_float2(prep_value<IN_KIND>(J, row, m), hasb ? prep_value<IN_KIND>(J, row_b, m) : 0.f);
+    for (int m = tid; m < n; m += T) buf[m] = make_float2(prep_value<IN_KIND>(J, row, m), 0.f);
     __syncthreads();
 
     // ---- FFT1Rev (cfft.cpp:431-434 -> FFTrev with nmin < 0)
@@ -234,8 +228,7 @@ __global__ void __launch_bounds__(1024) filter_rows_kernel(FilterJob J)
         }
         __syncthreads();
         const float a = 1.f / (float)n;
-        float* outf = (float*)buf;                 // reuse LDS as the cropped real row(s)
-        float* outf_b = outf + J.N;
+        float* outf = (float*)buf;                 // reuse LDS as the cropped real row
         const bool lp = (J.store != 0);
 #pragma unroll
         for (int i = 0; i < kMaxBfly; i++) {
@@ -247,34 +240,159 @@ __global__ void __launch_bounds__(1024) filter_rows_kernel(FilterJob J)
                 // the caller keeps p[c + crop].re (cudaReconstHost.cpp:756).
                 const float v0 = (A[i].x - s.x) * a;   // f[k]
                 const float v1 = (A[i].x + s.x) * a;   // f[k + half]
-                const float w0 = (A[i].y - s.y) * a;   // imaginary parts: row b (PAIR)
-                const float w1 = (A[i].y + s.y) * a;
                 const int c0 = n - 1 - k - J.crop;
                 const int c1 = n - 1 - (k + half) - J.crop;
                 if (!lp) {
                     float* o = J.out + (long long)rout * J.out_stride;
                     if (c0 >= 0 && c0 < J.N) o[c0] = v0;
                     if (c1 >= 0 && c1 < J.N) o[c1] = v1;
-                    if (hasb) {
-                        float* ob = J.out + (long long)rout_b * J.out_stride;
-                        if (c0 >= 0 && c0 < J.N) ob[c0] = w0;
-                        if (c1 >= 0 && c1 < J.N) ob[c1] = w1;
-                    }
                 } else {
                     if (c0 >= 0 && c0 < J.N) outf[c0] = v0;
                     if (c1 >= 0 && c1 < J.N) outf[c1] = v1;
-                    if (hasb) {
-                        if (c0 >= 0 && c0 < J.N) outf_b[c0] = w0;
-                        if (c1 >= 0 && c1 < J.N) outf_b[c1] = w1;
-                    }
                 }
             }
         }
         if (lp) {
             __shared__ float red[32];
             store_lp_row(J, outf, rout, red);
-            if (hasb) store_lp_row(J, outf_b, rout_b, red);
         }
+    }
+}
+
+// ------------------------------------------------ paired rows, radix-4 transform
+// Production filter for the LINEAR modes: rows a and b become the real and
+// imaginary parts of one complex row; with the Hermitian-symmetrised filter
+// Hs[k] = (H[k] + H[n-k]) / 2 (real and even) the result is
+//   y = IFFT(Hs * FFT(a + i b)) = a (*) h + i (b (*) h),  h = IFFT(Hs) = h_re,
+// the same circular convolution the CFft pipeline computes (one-sided filter,
+// real part), in the standard DFT convention.  The transforms are Stockham
+// autosort passes of radix 4 (radix 2 first when log2 n is odd), after
+// Govindaraju et al., "High performance discrete Fourier transforms on
+// graphics processors" (SC08): pass with sub-length Ns reads x[j + r*n/R],
+// twiddles by exp(-2 pi i r (j mod Ns) / (Ns R)), does an R-point DFT and
+// writes x'[(j / Ns) * Ns * R + j mod Ns + r * Ns].  The inverse is
+// conj(FFT(conj(.))) / n, the conjugation folded into the filter multiply.
+__device__ __forceinline__ float2 cmulf(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+template <int R, int NB>
+__device__ __forceinline__ void stockham_pass(float2* buf, const float2* twf, int n, int Ns, int T, int tid)
+{
+    const int nb = n / R;
+    const int tstep = n / (Ns * R);
+    float2 v[NB][R];
+#pragma unroll
+    for (int i = 0; i < NB; i++) {
+        const int j = tid + i * T;
+        if (j < nb) {
+            const int jm = j & (Ns - 1);
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                float2 x = buf[j + r * nb];
+                if (r > 0) x = cmulf(x, twf[(r * jm * tstep) & (n - 1)]);
+                v[i][r] = x;
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NB; i++) {
+        const int j = tid + i * T;
+        if (j < nb) {
+            const int jm = j & (Ns - 1);
+            const int d = (j - jm) * R + jm;
+            if (R == 2) {
+                buf[d] = make_float2(v[i][0].x + v[i][1].x, v[i][0].y + v[i][1].y);
+                buf[d + Ns] = make_float2(v[i][0].x - v[i][1].x, v[i][0].y - v[i][1].y);
+            } else {
+                const float2 t0 = make_float2(v[i][0].x + v[i][2].x, v[i][0].y + v[i][2].y);
+                const float2 t1 = make_float2(v[i][0].x - v[i][2].x, v[i][0].y - v[i][2].y);
+                const float2 t2 = make_float2(v[i][1].x + v[i][3].x, v[i][1].y + v[i][3].y);
+                const float2 t3 = make_float2(v[i][1].x - v[i][3].x, v[i][1].y - v[i][3].y);
+                buf[d] = make_float2(t0.x + t2.x, t0.y + t2.y);
+                buf[d + Ns] = make_float2(t1.x + t3.y, t1.y - t3.x);          // t1 - i t3
+                buf[d + 2 * Ns] = make_float2(t0.x - t2.x, t0.y - t2.y);
+                buf[d + 3 * Ns] = make_float2(t1.x - t3.y, t1.y + t3.x);      // t1 + i t3
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// Forward DFT of buf (n = 2^np points, T threads, n <= 16 T).
+__device__ __forceinline__ void fft_radix4(float2* buf, const float2* twf, int n, int np, int T, int tid)
+{
+    int Ns = 1;
+    if (np & 1) {
+        stockham_pass<2, 8>(buf, twf, n, Ns, T, tid);
+        Ns = 2;
+    }
+    for (; Ns < n; Ns *= 4) stockham_pass<4, 4>(buf, twf, n, Ns, T, tid);
+}
+
+template <int IN_KIND>
+__global__ void __launch_bounds__(1024) filter_pair_kernel(FilterJob J)
+{
+    extern __shared__ float2 buf[];
+    const int ra = 2 * (int)blockIdx.x;
+    const bool hasb = ra + 1 < J.nrows;
+    const int rin = J.row_in ? J.row_in[ra] : ra;
+    const int rout = J.row_out ? J.row_out[ra] : ra;
+    const int rin_b = hasb ? (J.row_in ? J.row_in[ra + 1] : ra + 1) : 0;
+    const int rout_b = hasb ? (J.row_out ? J.row_out[ra + 1] : ra + 1) : 0;
+    const int n = J.n, np = J.np, T = blockDim.x, tid = threadIdx.x;
+    const float2* twf = (const float2*)J.tw;
+    const void* row = (IN_KIND == 0)
+        ? (const void*)((const int16_t*)J.sino + (long long)rin * J.in_stride)
+        : (const void*)((const float*)J.sino + (long long)rin * J.in_stride);
+    const void* row_b = (IN_KIND == 0)
+        ? (const void*)((const int16_t*)J.sino + (long long)rin_b * J.in_stride)
+        : (const void*)((const float*)J.sino + (long long)rin_b * J.in_stride);
+    for (int m = tid; m < n; m += T)
+        buf[m] = make_float2(prep_value<IN_KIND>(J, row, m), hasb ? prep_value<IN_KIND>(J, row_b, m) : 0.f);
+    __syncthreads();
+    fft_radix4(buf, twf, n, np, T, tid);
+    for (int k = tid; k < n; k += T) {            // conj(Hs * X)
+        const float h = J.filt[k];
+        const float2 x = buf[k];
+        buf[k] = make_float2(x.x * h, -(x.y * h));
+    }
+    __syncthreads();
+    fft_radix4(buf, twf, n, np, T, tid);
+    // y = conj(buf) / n; rows: real part -> a, imaginary part -> b
+    const float a = 1.f / (float)n;
+    if (J.store == 0) {
+        float* o = J.out + (long long)rout * J.out_stride;
+        float* ob = J.out + (long long)rout_b * J.out_stride;
+        for (int c = tid; c < J.N; c += T) {
+            const float2 y = buf[c + J.crop];
+            o[c] = y.x * a;
+            if (hasb) ob[c] = -y.y * a;
+        }
+    } else {
+        float va[8], vb[8];
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const int c = tid + i * T;
+            if (c < J.N) {
+                const float2 y = buf[c + J.crop];
+                va[i] = y.x * a;
+                vb[i] = -y.y * a;
+            }
+        }
+        __syncthreads();
+        float* outf = (float*)buf;
+        float* outf_b = outf + J.N;
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const int c = tid + i * T;
+            if (c < J.N) { outf[c] = va[i]; outf_b[c] = vb[i]; }
+        }
+        __shared__ float red[32];
+        store_lp_row(J, outf, rout, red);
+        if (hasb) store_lp_row(J, outf_b, rout_b, red);
     }
 }
 
@@ -285,13 +403,13 @@ void prepare_filter(int n)
     static int done = 0;
     const size_t lds = sizeof(float2) * (size_t)n;
     if (lds <= 65536 || n <= done) return;
-    (void)hipFuncSetAttribute((const void*)&filter_rows_kernel<0, false>,
+    (void)hipFuncSetAttribute((const void*)&filter_rows_kernel<0>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    (void)hipFuncSetAttribute((const void*)&filter_rows_kernel<1, false>,
+    (void)hipFuncSetAttribute((const void*)&filter_rows_kernel<1>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    (void)hipFuncSetAttribute((const void*)&filter_rows_kernel<0, true>,
+    (void)hipFuncSetAttribute((const void*)&filter_pair_kernel<0>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    (void)hipFuncSetAttribute((const void*)&filter_rows_kernel<1, true>,
+    (void)hipFuncSetAttribute((const void*)&filter_pair_kernel<1>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     done = n;
 }
@@ -308,13 +426,13 @@ void launch_filter(const FilterJob& job, hipStream_t stream)
     if (job.pair) {
         const dim3 g((job.nrows + 1) / 2);
         if (job.in_kind == 0)
-            hipLaunchKernelGGL((filter_rows_kernel<0, true>), g, dim3(T), lds, stream, job);
+            hipLaunchKernelGGL(filter_pair_kernel<0>, g, dim3(T), lds, stream, job);
         else
-            hipLaunchKernelGGL((filter_rows_kernel<1, true>), g, dim3(T), lds, stream, job);
+            hipLaunchKernelGGL(filter_pair_kernel<1>, g, dim3(T), lds, stream, job);
     } else if (job.in_kind == 0) {
-        hipLaunchKernelGGL((filter_rows_kernel<0, false>), dim3(job.nrows), dim3(T), lds, stream, job);
+        hipLaunchKernelGGL(filter_rows_kernel<0>, dim3(job.nrows), dim3(T), lds, stream, job);
     } else {
-        hipLaunchKernelGGL((filter_rows_kernel<1, false>), dim3(job.nrows), dim3(T), lds, stream, job);
+        hipLaunchKernelGGL(filter_rows_kernel<1>, dim3(job.nrows), dim3(T), lds, stream, job);
     }
 }
 

@@ -29,7 +29,8 @@ struct FilterJob {
     int store;             // 0 f32, 1 f16, 2 bf16
     void* out_lp;          // [rows][out_stride] 16-bit rows
     float* out_scale;      // [rows] per-row scale (power of two)
-    int pair;              // 1: two rows per transform, filt = symmetrised filter
+    int pair;              // 1: two rows per transform (filter_pair_kernel): filt = symmetrised
+                           //    filter, tw = full table exp(-2 pi i k / n), k < n
 };
 
 void launch_filter(const FilterJob& job, hipStream_t stream);

@@ -163,11 +163,13 @@ int launch_filters(rv_plan_t p, const void* const* d_sino, hipStream_t st)
         J.pad_mode = zero ? 1 : 0;
         J.crop = zero ? g.n / 2 - g.h : g.ihoffset;
         J.filt = p->d_filts + S.filt_off;
+        J.tw = tw;
         if (p->pair_filter) {
             J.pair = 1;
             J.filt = p->d_filts + p->filt_total + S.filt_off;
+            J.tw = rv::device_twiddles_full(p->dev, g.ndimp);
+            if (!J.tw) { rv::set_message("twiddle table allocation failed"); return -1; }
         }
-        J.tw = tw;
         J.out_stride = g.N;
         J.store = p->store;
         if (p->store == 0) {
@@ -406,7 +408,9 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
     // execution (and hipGraph capture of it) only launches kernels.
     int max_n = 0;
     for (const auto& S : p->sl) {
-        if (!rv::device_twiddles(device, S.g.ndimp)) return fail(p, "twiddle table allocation failed");
+        if (!rv::device_twiddles(device, S.g.ndimp) ||
+            (p->pair_filter && !rv::device_twiddles_full(device, S.g.ndimp)))
+            return fail(p, "twiddle table allocation failed");
         max_n = std::max(max_n, S.g.n);
     }
     rv::prepare_filter(max_n);

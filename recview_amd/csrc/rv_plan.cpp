@@ -418,16 +418,25 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
     return 0;
 }
 
+// Every slice needs a non-null device pointer (the arrays hold one per slice).
+static bool all_set(rv_plan_t p, const void* const* ptrs, const char* what)
+{
+    if (!ptrs) { rv::set_message(std::string(what) + " array is null"); return false; }
+    for (size_t s = 0; s < p->sl.size(); s++)
+        if (!ptrs[s]) { rv::set_message(std::string(what) + " pointer of slice " + std::to_string(s) + " is null"); return false; }
+    return true;
+}
+
 int rvPlanFilter(rv_plan_t p, const void* const* d_sino, rv_stream_t stream)
 {
-    if (!p) return -1;
+    if (!p || !all_set(p, d_sino, "sinogram")) return -1;
     if (!rv::hip_ok(hipSetDevice(p->dev), "hipSetDevice")) return -1;
     return launch_filters(p, d_sino, (hipStream_t)stream);
 }
 
 int rvPlanBackproject(rv_plan_t p, void* const* d_out, rv_stream_t stream)
 {
-    if (!p) return -1;
+    if (!p || !all_set(p, (const void* const*)d_out, "image")) return -1;
     if (!rv::hip_ok(hipSetDevice(p->dev), "hipSetDevice")) return -1;
     if (!upload_slices(p, d_out)) return -1;
     return launch_backprojection(p, (hipStream_t)stream);
@@ -435,7 +444,7 @@ int rvPlanBackproject(rv_plan_t p, void* const* d_out, rv_stream_t stream)
 
 int rvPlanExecute(rv_plan_t p, const void* const* d_sino, void* const* d_out, rv_stream_t stream)
 {
-    if (!p) return -1;
+    if (!p || !all_set(p, d_sino, "sinogram") || !all_set(p, (const void* const*)d_out, "image")) return -1;
     if (!rv::hip_ok(hipSetDevice(p->dev), "hipSetDevice")) return -1;
     if (!upload_slices(p, d_out)) return -1;
     if (launch_filters(p, d_sino, (hipStream_t)stream)) return -1;
@@ -444,7 +453,7 @@ int rvPlanExecute(rv_plan_t p, const void* const* d_sino, void* const* d_out, rv
 
 int rvPlanExecuteGraph(rv_plan_t p, const void* const* d_sino, void* const* d_out, rv_stream_t stream)
 {
-    if (!p) return -1;
+    if (!p || !all_set(p, d_sino, "sinogram") || !all_set(p, (const void* const*)d_out, "image")) return -1;
     if (!rv::hip_ok(hipSetDevice(p->dev), "hipSetDevice")) return -1;
     const size_t ns = p->sl.size();
     bool same = (p->gexec != nullptr) && p->g_sino.size() == ns;

@@ -109,15 +109,22 @@ class FbpPlan:
         _lib.check(_lib.lib().rvPlanRows(self._h, s, C.byref(n), C.byref(w)), "rvPlanRows")
         return n.value, w.value
 
+    def _count(self, tensors, what):
+        if len(tensors) != len(self.slices):
+            raise ValueError(f"{what}: {len(tensors)} tensors for a plan of {len(self.slices)} slices")
+        return _ptrs(tensors)
+
     def execute(self, sinos, outs, stream=None, graph=False):
         fn = _lib.lib().rvPlanExecuteGraph if graph else _lib.lib().rvPlanExecute
-        _lib.check(fn(self._h, _ptrs(sinos), _ptrs(outs), _stream_handle(stream)), "rvPlanExecute")
+        _lib.check(fn(self._h, self._count(sinos, "sinograms"), self._count(outs, "images"),
+                      _stream_handle(stream)), "rvPlanExecute")
 
     def filter(self, sinos, stream=None):
-        _lib.check(_lib.lib().rvPlanFilter(self._h, _ptrs(sinos), _stream_handle(stream)), "rvPlanFilter")
+        _lib.check(_lib.lib().rvPlanFilter(self._h, self._count(sinos, "sinograms"), _stream_handle(stream)),
+                   "rvPlanFilter")
 
     def backproject(self, outs, stream=None):
-        _lib.check(_lib.lib().rvPlanBackproject(self._h, _ptrs(outs), _stream_handle(stream)),
+        _lib.check(_lib.lib().rvPlanBackproject(self._h, self._count(outs, "images"), _stream_handle(stream)),
                    "rvPlanBackproject")
 
     def filtered_rows(self, s):

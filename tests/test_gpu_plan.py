@@ -285,3 +285,28 @@ def test_centre_sweep_matches_single_and_finds_axis():
     tv = [np.abs(np.diff(img, axis=0))[m[1:]].sum() + np.abs(np.diff(img, axis=1))[m[:, 1:]].sum()
           for img in sweep]
     assert int(np.argmin(tv)) == 3
+
+
+def test_plan_edge_cases():
+    """Degenerate and ragged inputs: one projection, the narrowest detector the
+    transform supports, a slice wider than tall batches, and clean errors."""
+    # one projection: the image is that filtered row smeared along the ray
+    s, fdeg, c = _sino(64, 1, 11)
+    _, (img,) = _run([fbp.SliceSpec(64, fdeg, c)], [torch.from_numpy(s)])
+    assert np.isfinite(img).all() and np.allclose(img, img[0:1, :], atol=1e-6 * np.abs(img).max() + 1e-12)
+    # W = 4 (n = 8 point transform) and a mixed batch with it
+    s4, f4, c4 = _sino(4, 6, 12)
+    s9, f9, c9 = _sino(96, 7, 13)
+    _, outs = _run([fbp.SliceSpec(4, f4, c4), fbp.SliceSpec(96, f9, c9)], [torch.from_numpy(s4), torch.from_numpy(s9)],
+                   persistent=True)
+    assert outs[0].shape == (4, 4) and np.isfinite(outs[0]).all() and np.isfinite(outs[1]).all()
+    # errors are raised, not crashes: no slices, width below the supported range
+    with pytest.raises(RuntimeError):
+        fbp.FbpPlan([])
+    with pytest.raises(RuntimeError):
+        fbp.FbpPlan([fbp.SliceSpec(3, f4, 1.5)])
+    # wrong number of sinograms for the plan
+    plan = fbp.FbpPlan([fbp.SliceSpec(96, f9, c9)])
+    with pytest.raises(Exception):
+        plan.execute([], [])
+    plan.close()

@@ -198,3 +198,41 @@ def test_device_queries():
     assert L.CUDA_MALLOC_HOST_INT(C.byref(p), 4096) == 0
     p[0] = 5
     assert L.CUDA_FREE_HOST(p) == 0
+
+
+def test_oversized_buffers_set_error():
+    """Buffers beyond the reference's 32-bit sizes (cudaReconstHost.cpp:39) are
+    refused like a failed allocation: RECONST_INFO_ERROR and code 28801, before
+    any row is read (every projection pointer below aims at one host row)."""
+    import ctypes as C
+    from recview_amd import _lib, reconst
+    W, P = 2048, 600_000                      # P x N floats > 4 GiB
+    row = np.zeros(W, np.int16)
+    ptrs = (C.POINTER(C.c_short) * P)()
+    p0 = row.ctypes.data_as(C.POINTER(C.c_short))
+    for i in range(P):
+        ptrs[i] = p0
+    info = reconst.ReconstInfo()
+    ri = info.ri
+    ri.iSinogr = C.cast(ptrs, C.POINTER(C.POINTER(C.c_short)))
+    ri.maxSinogrLen = P
+    ri.iStatus = 1
+    ri.iStepSino, ri.iStartSino = 1, 0
+    ri.ixdim, ri.iInterpolation = W, 2
+    ri.center = W / 2 + 0.5
+    ri.iLenSinogr = P + 1
+    img = np.zeros((W, W), np.int32)
+    cnt = np.zeros(1, np.int32)
+    binc = np.ones(P + 1, np.int8)
+    fdeg = np.linspace(0, 180, P, endpoint=False).astype(np.float32)
+    filt = reconst.set_filter(4096, 0)
+    ri.iReconst = img.ctypes.data_as(C.POINTER(C.c_int))
+    ri.nSinogr = cnt.ctypes.data_as(C.POINTER(C.c_int))
+    ri.bInc = binc.ctypes.data_as(C.POINTER(C.c_char))
+    ri.fdeg = fdeg.ctypes.data_as(C.POINTER(C.c_float))
+    ri.fFilter = filt.ctypes.data_as(C.POINTER(C.c_float))
+    ri.iMultiplex = 1
+    _lib.lib().RecviewClearErrors()
+    _lib.lib().CudaReconstHost(C.byref(ri), 0, False, True)
+    assert ri.iStatus == 2 and 28801 in _lib.errors()
+    info.free()

@@ -11,6 +11,9 @@ of the whole stack, sinograms already resident in HBM, images written to HBM.
 1800 angles and restored by the BF16-MFMA U-Net, beside full-angle FBP of the
 same slices (rate ratio, PSNR / SSIM).
 
+--workload dropin: the host-buffer entry point CudaReconstHost, one call per
+slice (int16 rows in, int32 image out): the PCIe-inclusive rate.
+
 --workload c3: 256 slices of uneven width {512..4096} and projection count
 {450..3600}, a quarter of them 360-degree offset-axis scans stitched on the GPU
 to double width, per-shape autotuned tiles, one hipGraph per step.
@@ -247,6 +250,40 @@ def run_c5(args):
     print(json.dumps(result), flush=True)
 
 
+def run_dropin(args):
+    """The drop-in entry point with host buffers: CudaReconstHost on int16 rows
+    (LUT16 integer semantics), image copied back -- the PCIe-inclusive rate."""
+    import torch
+    from recview_amd import phantom, reconst
+
+    dev = torch.device("cuda:0")
+    S, P, W = (args.slices or 8), args.proj, args.width
+    sinos, fdeg, c_det = make_stack(S, P, W, 1002, dev)
+    rows = [phantom.to_int16(sinos[s].double().cpu()).numpy() for s in range(S)]
+    info = reconst.ReconstInfo()
+    c_user = c_det - 0.5
+    for _ in range(max(1, args.warmup)):
+        reconst.deconv_back_proj(rows[0], fdeg, c_user, info=info, normalise=False)
+    t = time.perf_counter()
+    for k in range(args.steps):
+        for s in range(S):
+            reconst.deconv_back_proj(rows[s], fdeg, c_user, info=info, normalise=False)
+    el = (time.perf_counter() - t) / args.steps
+    info.free()
+    result = {
+        "metric": "drop-in CudaReconstHost slices/s incl. host transfers (2048^2 x 1800, LUT16 int)",
+        "value": round(S / el, 3), "unit": "slices/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(el * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "int32", "data": "synthetic",
+        "ms_per_slice": round(el * 1e3 / S, 3),
+        "tau_ns": round(el / S / (W * W * P) * 1e9, 7),
+        "config": {"workload": f"drop-in: {S} slices, {P} proj x {W} det int16 host rows -> int32 host "
+                               "image, one CudaReconstHost call per slice", "slices": S},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(result), flush=True)
+
+
 def _blas_threads():
     try:
         from threadpoolctl import threadpool_info
@@ -260,7 +297,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c5"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c5", "dropin"])
     ap.add_argument("--slices", type=int, default=0, help="default 64 (c2) / 256 (c3) / 16 (c5)")
     ap.add_argument("--proj", type=int, default=1800)
     ap.add_argument("--width", type=int, default=2048)
@@ -282,6 +319,9 @@ def main():
 
     if args.workload == "c5":
         run_c5(args)
+        return
+    if args.workload == "dropin":
+        run_dropin(args)
         return
 
     import torch

@@ -156,6 +156,19 @@ int GetCudaClockRate(int iDevice, int* piClockRate, int* piMemRate);
  * Zernike mode, 8- / 16-bit pixel floors.  Sets ri->iStatus to IDLE, or to
  * ERROR where the reference does (no incident frame). */
 void CudaSinogramHost(RECONST_INFO* ri, int idev, bool bReport);
+/* The same conversion on device-resident frames (no counterpart in the
+ * reference): d_frames = uint16 [nframes][mult][W] (dark frame last); binc,
+ * fexp, fdeg (and piDrift, may be NULL) are host arrays of nframes entries.
+ * Converted rows are written to d_out in the frames' layout (row i*mult + k),
+ * as int16 (x3000, the reference's values) or, with out_float = 1, as float
+ * line integrals -ln(I/I0) without scaling or rounding, ready for an rvPlan.
+ * Other rows of d_out are not written.  Returns the number of converted rows,
+ * -1 on bad arguments / device errors, -2 where the reference sets
+ * RECONST_INFO_ERROR (no incident frame; rows before it are converted). */
+int RecviewSinogramDevice(const unsigned short* d_frames, int nframes, int mult, int W, const char* binc,
+                          const float* fexp, const float* fdeg, unsigned flags, int drStart, int drEnd,
+                          double drX, double drY, int drOmit, const int* piDrift, int is8bit, int start,
+                          int step, int binning, int out_float, void* d_out, rv_stream_t stream);
 
 /* ------------------------------------------ Part 1: kernel wrappers -- */
 

@@ -74,7 +74,7 @@ EXPORTS = [
     "RecviewSetFilter", "RecviewGeometry", "RecviewOffsetCTPairs", "RecviewOffsetCTStitch",
     "rvUnetCreate", "rvUnetParamCount", "rvUnetGetParams", "rvUnetSetParams", "rvUnetForward",
     "rvUnetLossGrad", "rvUnetTrainStep", "rvUnetDestroy", "rvConv3x3Bf16", "rvConv3x3WgradBf16",
-    "RecviewOutputStage", "RecviewImageDescription", "RecviewWriteTif16",
+    "RecviewOutputStage", "RecviewImageDescription", "RecviewWriteTif16", "RecviewSinogramDevice",
 ]
 
 
@@ -130,6 +130,9 @@ def _declare(L):
     L.RecviewGeometry.argtypes = [C.c_int, C.c_int, ip, ip]
     L.RecviewOffsetCTPairs.argtypes = [fp, C.c_char_p, C.c_int, ip]
     L.RecviewOffsetCTStitch.argtypes = [vp, C.c_int, C.c_int, vp, C.c_int, C.c_double, C.c_double,
+                                        C.c_int, C.c_int, C.c_int, vp, vp]
+    L.RecviewSinogramDevice.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_char_p, fp, fp, C.c_uint, C.c_int,
+                                        C.c_int, C.c_double, C.c_double, C.c_int, ip, C.c_int, C.c_int,
                                         C.c_int, C.c_int, C.c_int, vp, vp]
     L.RecviewOutputStage.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_int, vp, C.POINTER(C.c_double),
                                      C.POINTER(C.c_double), vp]

@@ -321,25 +321,14 @@ rv_error_t CUDA_MALLOC_HOST_INT(int** ptr, size_t size)
 
 rv_error_t CUDA_FREE_HOST(void* ptr) { return (rv_error_t)hipHostFree(ptr); }
 
-void CudaSinogramHost(RECONST_INFO* ri, int idev, bool)
+}  // extern "C"
+
+namespace rv {
+
+bool sinogram_jobs(const SinoWalk& w, std::vector<SinoJob>& jobs)
 {
-    // Production semantics of GenerateSinogramThread (general.cpp:2377-2532): the
-    // frame walk below (incident frames i0 / i1, exposure weight, drift, SKIP)
-    // is sequential and runs on the host; the per-sample work runs in one kernel
-    // launch over every converted row.  Rows are converted in place.
-    if (!ri || !ri->iSinogr || !ri->bInc || !ri->fexp || !ri->fdeg || ri->ixdim <= 0 ||
-        ri->iMultiplex <= 0 || ri->maxSinogrLen < 2 || ri->iStepSino <= 0) {
-        rv::set_message("CudaSinogramHost: incomplete RECONST_INFO");
-        rv::log_code(28901);
-        if (ri) ri->iStatus = RECONST_INFO_ERROR;
-        return;
-    }
-    const int isino = ri->maxSinogrLen, W = ri->ixdim, M = ri->iMultiplex;
-    const int binning = ri->iInterpolation == 0 ? 4 : (ri->iInterpolation == 1 ? 2 : 1);
-    const char* binc = ri->bInc;
-    const float* fexp = ri->fexp;
-    std::vector<rv::SinoJob> jobs;
-    bool error = false;
+    const char* binc = w.binc;
+    const int isino = w.nframes;
     int i0 = -1, i1 = -1;
     for (int i = 0; i < isino - 1; i++) {
         if (binc[i] & CGAZODOC_BINC_NOUSE) continue;
@@ -356,33 +345,107 @@ void CudaSinogramHost(RECONST_INFO* ri, int idev, bool)
             }
             if (i1 < 0) {
                 if (i0 >= 0) i1 = i0;
-                else { error = true; break; }
+                else return false;
             }
         }
-        if (i0 < 0 || i1 < 0) { error = true; break; }
-        if (i % ri->iStepSino != ri->iStartSino) continue;
-        float t0 = fexp[i0];
-        const float t1 = fexp[i1], ti = fexp[i];
+        if (i0 < 0 || i1 < 0) return false;
+        if (i % w.step != w.start) continue;
+        float t0 = w.fexp[i0];
+        const float t1 = w.fexp[i1], ti = w.fexp[i];
         if (t0 == t1) t0 = 0;
         else t0 = (ti - t0) / (t1 - t0);
-        const double th = rv::theta_rad(ri->fdeg[i], 0.f), cth = std::cos(th), sth = std::sin(th);
+        const double th = theta_rad(w.fdeg[i], 0.f), cth = std::cos(th), sth = std::sin(th);
         int idrift = 0, action = 1;
-        if (ri->drEnd > ri->drStart && (ri->dReconFlags & RQFLAGS_DRIFTPARAMS)) {
-            if (ri->drStart < i && i < ri->drEnd && ri->drOmit) action = 2;
-            else if (i >= ri->drEnd) idrift = (int)(ri->drX * cth - ri->drY * sth);
-            else if (i > ri->drStart)
-                idrift = (int)((ri->drX * cth - ri->drY * sth) * (i - ri->drStart) / (ri->drEnd - ri->drStart));
+        if (w.drEnd > w.drStart && (w.flags & RQFLAGS_DRIFTPARAMS)) {
+            if (w.drStart < i && i < w.drEnd && w.drOmit) action = 2;
+            else if (i >= w.drEnd) idrift = (int)(w.drX * cth - w.drY * sth);
+            else if (i > w.drStart)
+                idrift = (int)((w.drX * cth - w.drY * sth) * (i - w.drStart) / (w.drEnd - w.drStart));
         }
         if (action == 1 && (binc[i] & CGAZODOC_BINC_SKIP)) action = 2;
-        if (action == 1 && (ri->dReconFlags & RQFLAGS_DRIFTLIST) && ri->piDrift) idrift -= ri->piDrift[i];
-        for (int k = 0; k < M; k++) {
-            rv::SinoJob J{};
+        if (action == 1 && (w.flags & RQFLAGS_DRIFTLIST) && w.piDrift) idrift -= w.piDrift[i];
+        for (int k = 0; k < w.mult; k++) {
+            SinoJob J{};
             J.frame = i; J.k = k; J.i0 = i0; J.i1 = i1; J.t0 = t0;
-            J.shift = action == 1 ? idrift / binning : 0;
+            J.shift = action == 1 ? idrift / w.binning : 0;
             J.action = action;
             jobs.push_back(J);
         }
     }
+    return true;
+}
+
+}  // namespace rv
+
+extern "C" {
+
+int RecviewSinogramDevice(const unsigned short* d_frames, int nframes, int mult, int W, const char* binc,
+                          const float* fexp, const float* fdeg, unsigned flags, int drStart, int drEnd,
+                          double drX, double drY, int drOmit, const int* piDrift, int is8bit, int start,
+                          int step, int binning, int out_float, void* d_out, rv_stream_t stream)
+{
+    if (!d_frames || !d_out || !binc || !fexp || !fdeg || nframes < 2 || mult <= 0 || W <= 0 || step <= 0 ||
+        binning <= 0) {
+        rv::set_message("RecviewSinogramDevice: bad arguments");
+        return -1;
+    }
+    rv::SinoWalk walk{};
+    walk.nframes = nframes; walk.mult = mult; walk.binc = binc; walk.fexp = fexp; walk.fdeg = fdeg;
+    walk.flags = flags; walk.drStart = drStart; walk.drEnd = drEnd; walk.drOmit = drOmit;
+    walk.drX = drX; walk.drY = drY; walk.piDrift = piDrift;
+    walk.start = start; walk.step = step; walk.binning = binning;
+    std::vector<rv::SinoJob> jobs;
+    const bool good_walk = rv::sinogram_jobs(walk, jobs);
+    if (!jobs.empty()) {
+        hipStream_t s = (hipStream_t)stream;
+        rv::SinoJob* d_jobs = nullptr;
+        bool good = rv::hip_ok(hipMalloc(&d_jobs, jobs.size() * sizeof(rv::SinoJob)), "hipMalloc") &&
+                    rv::hip_ok(hipMemcpyAsync(d_jobs, jobs.data(), jobs.size() * sizeof(rv::SinoJob),
+                                              hipMemcpyHostToDevice, s), "H2D");
+        if (good) {
+            rv::launch_sinogram(d_frames, nframes, mult, W, d_jobs, (int)jobs.size(), d_out, is8bit ? 3 : 20,
+                                (flags & RQFLAGS_ZERNIKE) ? 1 : 0, out_float ? 1 : 0, 1, s);
+            good = rv::hip_ok(hipGetLastError(), "sinogram kernel") && rv::hip_ok(hipStreamSynchronize(s), "sync");
+        }
+        if (d_jobs) (void)hipFree(d_jobs);
+        if (!good) return -1;
+    }
+    if (!good_walk) {
+        rv::set_message("RecviewSinogramDevice: no incident frame for a sample frame");
+        return -2;
+    }
+    return (int)jobs.size();
+}
+
+void CudaSinogramHost(RECONST_INFO* ri, int idev, bool)
+{
+    // Production semantics of GenerateSinogramThread (general.cpp:2377-2532): the
+    // frame walk (rv::sinogram_jobs) is sequential and runs on the host; the
+    // per-sample work runs in one kernel launch over every converted row.  Rows
+    // are converted in place.
+    if (!ri || !ri->iSinogr || !ri->bInc || !ri->fexp || !ri->fdeg || ri->ixdim <= 0 ||
+        ri->iMultiplex <= 0 || ri->maxSinogrLen < 2 || ri->iStepSino <= 0) {
+        rv::set_message("CudaSinogramHost: incomplete RECONST_INFO");
+        rv::log_code(28901);
+        if (ri) ri->iStatus = RECONST_INFO_ERROR;
+        return;
+    }
+    const int isino = ri->maxSinogrLen, W = ri->ixdim, M = ri->iMultiplex;
+    rv::SinoWalk walk{};
+    walk.nframes = isino;
+    walk.mult = M;
+    walk.binc = ri->bInc;
+    walk.fexp = ri->fexp;
+    walk.fdeg = ri->fdeg;
+    walk.flags = ri->dReconFlags;
+    walk.drStart = ri->drStart; walk.drEnd = ri->drEnd; walk.drOmit = ri->drOmit;
+    walk.drX = ri->drX; walk.drY = ri->drY;
+    walk.piDrift = ri->piDrift;
+    walk.start = ri->iStartSino;
+    walk.step = ri->iStepSino;
+    walk.binning = ri->iInterpolation == 0 ? 4 : (ri->iInterpolation == 1 ? 2 : 1);
+    std::vector<rv::SinoJob> jobs;
+    const bool error = !rv::sinogram_jobs(walk, jobs);
     if (!jobs.empty()) {
         if (!rv::hip_ok(hipSetDevice(idev), "hipSetDevice")) { ri->iStatus = RECONST_INFO_ERROR; return; }
         const size_t nrow = (size_t)isino * M, frame_bytes = nrow * W * sizeof(uint16_t);
@@ -405,7 +468,7 @@ void CudaSinogramHost(RECONST_INFO* ri, int idev, bool)
                                              hipMemcpyHostToDevice, st), "H2D");
             if (good) {
                 rv::launch_sinogram(d_frames, isino, M, W, d_jobs, (int)jobs.size(), d_out, pixmin,
-                                    (ri->dReconFlags & RQFLAGS_ZERNIKE) ? 1 : 0, st);
+                                    (ri->dReconFlags & RQFLAGS_ZERNIKE) ? 1 : 0, 0, 0, st);
                 good = rv::hip_ok(hipGetLastError(), "sinogram kernel") &&
                        rv::hip_ok(hipMemcpyAsync(h_out, d_out, out_bytes, hipMemcpyDeviceToHost, st), "D2H") &&
                        rv::hip_ok(hipStreamSynchronize(st), "sync");

@@ -19,6 +19,9 @@
 #include <hip/hip_fp16.h>
 #include <hip/hip_bf16.h>
 
+#include <map>
+#include <mutex>
+
 namespace rv {
 
 constexpr int kMaxBfly = 8;  // butterflies per thread and stage (n/2 <= 8 * blockDim)
@@ -400,9 +403,14 @@ __global__ void __launch_bounds__(1024) filter_pair_kernel(FilterJob J)
 // raised once, outside any stream capture.
 void prepare_filter(int n)
 {
-    static int done = 0;
+    // per device: the attribute applies to the current device's copy of the kernels
+    static std::mutex mu;
+    static std::map<int, int> done;
     const size_t lds = sizeof(float2) * (size_t)n;
-    if (lds <= 65536 || n <= done) return;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    std::lock_guard<std::mutex> lk(mu);
+    if (lds <= 65536 || n <= done[dev]) return;
     (void)hipFuncSetAttribute((const void*)&filter_rows_kernel<0>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     (void)hipFuncSetAttribute((const void*)&filter_rows_kernel<1>,
@@ -411,7 +419,7 @@ void prepare_filter(int n)
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     (void)hipFuncSetAttribute((const void*)&filter_pair_kernel<1>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    done = n;
+    done[dev] = n;
 }
 
 void launch_filter(const FilterJob& job, hipStream_t stream)

@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 namespace rv {
 
 // ------------------------------------------------------------- filter stage
@@ -52,7 +54,24 @@ struct SinoJob {
     int shift, action, pad;
 };
 void launch_sinogram(const uint16_t* frames, int nframes, int mult, int W, const SinoJob* jobs,
-                     int njobs, int16_t* out, int pixmin, int zernike, hipStream_t stream);
+                     int njobs, void* out, int pixmin, int zernike, int out_float, int out_full,
+                     hipStream_t stream);
+// The sequential frame walk of GenerateSinogramThread (general.cpp:2394-2466):
+// incident frames, exposure weights, drift shifts, SKIP / drift-omit zeroing.
+// Returns false where the reference sets RECONST_INFO_ERROR (jobs before the
+// failing frame are kept, as the reference converts them).
+struct SinoWalk {
+    int nframes, mult;          // frames include the dark frame (last)
+    const char* binc;
+    const float* fexp;
+    const float* fdeg;
+    unsigned flags;             // RQFLAGS_*
+    int drStart, drEnd, drOmit;
+    double drX, drY;
+    const int* piDrift;
+    int start, step, binning;
+};
+bool sinogram_jobs(const SinoWalk& w, std::vector<SinoJob>& jobs);
 
 // ------------------------------------------------------- output stage
 // Min/max + normalisation to 0..65534 inside the circle of squared radius

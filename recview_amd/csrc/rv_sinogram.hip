@@ -8,13 +8,19 @@
 
 namespace rv {
 
+// out_float = 0: int16 rows as the reference; 1: float rows, -ln(I/I0) (or the
+// Zernike ratio) without the x3000 scale and rounding.  out_full = 1 writes row
+// frame * mult + k (the frames' layout), 0 writes row j of the job list.
 __global__ void sinogram_kernel(const uint16_t* frames, int nframes, int mult, int W,
-                                const SinoJob* jobs, int16_t* out, int pixmin, int zernike)
+                                const SinoJob* jobs, void* out, int pixmin, int zernike, int out_float,
+                                int out_full)
 {
     const SinoJob J = jobs[blockIdx.y];
     const int j = blockIdx.x * 256 + threadIdx.x;
     if (j >= W) return;
+    const size_t orow = out_full ? (size_t)J.frame * mult + J.k : (size_t)blockIdx.y;
     int16_t v = 0;
+    float fv = 0.f;
     const int src = j - J.shift;   // general.cpp:2501-2513: out[j] = row[j - shift], zero fill
     if (J.action == 1 && src >= 0 && src < W) {
         const uint16_t* dark = frames + ((size_t)(nframes - 1) * mult + J.k) * W;
@@ -27,20 +33,28 @@ __global__ void sinogram_kernel(const uint16_t* frames, int nframes, int mult, i
         const double inc = (double)(fb - (float)dark[src]);
         if (inc > 0) {
             const int is = (int)row[src] - (int)dark[src];
-            if (is >= pixmin)
-                v = zernike ? (int16_t)(int)(inc * 1000 / is + 0.5)
-                            : (int16_t)(int)(log(inc / is) * 3000 + 0.5);
+            if (is >= pixmin) {
+                if (out_float)
+                    fv = zernike ? (float)(inc / is) : (float)log(inc / is);
+                else
+                    v = zernike ? (int16_t)(int)(inc * 1000 / is + 0.5)
+                                : (int16_t)(int)(log(inc / is) * 3000 + 0.5);
+            }
         }
     }
-    out[(size_t)blockIdx.y * W + j] = v;
+    if (out_float)
+        ((float*)out)[orow * W + j] = fv;
+    else
+        ((int16_t*)out)[orow * W + j] = v;
 }
 
 void launch_sinogram(const uint16_t* frames, int nframes, int mult, int W, const SinoJob* jobs,
-                     int njobs, int16_t* out, int pixmin, int zernike, hipStream_t stream)
+                     int njobs, void* out, int pixmin, int zernike, int out_float, int out_full,
+                     hipStream_t stream)
 {
     if (njobs <= 0) return;
     hipLaunchKernelGGL(sinogram_kernel, dim3((W + 255) / 256, njobs), dim3(256), 0, stream, frames,
-                       nframes, mult, W, jobs, out, pixmin, zernike);
+                       nframes, mult, W, jobs, out, pixmin, zernike, out_float, out_full);
 }
 
 }  // namespace rv

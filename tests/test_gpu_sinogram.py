@@ -67,3 +67,44 @@ def test_sinogram_error_without_incident_frame():
     binc[:] = 1
     got, st = sinogram.generate(frames, binc, fexp, fdeg)
     assert st == 2      # RECONST_INFO_ERROR
+
+
+@pytest.mark.parametrize("case", ["plain", "drift_list", "skip_nouse"])
+def test_device_frames_int16_equals_host(case):
+    import torch
+    frames, binc, fexp, fdeg = _scan(31 + len(case))
+    kw = {}
+    if case == "drift_list":
+        kw = {"flags": 4, "pidrift": np.random.default_rng(5).integers(-9, 10, len(binc))}
+    elif case == "skip_nouse":
+        binc[4] |= 4
+        binc[13] |= 2
+    host, st = sinogram.generate(frames, binc, fexp, fdeg, **kw)
+    d = torch.from_numpy(frames.view(np.int16)).cuda()
+    dev, n = sinogram.generate_device(d, binc, fexp, fdeg, out_float=False, **kw)
+    torch.cuda.synchronize()
+    dev = dev.cpu().numpy()
+    conv = np.zeros(frames.shape[0], bool)
+    conv[:-1] = (binc[:-1] & 1) != 0
+    assert n == conv.sum() * frames.shape[1]
+    assert np.array_equal(dev[conv], host[conv])
+
+
+def test_device_frames_float_rows():
+    """Float rows are the unquantised line integrals of the int16 rows."""
+    import torch
+    frames, binc, fexp, fdeg = _scan(41)
+    d = torch.from_numpy(frames.view(np.int16)).cuda()
+    f, _ = sinogram.generate_device(d, binc, fexp, fdeg, out_float=True)
+    i16, _ = sinogram.generate_device(d, binc, fexp, fdeg, out_float=False)
+    torch.cuda.synchronize()
+    f, i16 = f.cpu().numpy(), i16.cpu().numpy().astype(np.float64)
+    conv = (binc[:-1] & 1) != 0
+    ff = f[:-1][conv].astype(np.float64) * 3000
+    q = i16[:-1][conv]
+    # int16 = (short)(3000 p + 0.5): truncation toward zero, also for negative p;
+    # the float rows round p to FP32 first, so a borderline value may land one off
+    expect = np.trunc(ff + 0.5)
+    assert np.abs(expect - q).max() <= 1
+    assert np.mean(expect != q) < 1e-3
+    assert (np.abs(f[:-1][~conv]) == 0).all()                 # incident / dark rows untouched

@@ -187,6 +187,13 @@ void CudaSinoPx2igpStream(int ixdimp, int* d_igp, float* d_px, rv_stream_t strea
 /* cudaReconst.h:58-59 (cudaReconstHost.cpp:216-218): pinned host memory. */
 rv_error_t CUDA_MALLOC_HOST_INT(int** ptr, size_t size);
 rv_error_t CUDA_FREE_HOST(void* ptr);
+/* Device memory and synchronous copies for hosts that include only this
+ * header (no counterpart in the reference, which calls the CUDA runtime
+ * directly): hipMalloc / hipFree / hipMemcpy on the current device. */
+rv_error_t RecviewDeviceAlloc(void** ptr, size_t bytes);
+rv_error_t RecviewDeviceFree(void* ptr);
+rv_error_t RecviewCopyToHost(void* dst, const void* src, size_t bytes);
+rv_error_t RecviewCopyToDevice(void* dst, const void* src, size_t bytes);
 
 /* cudaReconst.cu:19: block size of the compatibility kernels, written by the
  * application (gazo.cpp:549-550).  The tiled kernels use their own shapes. */

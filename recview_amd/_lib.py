@@ -75,6 +75,7 @@ EXPORTS = [
     "rvUnetCreate", "rvUnetParamCount", "rvUnetGetParams", "rvUnetSetParams", "rvUnetForward",
     "rvUnetLossGrad", "rvUnetTrainStep", "rvUnetDestroy", "rvConv3x3Bf16", "rvConv3x3WgradBf16",
     "RecviewOutputStage", "RecviewImageDescription", "RecviewWriteTif16", "RecviewSinogramDevice",
+    "RecviewDeviceAlloc", "RecviewDeviceFree", "RecviewCopyToHost", "RecviewCopyToDevice",
 ]
 
 

@@ -7,6 +7,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "librecview_hip.so")
+HARNESS = os.path.join(HERE, "recview_harness")
 
 HOST_SOURCES = ["rv_common.cpp", "rv_compat.cpp", "rv_plan.cpp", "rv_offset.cpp", "rv_unet.cpp", "rv_output.cpp"]
 DEVICE_SOURCES = ["rv_filter.hip", "rv_bp.hip", "rv_stitch.hip", "rv_unet.hip", "rv_sinogram.hip", "rv_output.hip"]
@@ -47,6 +48,11 @@ def build(verbose=False):
     tmp = OUT + ".tmp"
     subprocess.run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs, check=True)
     os.replace(tmp, OUT)
+    # A plain C++ host linked against the C ABI only (no HIP headers): the way
+    # RecView's own code would use the library.
+    subprocess.run(["g++", "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"),
+                    os.path.join(HERE, "host", "recview_harness.cpp"), "-L" + HERE, "-lrecview_hip",
+                    "-Wl,-rpath,$ORIGIN", "-o", HARNESS], check=True)
     return OUT
 
 

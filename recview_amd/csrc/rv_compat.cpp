@@ -321,6 +321,17 @@ rv_error_t CUDA_MALLOC_HOST_INT(int** ptr, size_t size)
 
 rv_error_t CUDA_FREE_HOST(void* ptr) { return (rv_error_t)hipHostFree(ptr); }
 
+rv_error_t RecviewDeviceAlloc(void** ptr, size_t bytes) { return (rv_error_t)hipMalloc(ptr, bytes); }
+rv_error_t RecviewDeviceFree(void* ptr) { return (rv_error_t)hipFree(ptr); }
+rv_error_t RecviewCopyToHost(void* dst, const void* src, size_t bytes)
+{
+    return (rv_error_t)hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
+}
+rv_error_t RecviewCopyToDevice(void* dst, const void* src, size_t bytes)
+{
+    return (rv_error_t)hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+}
+
 }  // extern "C"
 
 namespace rv {

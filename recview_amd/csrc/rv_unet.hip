@@ -189,6 +189,150 @@ __global__ __launch_bounds__(256, 2) void conv3x3_kernel(ConvArgs a) {
     }
 }
 
+// 32-channel inputs (one chunk): a block walks PPB consecutive 16 x 16
+// patches with the weights staged once.  The next patch's halo is fetched into
+// registers while the current one is multiplied; the output tile has its own
+// LDS so the weights survive the epilogue.
+template <int NF, int PPB>
+__global__ __launch_bounds__(256, 2) void conv3x3_c32_kernel(ConvArgs a) {
+    constexpr int HSZ = HALO_PIX * HALO_PITCH;
+    constexpr int WSZ = NF * 576 * 8;
+    constexpr int TC = 16 * NF;
+    constexpr int OPITCH = TC + 8;
+    constexpr int W_ITEMS = NF * 576;
+    constexpr int W_PER = (W_ITEMS + 255) / 256;
+    __shared__ __attribute__((aligned(16))) uint16_t smem[HSZ + WSZ + 256 * OPITCH];
+    uint16_t* in_s = smem;
+    uint16_t* w_s = smem + HSZ;
+    uint16_t* o_s = w_s + WSZ;
+    const int tiles_x = (a.W + 15) >> 4;
+    const int ntiles = tiles_x * ((a.H + CONV_ROWS - 1) / CONV_ROWS);
+    const int tile0 = blockIdx.x * PPB;
+    const int npatch = ntiles - tile0 < PPB ? ntiles - tile0 : PPB;
+    const int bb = blockIdx.z, f0 = blockIdx.y * NF;
+    const int t = threadIdx.x, l = t & 63, w = t >> 6;
+    const uint16_t* in = a.in + (long long)bb * a.H * a.W * a.in_stride + a.in_off;
+    const uint16_t* up = a.up ? a.up + (long long)bb * (a.H >> 1) * (a.W >> 1) * a.Cup : nullptr;
+
+#pragma unroll
+    for (int k = 0; k < W_PER; k++) {
+        const int i = t + 256 * k;
+        if (i < W_ITEMS) {
+            const int f = i / 576, r = i - f * 576;
+            st16(w_s + i * 8, ld16(a.wfrag + ((long long)(f0 + f) * 576 + r) * 8));
+        }
+    }
+    uint4 hreg[HALO_PER];
+    auto fetch = [&](int p) {
+        const int tile = tile0 + p;
+        const int x0 = (tile % tiles_x) * 16, y0 = (tile / tiles_x) * CONV_ROWS;
+#pragma unroll
+        for (int k = 0; k < HALO_PER; k++) {
+            const int i = t + 256 * k;
+            const int hp = i >> 2, g = i & 3;
+            const int hy = hp / 18, hx = hp - hy * 18;
+            const int y = y0 - 1 + hy, x = x0 - 1 + hx;
+            hreg[k] = make_uint4(0, 0, 0, 0);
+            if (i < HALO_ITEMS && y >= 0 && y < a.H && x >= 0 && x < a.W) {
+                if (a.Cup > 0)
+                    hreg[k] = ld16(up + ((long long)(y >> 1) * (a.W >> 1) + (x >> 1)) * a.Cup + g * 8);
+                else
+                    hreg[k] = ld16(in + ((long long)y * a.W + x) * a.in_stride + g * 8);
+            }
+        }
+    };
+    fetch(0);
+    for (int p = 0; p < npatch; p++) {
+        const int tile = tile0 + p;
+        const int x0 = (tile % tiles_x) * 16, y0 = (tile / tiles_x) * CONV_ROWS;
+#pragma unroll
+        for (int k = 0; k < HALO_PER; k++) {
+            const int i = t + 256 * k;
+            if (i < HALO_ITEMS) st16(in_s + (i >> 2) * HALO_PITCH + (i & 3) * 8, hreg[k]);
+        }
+        __syncthreads();
+        if (p + 1 < npatch) fetch(p + 1);
+        floatx4 acc[4][NF];
+#pragma unroll
+        for (int m = 0; m < 4; m++)
+#pragma unroll
+            for (int f = 0; f < NF; f++) acc[m][f] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int tap = 0; tap < 9; tap++) {
+            const int ky = tap / 3, kx = tap % 3;
+            bf16x8 av[4];
+#pragma unroll
+            for (int m = 0; m < 4; m++)
+                av[m] = *(const bf16x8*)(in_s + ((4 * w + m + ky) * 18 + (l & 15) + kx) * HALO_PITCH +
+                                         8 * (l >> 4));
+#pragma unroll
+            for (int f = 0; f < NF; f++) {
+                const bf16x8 bv = *(const bf16x8*)(w_s + ((f * 9 + tap) * 64 + l) * 8);
+#pragma unroll
+                for (int m = 0; m < 4; m++)
+                    acc[m][f] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[m], bv, acc[m][f], 0, 0, 0);
+            }
+        }
+        // bias, ReLU, bf16 into the output tile [256 pixels][TC]
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+            const int co = f * 16 + (l & 15);
+            const float bias = a.bias ? a.bias[(f0 * 16) + co] : 0.f;
+#pragma unroll
+            for (int m = 0; m < 4; m++)
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    float v = acc[m][f][i] + bias;
+                    if (a.relu) v = fmaxf(v, 0.f);
+                    o_s[((4 * w + m) * 16 + 4 * (l >> 4) + i) * OPITCH + co] = f2bf(v);
+                }
+        }
+        __syncthreads();
+        if (a.pool_out) {
+            const int Ho = a.H >> 1, Wo = a.W >> 1;
+            for (int it = t; it < 64 * (TC / 8); it += 256) {
+                const int q = it / (TC / 8), g = it - q * (TC / 8);
+                const int py = q >> 3, pxx = q & 7;
+                const int yo = (y0 >> 1) + py, xo = (x0 >> 1) + pxx;
+                if (yo >= Ho || xo >= Wo) continue;
+                float sum[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const int px = (2 * py + (k >> 1)) * 16 + 2 * pxx + (k & 1);
+                    const uint4 v = ld16(o_s + px * OPITCH + 8 * g);
+                    const uint16_t* v8 = (const uint16_t*)&v;
+#pragma unroll
+                    for (int e = 0; e < 8; e++) sum[e] += bf2f(v8[e]);
+                }
+                uint4 r;
+                uint16_t* r8 = (uint16_t*)&r;
+#pragma unroll
+                for (int e = 0; e < 8; e++) r8[e] = f2bf(0.25f * sum[e]);
+                st16(a.pool_out + (((long long)bb * Ho + yo) * Wo + xo) * a.Cout + f0 * 16 + 8 * g, r);
+            }
+        }
+        for (int it = t; it < 256 * (TC / 8); it += 256) {
+            const int px = it / (TC / 8), g = it - px * (TC / 8);
+            const int y = y0 + (px >> 4), x = x0 + (px & 15);
+            if (y >= a.H || x >= a.W) continue;
+            const long long pix = ((long long)bb * a.H + y) * a.W + x;
+            const int c = f0 * 16 + 8 * g;
+            uint4 v = ld16(o_s + px * OPITCH + 8 * g);
+            if (a.mask) {
+                const uint4 mv = ld16(a.mask + pix * a.mask_stride + a.mask_off + c);
+                uint16_t* v8 = (uint16_t*)&v;
+                const uint16_t* m8 = (const uint16_t*)&mv;
+#pragma unroll
+                for (int e = 0; e < 8; e++)
+                    if (!(bf2f(m8[e]) > 0.f)) v8[e] = 0;
+            }
+            st16(a.out + pix * a.out_stride + a.out_off + c, v);
+        }
+        // the next iteration's halo store and output-tile writes are each
+        // separated from this iteration's reads by one barrier
+    }
+}
+
 hipError_t launch_conv3x3(const ConvArgs& a, hipStream_t s) {
     if (a.Cin % 32 || a.Cout % 32 || a.in_stride % 8 || a.in_off % 8 || a.out_stride % 8 ||
         a.out_off % 8 || a.B <= 0 || a.H <= 0 || a.W <= 0 || a.in_off + a.Cin > a.in_stride ||
@@ -198,10 +342,14 @@ hipError_t launch_conv3x3(const ConvArgs& a, hipStream_t s) {
         (a.pool_out && (a.H % 2 || a.W % 2)))
         return hipErrorInvalidValue;
     const int tiles = ((a.W + 15) / 16) * ((a.H + CONV_ROWS - 1) / CONV_ROWS);
+    // 32 -> 32 layers (one chunk, 64 output channels of work per 16 KB of halo)
+    // walk four patches per block with the weights staged once.
     if (a.Cout % 64 == 0)
-        hipLaunchKernelGGL(conv3x3_kernel<4>, dim3(tiles, a.Cout / 64, a.B), dim3(256), 0, s, a);
+        hipLaunchKernelGGL((conv3x3_kernel<4>), dim3(tiles, a.Cout / 64, a.B), dim3(256), 0, s, a);
+    else if (a.Cin == 32)
+        hipLaunchKernelGGL((conv3x3_c32_kernel<2, 4>), dim3((tiles + 3) / 4, a.Cout / 32, a.B), dim3(256), 0, s, a);
     else
-        hipLaunchKernelGGL(conv3x3_kernel<2>, dim3(tiles, a.Cout / 32, a.B), dim3(256), 0, s, a);
+        hipLaunchKernelGGL((conv3x3_kernel<2>), dim3(tiles, a.Cout / 32, a.B), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

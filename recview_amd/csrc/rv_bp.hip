@@ -20,6 +20,10 @@
 //                d_j = p[k+1]-p[k] and a_j = p[k] - j*d_j (j = sample index
 //                within the segment), so value(u) = a_j + u*d_j for the
 //                segment-relative coordinate u: one FMA, no fract.  8 B/update.
+//  BP_LINEAR2  : BP_LINEAR for two slices of identical geometry at once: LDS
+//                holds (a_j, d_j, a'_j, d'_j), so one coordinate and one
+//                16-byte LDS read (ds_read_b128, which returns ~1.8x the bytes
+//                per clock of ds_read_b64 on gfx950) serve two updates.
 //  BP_LINEAR_H : FP16/BF16 filtered sinogram.  LDS holds FP16 pairs (p_k, d_k)
 //                rescaled to one power-of-two scale per chunk of angles, value =
 //                p + fract(u)*d (mixed-precision FMA), FP32 sums.  4 B/update.
@@ -49,21 +53,46 @@ struct AccT { using type = float; };
 template <>
 struct AccT<BP_LUT_I32> { using type = int; };
 
-constexpr bool is_linear(int mode) { return mode == BP_LINEAR || mode == BP_LINEAR_H; }
+constexpr bool is_linear(int mode) { return mode == BP_LINEAR || mode == BP_LINEAR_H || mode == BP_LINEAR2; }
 
 // LDS bytes per staged angle
 constexpr int entry_bytes(int mode, int seg)
 {
-    return mode == BP_LINEAR ? seg * 8 : (mode == BP_LINEAR_H ? seg * 4 : seg * 16 * 4);
+    return mode == BP_LINEAR2 ? seg * 16 : mode == BP_LINEAR ? seg * 8 : (mode == BP_LINEAR_H ? seg * 4 : seg * 16 * 4);
+}
+
+__device__ __forceinline__ float load_row_val(const void* rows, const float* scales, const BpSlice& S, int row,
+                                              int k)
+{
+    const long long off = (long long)row * S.row_stride + k;
+    if (S.store == 0) return ((const float*)rows)[off];
+    const float sc = scales[row];
+    if (S.store == 1) return __half2float(((const __half*)rows)[off]) * sc;
+    return __bfloat162float(((const __hip_bfloat16*)rows)[off]) * sc;
 }
 
 __device__ __forceinline__ float load_row_val(const BpSlice& S, int row, int k)
 {
-    const long long off = (long long)row * S.row_stride + k;
-    if (S.store == 0) return ((const float*)S.rows)[off];
-    const float sc = S.scales[row];
-    if (S.store == 1) return __half2float(((const __half*)S.rows)[off]) * sc;
-    return __bfloat162float(((const __hip_bfloat16*)S.rows)[off]) * sc;
+    return load_row_val(S.rows, S.scales, S, row, k);
+}
+
+// (a_j, d_j) of the staged segment: value(u) = a + u*d for u in [j, j+1)
+__device__ __forceinline__ float2 linear_pair(const void* rows, const float* scales, const BpSlice& S, int row,
+                                              int k, int j)
+{
+    const int N = S.row_len;
+    float av = 0.f, dv = 0.f;
+    if (row >= 0 && k >= 0 && k < N) {
+        const float p0 = load_row_val(rows, scales, S, row, k);
+        if (k < N - 1) {
+            const float p1 = load_row_val(rows, scales, S, row, k + 1);
+            dv = p1 - p0;
+            av = fmaf(-(float)j, dv, p0);
+        } else {
+            av = p0;
+        }
+    }
+    return make_float2(av, dv);
 }
 
 template <int MODE, int PX, int PY>
@@ -72,7 +101,8 @@ __global__ void __launch_bounds__(256) bp_tiled_kernel(BpLaunch L)
     using C = TileCfg<PX, PY>;
     using Acc = typename AccT<MODE>::type;
     constexpr int SEG = C::SEG;
-    constexpr int EPA = (MODE == BP_LINEAR || MODE == BP_LINEAR_H) ? SEG : 16 * SEG;   // entries per angle
+    constexpr int EPA = is_linear(MODE) ? SEG : 16 * SEG;   // entries per angle
+    constexpr int NS = MODE == BP_LINEAR2 ? 2 : 1;          // slices per work item
     const int chunk = L.chunk;
 
     // LDS layout (pointer arithmetic only, so every access stays a ds_* op):
@@ -119,11 +149,17 @@ __global__ void __launch_bounds__(256) bp_tiled_kernel(BpLaunch L)
         const int Y0 = (t / S.tiles_x) * C::TH;
         const int N = S.row_len;
 
-        Acc acc[PY][PX];
+        Acc acc[PY][PX], acc2[NS == 2 ? PY : 1][NS == 2 ? PX : 1];
 #pragma unroll
         for (int j = 0; j < PY; j++)
 #pragma unroll
             for (int i = 0; i < PX; i++) acc[j][i] = (Acc)0;
+        if (NS == 2) {
+#pragma unroll
+            for (int j = 0; j < PY; j++)
+#pragma unroll
+                for (int i = 0; i < PX; i++) acc2[j][i] = (Acc)0;
+        }
 
         for (int a0 = 0; a0 < S.nang; a0 += chunk) {
             // ---- per-angle tile geometry
@@ -169,20 +205,15 @@ __global__ void __launch_bounds__(256) bp_tiled_kernel(BpLaunch L)
                 float2* pairs = (float2*)data;
                 for (int idx = tid; idx < chunk * SEG; idx += C::T) {
                     const int a = idx / SEG, j = idx - a * SEG;
-                    const int row = rws[a];
-                    const int k = s0s[a] + j;
-                    float av = 0.f, dv = 0.f;
-                    if (row >= 0 && k >= 0 && k < N) {
-                        const float p0 = load_row_val(S, row, k);
-                        if (k < N - 1) {
-                            const float p1 = load_row_val(S, row, k + 1);
-                            dv = p1 - p0;
-                            av = fmaf(-(float)j, dv, p0);
-                        } else {
-                            av = p0;
-                        }
-                    }
-                    pairs[idx] = make_float2(av, dv);
+                    pairs[idx] = linear_pair(S.rows, S.scales, S, rws[a], s0s[a] + j, j);
+                }
+            } else if (MODE == BP_LINEAR2) {
+                float4* quads = (float4*)data;
+                for (int idx = tid; idx < chunk * SEG; idx += C::T) {
+                    const int a = idx / SEG, j = idx - a * SEG;
+                    const float2 q0 = linear_pair(S.rows, S.scales, S, rws[a], s0s[a] + j, j);
+                    const float2 q1 = linear_pair(S.rows2, S.scales2, S, rws[a], s0s[a] + j, j);
+                    quads[idx] = make_float4(q0.x, q0.y, q1.x, q1.y);
                 }
             } else if (MODE == BP_LINEAR_H) {
                 __half2* pairs = (__half2*)data;
@@ -254,6 +285,24 @@ __global__ void __launch_bounds__(256) bp_tiled_kernel(BpLaunch L)
                             const float u = (i == 0) ? bj : fmaf((float)(8 * i), tb.x, bj);
                             const float2 ad = seg[(int)u];
                             acc[j][i] += fmaf(u, ad.y, ad.x);
+                        }
+                    }
+                }
+            } else if (MODE == BP_LINEAR2) {
+                const float4* quads = (const float4*)data;
+                for (int a = 0; a < chunk; a++) {
+                    const float4 tb = tab[a];
+                    const float4* seg = quads + a * SEG;
+                    const float base = fmaf(dyl, tb.y, fmaf(dxl, tb.x, tb.z));
+#pragma unroll
+                    for (int j = 0; j < PY; j++) {
+                        const float bj = (j == 0) ? base : fmaf((float)(8 * j), tb.y, base);
+#pragma unroll
+                        for (int i = 0; i < PX; i++) {
+                            const float u = (i == 0) ? bj : fmaf((float)(8 * i), tb.x, bj);
+                            const float4 q = seg[(int)u];
+                            acc[j][i] += fmaf(u, q.y, q.x);
+                            acc2[NS == 2 ? j : 0][NS == 2 ? i : 0] += fmaf(u, q.w, q.z);
                         }
                     }
                 }
@@ -330,6 +379,11 @@ __global__ void __launch_bounds__(256) bp_tiled_kernel(BpLaunch L)
                     float* out = (float*)S.out;
                     const float v = (float)acc[j][i] * S.out_scale;
                     out[o] = S.accumulate ? out[o] + v : v;
+                    if (NS == 2) {
+                        float* out2 = (float*)S.out2;
+                        const float v2 = (float)acc2[NS == 2 ? j : 0][NS == 2 ? i : 0] * S.out_scale;
+                        out2[o] = S.accumulate ? out2[o] + v2 : v2;
+                    }
                 }
             }
         }
@@ -371,6 +425,7 @@ static int tile_seg(int tile)
 int bp_default_chunk(int mode, int tile)
 {
     if (mode == BP_LINEAR) return (tile == 2) ? 64 : 32;
+    if (mode == BP_LINEAR2) return (tile == 2) ? 32 : 16;
     if (mode == BP_LINEAR_H) return (tile == 2) ? 64 : 32;
     return (tile == 2) ? 16 : 8;
 }
@@ -398,6 +453,7 @@ static const void* kernel_ptr(int tile)
 static const void* kernel_for(int mode, int tile)
 {
     if (mode == BP_LINEAR) return kernel_ptr<BP_LINEAR>(tile);
+    if (mode == BP_LINEAR2) return kernel_ptr<BP_LINEAR2>(tile);
     if (mode == BP_LINEAR_H) return kernel_ptr<BP_LINEAR_H>(tile);
     if (mode == BP_LUT_F32) return kernel_ptr<BP_LUT_F32>(tile);
     return kernel_ptr<BP_LUT_I32>(tile);
@@ -443,6 +499,7 @@ static void launch_mode(const BpLaunch& L, int grid, hipStream_t stream)
 void launch_bp(const BpLaunch& L, int grid, hipStream_t stream)
 {
     if (L.mode == BP_LINEAR) launch_mode<BP_LINEAR>(L, grid, stream);
+    else if (L.mode == BP_LINEAR2) launch_mode<BP_LINEAR2>(L, grid, stream);
     else if (L.mode == BP_LINEAR_H) launch_mode<BP_LINEAR_H>(L, grid, stream);
     else if (L.mode == BP_LUT_F32) launch_mode<BP_LUT_F32>(L, grid, stream);
     else launch_mode<BP_LUT_I32>(L, grid, stream);

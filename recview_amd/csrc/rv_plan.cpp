@@ -43,7 +43,9 @@ struct SliceState {
 
 struct Group {
     int tile = 0, chunk = 0;
-    std::vector<int> slices;
+    int mode = 0;                // BP mode of the group's launch
+    std::vector<int> slices;     // first slice of each work-item entry
+    std::vector<int> twins;      // BP_LINEAR2: the entry's second slice
     std::vector<rv::BpSlice> h;
     rv::BpSlice* d_slices = nullptr;
     int* d_order = nullptr;
@@ -115,6 +117,11 @@ bool upload_slices(rv_plan_t p, void* const* d_out)
             const int s = G.slices[q];
             if (p->cur_out[s] != d_out[s]) {
                 G.h[q].out = d_out[s];
+                changed = true;
+            }
+            const int s2 = G.twins[q];
+            if (s2 >= 0 && p->cur_out[s2] != d_out[s2]) {
+                G.h[q].out2 = d_out[s2];
                 changed = true;
             }
         }
@@ -193,7 +200,7 @@ int launch_backprojection(rv_plan_t p, hipStream_t st)
         L.total_items = G.total_items;
         L.item_order = G.d_order;
         L.counter = p->opt.persistent ? p->d_counter + gi : nullptr;
-        L.mode = p->mode;
+        L.mode = G.mode;
         L.rule = p->opt.index_rule ? 1 : 0;
         L.chunk = G.chunk;
         L.tile = G.tile;
@@ -315,21 +322,65 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
         }
     }
 
-    // groups by tile configuration; work items cost-sorted within each group
-    std::map<int, int> gidx;
+    // LINEAR with FP32 pairs in LDS: slices whose back-projection geometry is
+    // identical (angle table, sizes, tile, scale) share work items two by two.
+    std::vector<int> twin_of(nslices, -1);
+    std::vector<char> second(nslices, 0);
+    if (p->mode == rv::BP_LINEAR && !opt->no_slice_pairs) {
+        auto same = [&](int a, int b) {
+            const SliceState& A = p->sl[a];
+            const SliceState& B = p->sl[b];
+            return A.nang == B.nang && A.g.N == B.g.N && A.out_n == B.out_n && A.tile == B.tile &&
+                   std::memcmp(ha.data() + A.ang_off, ha.data() + B.ang_off, sizeof(rv::AngleEntry) * A.nang) == 0;
+        };
+        std::map<std::vector<long long>, std::vector<int>> open;
+        for (int s = 0; s < nslices; s++) {
+            const SliceState& S = p->sl[s];
+            // cheap key first (sizes and the first and last angle entries), full comparison on a match
+            const rv::AngleEntry& e0 = ha[S.ang_off];
+            const rv::AngleEntry& e1 = ha[S.ang_off + S.nang - 1];
+            std::vector<long long> key{S.nang, S.g.N, S.out_n, S.tile};
+            for (const double d : {e0.u0, e0.dux, e0.duy, e1.u0, e1.dux, e1.duy}) {
+                long long bits;
+                std::memcpy(&bits, &d, sizeof(bits));
+                key.push_back(bits);
+            }
+            auto& lst = open[key];
+            bool paired = false;
+            for (size_t q = 0; q < lst.size(); q++) {
+                if (same(lst[q], s)) {
+                    twin_of[lst[q]] = s;
+                    second[s] = 1;
+                    lst.erase(lst.begin() + q);
+                    paired = true;
+                    break;
+                }
+            }
+            if (!paired) lst.push_back(s);
+        }
+    }
+
+    // groups by tile configuration and slices per item; work items cost-sorted within each group
+    std::map<std::pair<int, int>, int> gidx;
     for (int s = 0; s < nslices; s++) {
-        auto it = gidx.find(p->sl[s].tile);
+        if (second[s]) continue;
+        const int mode = twin_of[s] >= 0 ? (int)rv::BP_LINEAR2 : p->mode;
+        const auto gk = std::make_pair(p->sl[s].tile, mode);
+        auto it = gidx.find(gk);
         if (it == gidx.end()) {
-            gidx[p->sl[s].tile] = (int)p->groups.size();
+            gidx[gk] = (int)p->groups.size();
             Group G;
             G.tile = p->sl[s].tile;
-            G.chunk = opt->angles_per_chunk > 0 ? opt->angles_per_chunk : rv::bp_default_chunk(p->mode, G.tile);
-            if (G.chunk > rv::bp_max_chunk(p->mode)) G.chunk = rv::bp_max_chunk(p->mode);
-            while (G.chunk > 1 && rv::bp_lds_bytes(p->mode, G.tile, G.chunk) > 160 * 1024) G.chunk /= 2;
+            G.mode = mode;
+            G.chunk = opt->angles_per_chunk > 0 ? opt->angles_per_chunk : rv::bp_default_chunk(mode, G.tile);
+            if (G.chunk > rv::bp_max_chunk(mode)) G.chunk = rv::bp_max_chunk(mode);
+            while (G.chunk > 1 && rv::bp_lds_bytes(mode, G.tile, G.chunk) > 160 * 1024) G.chunk /= 2;
             p->groups.push_back(G);
         }
-        p->sl[s].group = gidx[p->sl[s].tile];
+        p->sl[s].group = gidx[gk];
         p->groups[p->sl[s].group].slices.push_back(s);
+        p->groups[p->sl[s].group].twins.push_back(twin_of[s]);
+        if (twin_of[s] >= 0) p->sl[twin_of[s]].group = p->sl[s].group;
     }
 
     const size_t esz = (p->store == 0) ? 4 : 2;
@@ -387,6 +438,12 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
             B.lut_formula = opt->lut_formula ? 1 : 0;
             B.store = p->store;
             B.out_scale = opt->normalize ? (float)(6.283185307179586 / S.nang) : 1.f;
+            const int s2 = G.twins[q];
+            if (s2 >= 0) {
+                const SliceState& S2 = p->sl[s2];
+                B.rows2 = (const unsigned char*)p->d_rows + S2.rows_off * esz;
+                B.scales2 = (p->store == 0) ? nullptr : p->d_scales + S2.ang_off;
+            }
         }
         if (!rv::hip_ok(hipMalloc(&G.d_slices, sizeof(rv::BpSlice) * G.h.size()), "slice table") ||
             !rv::hip_ok(hipMalloc(&G.d_order, sizeof(int) * std::max<size_t>(order.size(), 1)), "work list") ||
@@ -394,9 +451,9 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
             !rv::hip_ok(hipMemcpy(G.d_slices, G.h.data(), sizeof(rv::BpSlice) * G.h.size(), hipMemcpyHostToDevice),
                         "slices"))
             return fail(p, std::string("plan upload failed: ") + RecviewLastMessage());
-        (void)rv::bp_blocks_per_cu(p->mode, G.tile, G.chunk);     // sets LDS attributes outside capture
+        (void)rv::bp_blocks_per_cu(G.mode, G.tile, G.chunk);     // sets LDS attributes outside capture
         if (opt->persistent) {
-            const int per_cu = rv::bp_blocks_per_cu(p->mode, G.tile, G.chunk);
+            const int per_cu = rv::bp_blocks_per_cu(G.mode, G.tile, G.chunk);
             G.grid = std::max(1, std::min(per_cu * rv::device_cu_count(device), G.total_items));
         } else {
             G.grid = G.total_items;

@@ -154,6 +154,31 @@ def test_plan_batch_uneven_equals_single():
             assert np.array_equal(batch[i], one)
 
 
+@pytest.mark.parametrize("store,persistent,normalize", [("f32", True, False), ("f32", False, True),
+                                                         ("f16", True, False), ("bf16", False, False)])
+def test_plan_slice_pairs_identical(store, persistent, normalize):
+    """Slices of identical geometry back-projected two per work item give exactly
+    the images of one slice per item: five equal-geometry slices (two pairs and
+    one left over) with different data, plus two slices of other geometries."""
+    print(f"case store={store} persistent={persistent} normalize={normalize}", flush=True)
+    specs, sinos = [], []
+    for i in range(5):
+        s, fdeg, c = _sino(128, 90, 500 + i)
+        specs.append(fbp.SliceSpec(128, fdeg, c))
+        sinos.append(torch.from_numpy(s))
+    for i, (W, P) in enumerate([(128, 91), (96, 90)]):
+        s, fdeg, c = _sino(W, P, 510 + i)
+        specs.append(fbp.SliceSpec(W, fdeg, c))
+        sinos.append(torch.from_numpy(s))
+    kw = dict(store=store, persistent=persistent, normalize=normalize)
+    _, paired = _run(specs, sinos, **kw)
+    _, single = _run(specs, sinos, slice_pairs=False, **kw)
+    for i in range(len(specs)):
+        assert np.array_equal(paired[i], single[i]), i
+    # the data of each slice stays with its own image
+    assert not np.array_equal(paired[0], paired[1])
+
+
 def test_plan_normalize_gives_attenuation():
     """normalize = 1 multiplies each image by 2 pi / P (per slice, also in a
     batch of different P), so the output is attenuation per detector sample."""

@@ -307,8 +307,8 @@ def main():
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--lds-half", type=int, default=0, help="FP16 pairs in LDS for 16-bit stores")
     ap.add_argument("--persistent", type=int, default=1)
-    ap.add_argument("--slice-pairs", type=int, default=1,
-                    help="LINEAR FP32 pairs: two slices of identical geometry per work item")
+    ap.add_argument("--slices-per-item", type=int, default=0,
+                    help="LINEAR FP32: slices of identical geometry per work item (0 = 2, or 1)")
     ap.add_argument("--graph", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -354,7 +354,7 @@ def main():
 
     plan = fbp.FbpPlan(specs, interp=args.interp, store=args.store, persistent=bool(args.persistent),
                        tile=args.tile, angles_per_chunk=args.chunk, lds_half=bool(args.lds_half),
-                       slice_pairs=bool(args.slice_pairs))
+                       slices_per_item=args.slices_per_item)
     outs = [torch.empty(plan.out_n(s), plan.out_n(s), dtype=odt, device=dev) for s in range(S)]
     stream = torch.cuda.current_stream()
     updates = float(sum(n * n * p for n, p in shapes))
@@ -407,11 +407,12 @@ def main():
 
     traffic = None
     if args.workload == "c2" and args.interp == "linear" and args.store == "f32" and S == 64 \
-            and args.proj == 1800 and args.width == 2048:
+            and args.proj == 1800 and args.width == 2048 and args.slices_per_item in (0, 2) \
+            and args.tile == 0 and args.chunk == 0:
         traffic = read_traffic("bp_linear_c2_traffic.md")
     config = {"workload": desc, "slices": S, "interp": args.interp, "store": args.store,
               "persistent": bool(args.persistent), "graph": bool(args.graph),
-              "lds_half": bool(args.lds_half), "slice_pairs": bool(args.slice_pairs), "updates": updates}
+              "lds_half": bool(args.lds_half), "slices_per_item": args.slices_per_item, "updates": updates}
     if args.workload == "c2":
         config.update({"projections": P, "detector": W, "image": W, "tile": args.tile})
     result = {

@@ -279,9 +279,10 @@ typedef struct {
     int exact_filter;    /* LINEAR modes: 1 keeps CFft's operation order (filtered rows bit-identical to
                           * the reference CPU filter); 0 (default) filters two rows per transform with
                           * the symmetrised filter, the same operator at half the cost */
-    int no_slice_pairs;  /* LINEAR with FP32 pairs in LDS: 0 (default) back-projects slices of identical
-                          * geometry two per work item (one coordinate and one 16-byte LDS read serve
-                          * both; results identical to one slice per item); 1 = one slice per item */
+    int slices_per_item; /* LINEAR with FP32 pairs in LDS: slices of identical geometry back-projected
+                          * two per work item, sharing each pixel's coordinate and 16-byte LDS reads
+                          * (results identical to one slice per item): 0 (default) or 2 = pairs,
+                          * 1 = one slice per item */
 } RvOptions;
 
 typedef struct rv_plan_opaque* rv_plan_t;

@@ -55,7 +55,7 @@ class RvOptions(C.Structure):
         ("interp", C.c_int), ("store", C.c_int), ("lut_formula", C.c_int),
         ("index_rule", C.c_int), ("persistent", C.c_int), ("tile", C.c_int),
         ("angles_per_chunk", C.c_int), ("lds_half", C.c_int), ("normalize", C.c_int),
-        ("exact_filter", C.c_int), ("no_slice_pairs", C.c_int),
+        ("exact_filter", C.c_int), ("slices_per_item", C.c_int),
     ]
 
 

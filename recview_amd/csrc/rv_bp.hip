@@ -22,8 +22,10 @@
 //                segment-relative coordinate u: one FMA, no fract.  8 B/update.
 //  BP_LINEAR2  : BP_LINEAR for two slices of identical geometry at once: LDS
 //                holds (a_j, d_j, a'_j, d'_j), so one coordinate and one
-//                16-byte LDS read (ds_read_b128, which returns ~1.8x the bytes
-//                per clock of ds_read_b64 on gfx950) serve two updates.
+//                16-byte LDS read serve two updates.  On gfx950 ds_read_b128
+//                returns ~1.7x the bytes per clock of ds_read_b64 (measured
+//                ~0.2 vs ~0.23 instructions per clock per CU), and the shared
+//                coordinate cuts VALU work from ~5.3 to ~3.6 per update.
 //  BP_LINEAR_H : FP16/BF16 filtered sinogram.  LDS holds FP16 pairs (p_k, d_k)
 //                rescaled to one power-of-two scale per chunk of angles, value =
 //                p + fract(u)*d (mixed-precision FMA), FP32 sums.  4 B/update.
@@ -53,12 +55,16 @@ struct AccT { using type = float; };
 template <>
 struct AccT<BP_LUT_I32> { using type = int; };
 
-constexpr bool is_linear(int mode) { return mode == BP_LINEAR || mode == BP_LINEAR_H || mode == BP_LINEAR2; }
+constexpr bool is_linear(int mode)
+{
+    return mode == BP_LINEAR || mode == BP_LINEAR_H || mode == BP_LINEAR2;
+}
 
 // LDS bytes per staged angle
 constexpr int entry_bytes(int mode, int seg)
 {
-    return mode == BP_LINEAR2 ? seg * 16 : mode == BP_LINEAR ? seg * 8 : (mode == BP_LINEAR_H ? seg * 4 : seg * 16 * 4);
+    return (mode == BP_LINEAR || mode == BP_LINEAR2) ? seg * 8 * bp_slices_per_item(mode)
+                                                                           : (mode == BP_LINEAR_H ? seg * 4 : seg * 16 * 4);
 }
 
 __device__ __forceinline__ float load_row_val(const void* rows, const float* scales, const BpSlice& S, int row,
@@ -102,7 +108,8 @@ __global__ void __launch_bounds__(256) bp_tiled_kernel(BpLaunch L)
     using Acc = typename AccT<MODE>::type;
     constexpr int SEG = C::SEG;
     constexpr int EPA = is_linear(MODE) ? SEG : 16 * SEG;   // entries per angle
-    constexpr int NS = MODE == BP_LINEAR2 ? 2 : 1;          // slices per work item
+    constexpr int NS = bp_slices_per_item(MODE);             // slices per work item
+    constexpr int NQ = NS / 2;                                // float4 per staged sample (multi-slice)
     const int chunk = L.chunk;
 
     // LDS layout (pointer arithmetic only, so every access stays a ds_* op):
@@ -149,17 +156,15 @@ __global__ void __launch_bounds__(256) bp_tiled_kernel(BpLaunch L)
         const int Y0 = (t / S.tiles_x) * C::TH;
         const int N = S.row_len;
 
-        Acc acc[PY][PX], acc2[NS == 2 ? PY : 1][NS == 2 ? PX : 1];
+        Acc acc[PY][PX], accx[NS > 1 ? NS - 1 : 1][PY][PX];
 #pragma unroll
         for (int j = 0; j < PY; j++)
 #pragma unroll
-            for (int i = 0; i < PX; i++) acc[j][i] = (Acc)0;
-        if (NS == 2) {
+            for (int i = 0; i < PX; i++) {
+                acc[j][i] = (Acc)0;
 #pragma unroll
-            for (int j = 0; j < PY; j++)
-#pragma unroll
-                for (int i = 0; i < PX; i++) acc2[j][i] = (Acc)0;
-        }
+                for (int e = 0; e + 1 < NS; e++) accx[e][j][i] = (Acc)0;
+            }
 
         for (int a0 = 0; a0 < S.nang; a0 += chunk) {
             // ---- per-angle tile geometry
@@ -207,13 +212,19 @@ __global__ void __launch_bounds__(256) bp_tiled_kernel(BpLaunch L)
                     const int a = idx / SEG, j = idx - a * SEG;
                     pairs[idx] = linear_pair(S.rows, S.scales, S, rws[a], s0s[a] + j, j);
                 }
-            } else if (MODE == BP_LINEAR2) {
+            } else if (NS > 1) {
                 float4* quads = (float4*)data;
                 for (int idx = tid; idx < chunk * SEG; idx += C::T) {
                     const int a = idx / SEG, j = idx - a * SEG;
-                    const float2 q0 = linear_pair(S.rows, S.scales, S, rws[a], s0s[a] + j, j);
-                    const float2 q1 = linear_pair(S.rows2, S.scales2, S, rws[a], s0s[a] + j, j);
-                    quads[idx] = make_float4(q0.x, q0.y, q1.x, q1.y);
+                    const int row = rws[a], k = s0s[a] + j;
+                    // plane q of angle a holds slices 2q, 2q+1 (16-byte stride per sample)
+#pragma unroll
+                    for (int q = 0; q < NQ; q++) {
+                        const float2 q0 = (q == 0) ? linear_pair(S.rows, S.scales, S, row, k, j)
+                                                   : linear_pair(S.rows_x[2 * q - 1], S.scales_x[2 * q - 1], S, row, k, j);
+                        const float2 q1 = linear_pair(S.rows_x[2 * q], S.scales_x[2 * q], S, row, k, j);
+                        quads[(a * NQ + q) * SEG + j] = make_float4(q0.x, q0.y, q1.x, q1.y);
+                    }
                 }
             } else if (MODE == BP_LINEAR_H) {
                 __half2* pairs = (__half2*)data;
@@ -288,11 +299,11 @@ __global__ void __launch_bounds__(256) bp_tiled_kernel(BpLaunch L)
                         }
                     }
                 }
-            } else if (MODE == BP_LINEAR2) {
+            } else if (NS > 1) {
                 const float4* quads = (const float4*)data;
                 for (int a = 0; a < chunk; a++) {
                     const float4 tb = tab[a];
-                    const float4* seg = quads + a * SEG;
+                    const float4* seg = quads + a * SEG * NQ;
                     const float base = fmaf(dyl, tb.y, fmaf(dxl, tb.x, tb.z));
 #pragma unroll
                     for (int j = 0; j < PY; j++) {
@@ -300,9 +311,14 @@ __global__ void __launch_bounds__(256) bp_tiled_kernel(BpLaunch L)
 #pragma unroll
                         for (int i = 0; i < PX; i++) {
                             const float u = (i == 0) ? bj : fmaf((float)(8 * i), tb.x, bj);
-                            const float4 q = seg[(int)u];
-                            acc[j][i] += fmaf(u, q.y, q.x);
-                            acc2[NS == 2 ? j : 0][NS == 2 ? i : 0] += fmaf(u, q.w, q.z);
+                            const float4* e = seg + (int)u;
+#pragma unroll
+                            for (int q = 0; q < NQ; q++) {
+                                const float4 v = e[q * SEG];
+                                if (q == 0) acc[j][i] += fmaf(u, v.y, v.x);
+                                else accx[2 * q - 1][j][i] += fmaf(u, v.y, v.x);
+                                accx[2 * q][j][i] += fmaf(u, v.w, v.z);
+                            }
                         }
                     }
                 }
@@ -379,10 +395,11 @@ __global__ void __launch_bounds__(256) bp_tiled_kernel(BpLaunch L)
                     float* out = (float*)S.out;
                     const float v = (float)acc[j][i] * S.out_scale;
                     out[o] = S.accumulate ? out[o] + v : v;
-                    if (NS == 2) {
-                        float* out2 = (float*)S.out2;
-                        const float v2 = (float)acc2[NS == 2 ? j : 0][NS == 2 ? i : 0] * S.out_scale;
-                        out2[o] = S.accumulate ? out2[o] + v2 : v2;
+#pragma unroll
+                    for (int e = 0; e + 1 < NS; e++) {
+                        float* outx = (float*)S.out_x[e];
+                        const float vx = (float)accx[e][j][i] * S.out_scale;
+                        outx[o] = S.accumulate ? outx[o] + vx : vx;
                     }
                 }
             }

@@ -82,8 +82,10 @@ void launch_output_stage(const void* img, int is_float, int N, int irad2, uint16
 
 // ------------------------------------------------------- back-projection
 // BP_LINEAR2: BP_LINEAR over two slices of identical geometry per work item
-// (second slice in BpSlice::rows2 / scales2 / out2).
+// (the second in BpSlice::rows_x[0] / scales_x[0] / out_x[0]).
 enum BpMode { BP_LINEAR = 0, BP_LUT_F32 = 1, BP_LUT_I32 = 2, BP_LINEAR_H = 3, BP_LINEAR2 = 4 };
+constexpr int bp_slices_per_item(int mode) { return mode == 4 ? 2 : 1; }
+constexpr int BP_MAX_SLICES_PER_ITEM = 2;
 
 // Per-angle table entry.  LINEAR: detector coordinate of pixel (ix, iy) of the
 // output grid is u = u0 + ix*dux + iy*duy (all double).  LUT16: the reference
@@ -109,9 +111,9 @@ struct BpSlice {
     int lut_formula;        // LUT modes
     int store;              // 0 f32, 1 f16, 2 bf16
     float out_scale;        // multiplies the final image (float modes)
-    const void* rows2;      // BP_LINEAR2: the second slice's rows, scales and image
-    const float* scales2;
-    void* out2;
+    const void* rows_x[BP_MAX_SLICES_PER_ITEM - 1];   // BP_LINEAR2: rows, scales and image
+    const float* scales_x[BP_MAX_SLICES_PER_ITEM - 1]; // of the item's other slices
+    void* out_x[BP_MAX_SLICES_PER_ITEM - 1];
 };
 
 struct BpLaunch {

@@ -13,6 +13,7 @@
 #include "rv_common.h"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -45,7 +46,7 @@ struct Group {
     int tile = 0, chunk = 0;
     int mode = 0;                // BP mode of the group's launch
     std::vector<int> slices;     // first slice of each work-item entry
-    std::vector<int> twins;      // BP_LINEAR2: the entry's second slice
+    std::vector<std::array<int, rv::BP_MAX_SLICES_PER_ITEM - 1>> others;   // BP_LINEAR2: other slices (-1: none)
     std::vector<rv::BpSlice> h;
     rv::BpSlice* d_slices = nullptr;
     int* d_order = nullptr;
@@ -119,10 +120,12 @@ bool upload_slices(rv_plan_t p, void* const* d_out)
                 G.h[q].out = d_out[s];
                 changed = true;
             }
-            const int s2 = G.twins[q];
-            if (s2 >= 0 && p->cur_out[s2] != d_out[s2]) {
-                G.h[q].out2 = d_out[s2];
-                changed = true;
+            for (int e = 0; e + 1 < rv::BP_MAX_SLICES_PER_ITEM; e++) {
+                const int s2 = G.others[q][e];
+                if (s2 >= 0 && p->cur_out[s2] != d_out[s2]) {
+                    G.h[q].out_x[e] = d_out[s2];
+                    changed = true;
+                }
             }
         }
         if (changed && !rv::hip_ok(hipMemcpy(G.d_slices, G.h.data(), sizeof(rv::BpSlice) * G.h.size(),
@@ -323,17 +326,19 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
     }
 
     // LINEAR with FP32 pairs in LDS: slices whose back-projection geometry is
-    // identical (angle table, sizes, tile, scale) share work items two by two.
-    std::vector<int> twin_of(nslices, -1);
-    std::vector<char> second(nslices, 0);
-    if (p->mode == rv::BP_LINEAR && !opt->no_slice_pairs) {
+    // identical (angle table, sizes, tile) share work items two at a time
+    // (classes of equal geometry, in slice order).
+    std::vector<std::vector<int>> classes;
+    const int per_item = opt->slices_per_item == 0 ? 2 : opt->slices_per_item;
+    if (per_item != 1 && per_item != 2) return fail(p, "slices_per_item must be 0, 1 or 2");
+    if (p->mode == rv::BP_LINEAR && per_item > 1) {
         auto same = [&](int a, int b) {
             const SliceState& A = p->sl[a];
             const SliceState& B = p->sl[b];
             return A.nang == B.nang && A.g.N == B.g.N && A.out_n == B.out_n && A.tile == B.tile &&
                    std::memcmp(ha.data() + A.ang_off, ha.data() + B.ang_off, sizeof(rv::AngleEntry) * A.nang) == 0;
         };
-        std::map<std::vector<long long>, std::vector<int>> open;
+        std::map<std::vector<long long>, std::vector<int>> by_key;   // key -> class indices
         for (int s = 0; s < nslices; s++) {
             const SliceState& S = p->sl[s];
             // cheap key first (sizes and the first and last angle entries), full comparison on a match
@@ -345,42 +350,53 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
                 std::memcpy(&bits, &d, sizeof(bits));
                 key.push_back(bits);
             }
-            auto& lst = open[key];
-            bool paired = false;
-            for (size_t q = 0; q < lst.size(); q++) {
-                if (same(lst[q], s)) {
-                    twin_of[lst[q]] = s;
-                    second[s] = 1;
-                    lst.erase(lst.begin() + q);
-                    paired = true;
+            auto& lst = by_key[key];
+            bool found = false;
+            for (const int c : lst) {
+                if (same(classes[c][0], s)) {
+                    classes[c].push_back(s);
+                    found = true;
                     break;
                 }
             }
-            if (!paired) lst.push_back(s);
+            if (!found) {
+                lst.push_back((int)classes.size());
+                classes.push_back({s});
+            }
         }
+    } else {
+        for (int s = 0; s < nslices; s++) classes.push_back({s});
     }
 
     // groups by tile configuration and slices per item; work items cost-sorted within each group
     std::map<std::pair<int, int>, int> gidx;
-    for (int s = 0; s < nslices; s++) {
-        if (second[s]) continue;
-        const int mode = twin_of[s] >= 0 ? (int)rv::BP_LINEAR2 : p->mode;
-        const auto gk = std::make_pair(p->sl[s].tile, mode);
-        auto it = gidx.find(gk);
-        if (it == gidx.end()) {
-            gidx[gk] = (int)p->groups.size();
-            Group G;
-            G.tile = p->sl[s].tile;
-            G.mode = mode;
-            G.chunk = opt->angles_per_chunk > 0 ? opt->angles_per_chunk : rv::bp_default_chunk(mode, G.tile);
-            if (G.chunk > rv::bp_max_chunk(mode)) G.chunk = rv::bp_max_chunk(mode);
-            while (G.chunk > 1 && rv::bp_lds_bytes(mode, G.tile, G.chunk) > 160 * 1024) G.chunk /= 2;
-            p->groups.push_back(G);
+    for (const auto& cls : classes) {
+        size_t i = 0;
+        while (i < cls.size()) {
+            const size_t left = cls.size() - i;
+            const int k = (per_item >= 2 && left >= 2) ? 2 : 1;
+            const int mode = k == 2 ? (int)rv::BP_LINEAR2 : p->mode;
+            const int s = cls[i];
+            const auto gk = std::make_pair(p->sl[s].tile, mode);
+            if (gidx.find(gk) == gidx.end()) {
+                gidx[gk] = (int)p->groups.size();
+                Group G;
+                G.tile = p->sl[s].tile;
+                G.mode = mode;
+                G.chunk = opt->angles_per_chunk > 0 ? opt->angles_per_chunk : rv::bp_default_chunk(mode, G.tile);
+                if (G.chunk > rv::bp_max_chunk(mode)) G.chunk = rv::bp_max_chunk(mode);
+                while (G.chunk > 1 && rv::bp_lds_bytes(mode, G.tile, G.chunk) > 160 * 1024) G.chunk /= 2;
+                p->groups.push_back(G);
+            }
+            Group& G = p->groups[gidx[gk]];
+            G.slices.push_back(s);
+            std::array<int, rv::BP_MAX_SLICES_PER_ITEM - 1> others;
+            others.fill(-1);
+            for (int e = 1; e < k; e++) others[e - 1] = cls[i + e];
+            G.others.push_back(others);
+            for (int e = 0; e < k; e++) p->sl[cls[i + e]].group = gidx[gk];
+            i += k;
         }
-        p->sl[s].group = gidx[gk];
-        p->groups[p->sl[s].group].slices.push_back(s);
-        p->groups[p->sl[s].group].twins.push_back(twin_of[s]);
-        if (twin_of[s] >= 0) p->sl[twin_of[s]].group = p->sl[s].group;
     }
 
     const size_t esz = (p->store == 0) ? 4 : 2;
@@ -438,11 +454,12 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
             B.lut_formula = opt->lut_formula ? 1 : 0;
             B.store = p->store;
             B.out_scale = opt->normalize ? (float)(6.283185307179586 / S.nang) : 1.f;
-            const int s2 = G.twins[q];
-            if (s2 >= 0) {
+            for (int e = 0; e + 1 < rv::BP_MAX_SLICES_PER_ITEM; e++) {
+                const int s2 = G.others[q][e];
+                if (s2 < 0) continue;
                 const SliceState& S2 = p->sl[s2];
-                B.rows2 = (const unsigned char*)p->d_rows + S2.rows_off * esz;
-                B.scales2 = (p->store == 0) ? nullptr : p->d_scales + S2.ang_off;
+                B.rows_x[e] = (const unsigned char*)p->d_rows + S2.rows_off * esz;
+                B.scales_x[e] = (p->store == 0) ? nullptr : p->d_scales + S2.ang_off;
             }
         }
         if (!rv::hip_ok(hipMalloc(&G.d_slices, sizeof(rv::BpSlice) * G.h.size()), "slice table") ||

@@ -77,18 +77,19 @@ class FbpPlan:
 
     def __init__(self, slices, interp="linear", store="f32", lut_formula=1, index_rule=1,
                  persistent=False, tile=0, angles_per_chunk=0, lds_half=False, normalize=False,
-                 exact_filter=False, slice_pairs=True, device=0):
+                 exact_filter=False, slices_per_item=0, device=0):
         """normalize: images in attenuation per detector sample (x 2 pi / projections)
         instead of raw back-projected sums.  exact_filter: LINEAR modes keep CFft's
         operation order (bit-identical filtered rows) instead of two rows per transform.
-        slice_pairs: LINEAR with FP32 LDS pairs back-projects slices of identical
-        geometry two per work item (same results, fewer instructions per update)."""
+        slices_per_item: LINEAR with FP32 LDS pairs back-projects slices of identical
+        geometry two per work item (0 or 2; 1 = one per item) -- same results,
+        fewer instructions per update."""
         self.slices = list(slices)
         L = _lib.lib()
         specs = (_lib.RvSliceSpec * len(self.slices))(*[s.to_c() for s in self.slices])
         opt = _lib.RvOptions(INTERP[interp], STORE[store], int(lut_formula), int(index_rule),
                              int(persistent), int(tile), int(angles_per_chunk), int(lds_half),
-                             int(normalize), int(exact_filter), int(not slice_pairs))
+                             int(normalize), int(exact_filter), int(slices_per_item))
         self.interp = interp
         self.store = store
         h = C.c_void_p()

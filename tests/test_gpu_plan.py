@@ -156,7 +156,7 @@ def test_plan_batch_uneven_equals_single():
 
 @pytest.mark.parametrize("store,persistent,normalize", [("f32", True, False), ("f32", False, True),
                                                          ("f16", True, False), ("bf16", False, False)])
-def test_plan_slice_pairs_identical(store, persistent, normalize):
+def test_plan_slice_groups_identical(store, persistent, normalize):
     """Slices of identical geometry back-projected two per work item give exactly
     the images of one slice per item: five equal-geometry slices (two pairs and
     one left over) with different data, plus two slices of other geometries."""
@@ -171,12 +171,13 @@ def test_plan_slice_pairs_identical(store, persistent, normalize):
         specs.append(fbp.SliceSpec(W, fdeg, c))
         sinos.append(torch.from_numpy(s))
     kw = dict(store=store, persistent=persistent, normalize=normalize)
-    _, paired = _run(specs, sinos, **kw)
-    _, single = _run(specs, sinos, slice_pairs=False, **kw)
-    for i in range(len(specs)):
-        assert np.array_equal(paired[i], single[i]), i
+    _, single = _run(specs, sinos, slices_per_item=1, **kw)
+    for k in (0, 2):
+        _, grouped = _run(specs, sinos, slices_per_item=k, **kw)
+        for i in range(len(specs)):
+            assert np.array_equal(grouped[i], single[i]), (k, i)
     # the data of each slice stays with its own image
-    assert not np.array_equal(paired[0], paired[1])
+    assert not np.array_equal(single[0], single[1])
 
 
 def test_plan_normalize_gives_attenuation():

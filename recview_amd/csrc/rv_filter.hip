@@ -142,7 +142,8 @@ __device__ __forceinline__ void fft_first_stage(float2* buf, int half, int T, in
 
 // Per-row power-of-two scale so that |v| / scale <= 2^14, then the 16-bit store
 // of the N cropped values held in outf (LDS).
-__device__ void store_lp_row(const FilterJob& J, const float* outf, int rout, float* red)
+__device__ void store_lp_row(const FilterJob& J, void* out_lp, float* out_scale, const float* outf, int rout,
+                             float* red)
 {
     const int T = blockDim.x, tid = threadIdx.x;
     __syncthreads();
@@ -161,29 +162,31 @@ __device__ void store_lp_row(const FilterJob& J, const float* outf, int rout, fl
     __syncthreads();
     const float sc = red[16];
     const float inv = 1.f / sc;            // power of two: exact
-    if (tid == 0) J.out_scale[rout] = sc;
+    if (tid == 0) out_scale[rout] = sc;
     if (J.store == 1) {
-        __half* o = (__half*)J.out_lp + (long long)rout * J.out_stride;
+        __half* o = (__half*)out_lp + (long long)rout * J.out_stride;
         for (int c = tid; c < J.N; c += T) o[c] = __float2half_rn(outf[c] * inv);
     } else {
-        __hip_bfloat16* o = (__hip_bfloat16*)J.out_lp + (long long)rout * J.out_stride;
+        __hip_bfloat16* o = (__hip_bfloat16*)out_lp + (long long)rout * J.out_stride;
         for (int c = tid; c < J.N; c += T) o[c] = __float2bfloat16(outf[c] * inv);
     }
     __syncthreads();
 }
 
 template <int IN_KIND>
-__global__ void __launch_bounds__(1024) filter_rows_kernel(FilterJob J)
+__global__ void __launch_bounds__(1024) filter_rows_kernel(FilterJob J, FilterBatch FB)
 {
     extern __shared__ float2 buf[];
-    const int r = blockIdx.x;
+    const int r = blockIdx.x, bi = blockIdx.y;
+    const void* sino = FB.sino[bi];
+    float* out = FB.out[bi];
     const int rin = J.row_in ? J.row_in[r] : r;
     const int rout = J.row_out ? J.row_out[r] : r;
     const int n = J.n, half = n >> 1, np = J.np, T = blockDim.x, tid = threadIdx.x;
     const float2* tw = (const float2*)J.tw;
     const void* row = (IN_KIND == 0)
-        ? (const void*)((const int16_t*)J.sino + (long long)rin * J.in_stride)
-        : (const void*)((const float*)J.sino + (long long)rin * J.in_stride);
+        ? (const void*)((const int16_t*)sino + (long long)rin * J.in_stride)
+        : (const void*)((const float*)sino + (long long)rin * J.in_stride);
 
     // Row preparation straight into LDS.
     for (int m = tid; m < n; m += T) buf[m] = make_float2(prep_value<IN_KIND>(J, row, m), 0.f);
@@ -246,7 +249,7 @@ __global__ void __launch_bounds__(1024) filter_rows_kernel(FilterJob J)
                 const int c0 = n - 1 - k - J.crop;
                 const int c1 = n - 1 - (k + half) - J.crop;
                 if (!lp) {
-                    float* o = J.out + (long long)rout * J.out_stride;
+                    float* o = out + (long long)rout * J.out_stride;
                     if (c0 >= 0 && c0 < J.N) o[c0] = v0;
                     if (c1 >= 0 && c1 < J.N) o[c1] = v1;
                 } else {
@@ -257,7 +260,7 @@ __global__ void __launch_bounds__(1024) filter_rows_kernel(FilterJob J)
         }
         if (lp) {
             __shared__ float red[32];
-            store_lp_row(J, outf, rout, red);
+            store_lp_row(J, FB.out_lp[bi], FB.out_scale[bi], outf, rout, red);
         }
     }
 }
@@ -336,10 +339,11 @@ __device__ __forceinline__ void fft_radix4(float2* buf, const float2* twf, int n
 }
 
 template <int IN_KIND>
-__global__ void __launch_bounds__(1024) filter_pair_kernel(FilterJob J)
+__global__ void __launch_bounds__(1024) filter_pair_kernel(FilterJob J, FilterBatch FB)
 {
     extern __shared__ float2 buf[];
-    const int ra = 2 * (int)blockIdx.x;
+    const int ra = 2 * (int)blockIdx.x, bi = blockIdx.y;
+    const void* sino = FB.sino[bi];
     const bool hasb = ra + 1 < J.nrows;
     const int rin = J.row_in ? J.row_in[ra] : ra;
     const int rout = J.row_out ? J.row_out[ra] : ra;
@@ -348,11 +352,11 @@ __global__ void __launch_bounds__(1024) filter_pair_kernel(FilterJob J)
     const int n = J.n, np = J.np, T = blockDim.x, tid = threadIdx.x;
     const float2* twf = (const float2*)J.tw;
     const void* row = (IN_KIND == 0)
-        ? (const void*)((const int16_t*)J.sino + (long long)rin * J.in_stride)
-        : (const void*)((const float*)J.sino + (long long)rin * J.in_stride);
+        ? (const void*)((const int16_t*)sino + (long long)rin * J.in_stride)
+        : (const void*)((const float*)sino + (long long)rin * J.in_stride);
     const void* row_b = (IN_KIND == 0)
-        ? (const void*)((const int16_t*)J.sino + (long long)rin_b * J.in_stride)
-        : (const void*)((const float*)J.sino + (long long)rin_b * J.in_stride);
+        ? (const void*)((const int16_t*)sino + (long long)rin_b * J.in_stride)
+        : (const void*)((const float*)sino + (long long)rin_b * J.in_stride);
     for (int m = tid; m < n; m += T)
         buf[m] = make_float2(prep_value<IN_KIND>(J, row, m), hasb ? prep_value<IN_KIND>(J, row_b, m) : 0.f);
     __syncthreads();
@@ -367,8 +371,8 @@ __global__ void __launch_bounds__(1024) filter_pair_kernel(FilterJob J)
     // y = conj(buf) / n; rows: real part -> a, imaginary part -> b
     const float a = 1.f / (float)n;
     if (J.store == 0) {
-        float* o = J.out + (long long)rout * J.out_stride;
-        float* ob = J.out + (long long)rout_b * J.out_stride;
+        float* o = FB.out[bi] + (long long)rout * J.out_stride;
+        float* ob = FB.out[bi] + (long long)rout_b * J.out_stride;
         for (int c = tid; c < J.N; c += T) {
             const float2 y = buf[c + J.crop];
             o[c] = y.x * a;
@@ -394,8 +398,8 @@ __global__ void __launch_bounds__(1024) filter_pair_kernel(FilterJob J)
             if (c < J.N) { outf[c] = va[i]; outf_b[c] = vb[i]; }
         }
         __shared__ float red[32];
-        store_lp_row(J, outf, rout, red);
-        if (hasb) store_lp_row(J, outf_b, rout_b, red);
+        store_lp_row(J, FB.out_lp[bi], FB.out_scale[bi], outf, rout, red);
+        if (hasb) store_lp_row(J, FB.out_lp[bi], FB.out_scale[bi], outf_b, rout_b, red);
     }
 }
 
@@ -422,7 +426,7 @@ void prepare_filter(int n)
     done[dev] = n;
 }
 
-void launch_filter(const FilterJob& job, hipStream_t stream)
+void launch_filter_batch(const FilterJob& job, const FilterBatch& fb, int count, hipStream_t stream)
 {
     const int half = job.n / 2;
     int T = half / kMaxBfly;
@@ -430,18 +434,28 @@ void launch_filter(const FilterJob& job, hipStream_t stream)
     if (T > 1024) T = 1024;
     const size_t lds = sizeof(float2) * (size_t)job.n;
     prepare_filter(job.n);
-    if (job.nrows <= 0) return;
+    if (job.nrows <= 0 || count <= 0 || count > kFilterBatch) return;
     if (job.pair) {
-        const dim3 g((job.nrows + 1) / 2);
+        const dim3 g((job.nrows + 1) / 2, count);
         if (job.in_kind == 0)
-            hipLaunchKernelGGL(filter_pair_kernel<0>, g, dim3(T), lds, stream, job);
+            hipLaunchKernelGGL(filter_pair_kernel<0>, g, dim3(T), lds, stream, job, fb);
         else
-            hipLaunchKernelGGL(filter_pair_kernel<1>, g, dim3(T), lds, stream, job);
+            hipLaunchKernelGGL(filter_pair_kernel<1>, g, dim3(T), lds, stream, job, fb);
     } else if (job.in_kind == 0) {
-        hipLaunchKernelGGL(filter_rows_kernel<0>, dim3(job.nrows), dim3(T), lds, stream, job);
+        hipLaunchKernelGGL(filter_rows_kernel<0>, dim3(job.nrows, count), dim3(T), lds, stream, job, fb);
     } else {
-        hipLaunchKernelGGL(filter_rows_kernel<1>, dim3(job.nrows), dim3(T), lds, stream, job);
+        hipLaunchKernelGGL(filter_rows_kernel<1>, dim3(job.nrows, count), dim3(T), lds, stream, job, fb);
     }
+}
+
+void launch_filter(const FilterJob& job, hipStream_t stream)
+{
+    FilterBatch fb{};
+    fb.sino[0] = job.sino;
+    fb.out[0] = job.out;
+    fb.out_lp[0] = job.out_lp;
+    fb.out_scale[0] = job.out_scale;
+    launch_filter_batch(job, fb, 1, stream);
 }
 
 // ------------------------------------------------ compatibility kernels

@@ -36,6 +36,16 @@ struct FilterJob {
 };
 
 void launch_filter(const FilterJob& job, hipStream_t stream);
+// Several slices of identical filter geometry in one launch (grid y = slice):
+// the per-slice pointers come from the batch, every other field from the job.
+constexpr int kFilterBatch = 16;
+struct FilterBatch {
+    const void* sino[kFilterBatch];
+    float* out[kFilterBatch];
+    void* out_lp[kFilterBatch];
+    float* out_scale[kFilterBatch];
+};
+void launch_filter_batch(const FilterJob& job, const FilterBatch& batch, int count, hipStream_t stream);
 void prepare_filter(int n);   // kernel attributes for an n-point transform (not capturable)
 
 // LUT build (cudaReconst.cu:230-246) and single-projection LUT back-projection

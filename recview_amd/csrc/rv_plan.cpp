@@ -35,6 +35,7 @@ struct SliceState {
     int tiles_x = 0, tiles_y = 0, tile_base = 0;
     int group = 0;
     int tile = 0;
+    int fclass = 0;              // first slice with the same filter job (batched launches)
     // offset CT
     bool offset = false;
     int raw_W = 0;
@@ -136,12 +137,51 @@ bool upload_slices(rv_plan_t p, void* const* d_out)
     return true;
 }
 
+rv::FilterJob filter_job(rv_plan_t p, size_t s)
+{
+    const SliceState& S = p->sl[s];
+    const rv::Geom& g = S.g;
+    const size_t esz = (p->store == 0) ? 4 : 2;
+    rv::FilterJob J{};
+    J.in_stride = g.W;
+    J.nrows = S.nang;
+    J.in_kind = (S.spec.sino_dtype != RV_SINO_I16) ? 1 : 0;
+    J.W = g.W; J.z = g.z; J.ic = g.ic; J.n = g.n; J.np = g.ndimp; J.N = g.N;
+    const bool zero = (S.spec.pad_mode == RV_PAD_ZERO);
+    J.place = zero ? g.n / 2 : g.n / 2 - 1;
+    J.pad_mode = zero ? 1 : 0;
+    J.crop = zero ? g.n / 2 - g.h : g.ihoffset;
+    J.filt = p->d_filts + S.filt_off;
+    J.tw = rv::device_twiddles(p->dev, g.ndimp);
+    if (p->pair_filter) {
+        J.pair = 1;
+        J.filt = p->d_filts + p->filt_total + S.filt_off;
+        J.tw = rv::device_twiddles_full(p->dev, g.ndimp);
+    }
+    J.out_stride = g.N;
+    J.store = p->store;
+    if (p->store == 0) {
+        J.out = (float*)p->d_rows + S.rows_off;
+    } else {
+        J.out_lp = (unsigned char*)p->d_rows + S.rows_off * esz;
+        J.out_scale = p->d_scales + S.ang_off;
+    }
+    return J;
+}
+
+// Slices of one filter class (same transform, filter table, row count and
+// input kind) are filtered by one launch, up to kFilterBatch slices each.
 int launch_filters(rv_plan_t p, const void* const* d_sino, hipStream_t st)
 {
-    const size_t esz = (p->store == 0) ? 4 : 2;
+    rv::FilterJob head{};
+    rv::FilterBatch fb{};
+    int count = 0, cls = -1;
+    auto flush = [&]() {
+        if (count) rv::launch_filter_batch(head, fb, count, st);
+        count = 0;
+    };
     for (size_t s = 0; s < p->sl.size(); s++) {
         const SliceState& S = p->sl[s];
-        const rv::Geom& g = S.g;
         const bool is_float = (S.spec.sino_dtype != RV_SINO_I16);
         const void* src = d_sino[s];
         if (S.offset) {
@@ -154,42 +194,26 @@ int launch_filters(rv_plan_t p, const void* const* d_sino, hipStream_t st)
             J.deltac = 0.0;
             J.pair = p->d_pairs + S.pair_off;
             J.nout = S.nang;
-            J.out_width = g.W;
+            J.out_width = S.g.W;
             J.out = (unsigned char*)p->d_stitch + S.stitch_off * (is_float ? 4 : 2);
             J.is_float = is_float ? 1 : 0;
             rv::launch_stitch(J, st);
             src = J.out;
         }
-        const float* tw = rv::device_twiddles(p->dev, g.ndimp);
-        if (!tw) { rv::set_message("twiddle table allocation failed"); return -1; }
-        rv::FilterJob J{};
-        J.sino = src;
-        J.in_stride = g.W;
-        J.nrows = S.nang;
-        J.in_kind = is_float ? 1 : 0;
-        J.W = g.W; J.z = g.z; J.ic = g.ic; J.n = g.n; J.np = g.ndimp; J.N = g.N;
-        const bool zero = (S.spec.pad_mode == RV_PAD_ZERO);
-        J.place = zero ? g.n / 2 : g.n / 2 - 1;
-        J.pad_mode = zero ? 1 : 0;
-        J.crop = zero ? g.n / 2 - g.h : g.ihoffset;
-        J.filt = p->d_filts + S.filt_off;
-        J.tw = tw;
-        if (p->pair_filter) {
-            J.pair = 1;
-            J.filt = p->d_filts + p->filt_total + S.filt_off;
-            J.tw = rv::device_twiddles_full(p->dev, g.ndimp);
-            if (!J.tw) { rv::set_message("twiddle table allocation failed"); return -1; }
+        const rv::FilterJob J = filter_job(p, s);
+        if (!J.tw) { rv::set_message("twiddle table allocation failed"); return -1; }
+        if (count == rv::kFilterBatch || S.fclass != cls) flush();
+        if (count == 0) {
+            head = J;
+            cls = S.fclass;
         }
-        J.out_stride = g.N;
-        J.store = p->store;
-        if (p->store == 0) {
-            J.out = (float*)p->d_rows + S.rows_off;
-        } else {
-            J.out_lp = (unsigned char*)p->d_rows + S.rows_off * esz;
-            J.out_scale = p->d_scales + S.ang_off;
-        }
-        rv::launch_filter(J, st);
+        fb.sino[count] = src;
+        fb.out[count] = J.out;
+        fb.out_lp[count] = J.out_lp;
+        fb.out_scale[count] = J.out_scale;
+        count++;
     }
+    flush();
     return rv::hip_ok(hipGetLastError(), "filter launch") ? 0 : -1;
 }
 
@@ -322,6 +346,24 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
             }
             rv::lut_params(S.fdeg[a], S.spec.tilt, S.g, 0, &e.fcos, &e.fsin, &e.foffset);
             e.row = a;
+        }
+    }
+
+    // filter classes: equal transform geometry, row count, input kind and filter table
+    for (int s = 0; s < nslices; s++) {
+        SliceState& S = p->sl[s];
+        S.fclass = s;
+        for (int t = 0; t < s; t++) {
+            const SliceState& T = p->sl[t];
+            if (T.fclass != t) continue;
+            if (T.g.W == S.g.W && T.g.z == S.g.z && T.g.ic == S.g.ic && T.g.n == S.g.n &&
+                T.g.ndimp == S.g.ndimp && T.g.N == S.g.N && T.g.h == S.g.h && T.g.ihoffset == S.g.ihoffset &&
+                T.nang == S.nang && T.spec.pad_mode == S.spec.pad_mode &&
+                (T.spec.sino_dtype != RV_SINO_I16) == (S.spec.sino_dtype != RV_SINO_I16) &&
+                std::memcmp(hf.data() + T.filt_off, hf.data() + S.filt_off, sizeof(float) * S.g.n) == 0) {
+                S.fclass = t;
+                break;
+            }
         }
     }
 

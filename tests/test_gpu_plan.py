@@ -180,6 +180,28 @@ def test_plan_slice_groups_identical(store, persistent, normalize):
     assert not np.array_equal(single[0], single[1])
 
 
+@pytest.mark.parametrize("store,dtype,exact", [("f32", "f32", False), ("bf16", "f32", False),
+                                               ("f32", "i16", False), ("f32", "f32", True)])
+def test_plan_filter_batches_equal_single(store, dtype, exact):
+    """Slices of one filter class are filtered by batched launches (up to 16 per
+    launch): 18 slices of one geometry, interleaved with two of another, give the
+    images of one plan per slice."""
+    print(f"case store={store} dtype={dtype} exact={exact}", flush=True)
+    specs, sinos = [], []
+    for i in range(20):
+        W, P = (64, 40) if i not in (5, 11) else (80, 40)
+        s, fdeg, c = _sino(W, P, 700 + i)
+        if dtype == "i16":
+            s = phantom.to_int16(torch.from_numpy(s)).numpy()
+        specs.append(fbp.SliceSpec(W, fdeg, c, sino_dtype=dtype))
+        sinos.append(torch.from_numpy(s))
+    kw = dict(store=store, exact_filter=exact)
+    _, batch = _run(specs, sinos, **kw)
+    for i in range(len(specs)):
+        _, (one,) = _run([specs[i]], [sinos[i]], **kw)
+        assert np.array_equal(batch[i], one), i
+
+
 def test_plan_normalize_gives_attenuation():
     """normalize = 1 multiplies each image by 2 pi / P (per slice, also in a
     batch of different P), so the output is attenuation per detector sample."""

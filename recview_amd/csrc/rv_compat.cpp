@@ -12,6 +12,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <vector>
 
@@ -55,6 +56,37 @@ bool grow(T*& ptr, unsigned int& maxb, size_t bytes)
 struct Selected {
     int i, slot, sidx;
 };
+
+size_t align64(size_t b) { return (b + 63) / 64 * 64; }
+
+// Grow-only pinned host staging per RECONST_INFO (rows and launch metadata of
+// one call; the reference keeps pinned host rows per worker too,
+// cudaReconstHost.cpp:678).  Every call ends with a stream synchronisation, so
+// the next call may overwrite it.  Released by CudaReconstMemFree.
+std::mutex g_stage_mu;
+std::map<const RECONST_INFO*, std::pair<void*, size_t>> g_stage;
+
+void* host_stage(const RECONST_INFO* ri, size_t bytes)
+{
+    std::lock_guard<std::mutex> lk(g_stage_mu);
+    auto& h = g_stage[ri];
+    if (h.first && h.second >= bytes) return h.first;
+    if (h.first) (void)hipHostFree(h.first);
+    h = {nullptr, 0};
+    void* p = nullptr;
+    if (hipHostMalloc(&p, bytes, 0) != hipSuccess) return nullptr;
+    h = {p, bytes};
+    return p;
+}
+
+void release_stage(const RECONST_INFO* ri)
+{
+    std::lock_guard<std::mutex> lk(g_stage_mu);
+    auto it = g_stage.find(ri);
+    if (it == g_stage.end()) return;
+    if (it->second.first) (void)hipHostFree(it->second.first);
+    g_stage.erase(it);
+}
 
 // Frame selection of cudaReconstHost.cpp:712-730 with the progress / abort
 // callback of :715-717.  Returns the projections to use, in order.
@@ -142,79 +174,47 @@ void reconst_body(RECONST_INFO* ri, int idev, bool bReport, bool bEnStream, bool
 
     std::vector<Selected> sel = select_frames(ri, bReport);
     const int nsel = (int)sel.size();
-
-    if (bDeconv && nsel > 0) {
-        // Gather the selected int16 rows into pinned memory, one H2D, one filter launch.
-        short* h_rows = nullptr;
-        if (hipHostMalloc((void**)&h_rows, sizeof(short) * (size_t)nsel * W, 0) != hipSuccess) {
-            ri->iStatus = RECONST_INFO_ERROR;
-            rv::log_code(28803);
-            return;
-        }
-        std::vector<int> slots(nsel);
-        for (int q = 0; q < nsel; q++) {
-            std::memcpy(h_rows + (size_t)q * W, ri->iSinogr[sel[q].sidx], sizeof(short) * W);
-            slots[q] = sel[q].slot;
-        }
-        int* d_slots = nullptr;
-        bool ok = rv::hip_ok(hipMalloc(&d_slots, sizeof(int) * nsel), "slot list") &&
-                  rv::hip_ok(hipMemcpyAsync(ri->d_strip, h_rows, sizeof(short) * (size_t)nsel * W,
-                                            hipMemcpyHostToDevice, st), "row upload") &&
-                  rv::hip_ok(hipMemcpy(d_slots, slots.data(), sizeof(int) * nsel,
-                                       hipMemcpyHostToDevice), "slot upload");
-        const float* tw = ok ? rv::device_twiddles(idev, g.ndimp) : nullptr;
-        if (ok && tw) {
-            rv::FilterJob J{};
-            J.sino = ri->d_strip;
-            J.in_stride = W;
-            J.row_in = nullptr;
-            J.row_out = d_slots;
-            J.nrows = nsel;
-            J.in_kind = 0;
-            J.W = W; J.z = g.z; J.ic = g.ic; J.n = n; J.np = g.ndimp; J.N = N;
-            J.place = cudafft ? n / 2 : n / 2 - 1;
-            J.pad_mode = cudafft ? 1 : 0;
-            J.crop = cudafft ? n / 2 - g.h : g.ihoffset;
-            J.filt = ri->d_filt;
-            J.tw = tw;
-            J.out = ri->d_px;
-            J.out_stride = N;
-            J.store = 0;
-            rv::launch_filter(J, st);
-            ok = rv::hip_ok(hipGetLastError(), "filter launch") &&
-                 rv::hip_ok(hipStreamSynchronize(st), "filter");
-        } else {
-            ok = false;
-        }
-        (void)hipFree(d_slots);
-        (void)hipHostFree(h_rows);
-        if (!ok) {
-            rv::log_code(28804);
-            ri->iStatus = RECONST_INFO_ERROR;
-            return;
-        }
-    }
-
-    // Back-projection of every selected projection in one launch.
     if (nsel == 0) {
         if (!rv::hip_ok(hipMemsetAsync(ri->d_ifp, 0, mem_ifp, st), "image clear")) rv::log_code(28806);
     } else {
+        // Metadata of the launch -- angle table, slice descriptor, row slots -- staged
+        // with the selected int16 rows in one pinned host buffer and uploaded
+        // asynchronously into the d_p scratch field (unused by this path) and d_strip.
+        const size_t tab_bytes = align64(sizeof(rv::AngleEntry) * (size_t)nsel);
+        const size_t meta_bytes = tab_bytes + align64(sizeof(rv::BpSlice)) + align64(sizeof(int) * (size_t)nsel);
+        const size_t row_bytes = bDeconv ? align64(sizeof(short) * (size_t)nsel * W) : 0;
+        unsigned char* h = (unsigned char*)host_stage(ri, meta_bytes + row_bytes);
+        if (!h || !grow(ri->d_p, ri->max_d_p, meta_bytes)) {
+            ri->iStatus = RECONST_INFO_ERROR;
+            rv::log_code(h ? 28801 : 28803);
+            return;
+        }
+        unsigned char* d = (unsigned char*)ri->d_p;
+        rv::AngleEntry* h_tab = (rv::AngleEntry*)h;
+        rv::BpSlice* h_slice = (rv::BpSlice*)(h + tab_bytes);
+        int* h_slots = (int*)(h + tab_bytes + align64(sizeof(rv::BpSlice)));
+        rv::AngleEntry* d_tab = (rv::AngleEntry*)d;
+        rv::BpSlice* d_slice = (rv::BpSlice*)(d + tab_bytes);
+        int* d_slots = (int*)(d + tab_bytes + align64(sizeof(rv::BpSlice)));
+
         const int icoff = g.ic - ri->iSinoCenter;                          // :775
-        std::vector<rv::AngleEntry> tab(nsel);
         for (int q = 0; q < nsel; q++) {
-            rv::AngleEntry& e = tab[q];
+            rv::AngleEntry& e = h_tab[q];
             std::memset(&e, 0, sizeof(e));
             rv::lut_params(ri->fdeg[sel[q].i], ri->fTiltAngle, g, icoff, &e.fcos, &e.fsin, &e.foffset);
             e.row = sel[q].slot;
+            h_slots[q] = sel[q].slot;
         }
         const int tile = 0;
         const int tw = rv::bp_tile_side(tile), th = rv::bp_tile_h(tile);
-        rv::BpSlice S{};
+        rv::BpSlice& S = *h_slice;
+        std::memset(&S, 0, sizeof(S));
         S.rows = ri->d_px;
         S.scales = nullptr;
         S.row_stride = N;
         S.row_len = N;
         S.nang = nsel;
+        S.ang = d_tab;
         S.out = ri->d_ifp;
         S.out_n = N;
         S.tiles_x = (N + tw - 1) / tw;
@@ -224,22 +224,45 @@ void reconst_body(RECONST_INFO* ri, int idev, bool bReport, bool bEnStream, bool
         S.lut_formula = cudafft ? 1 : g_lut_formula;
         S.store = 0;
         S.out_scale = 1.f;
-        // table + slice descriptor in the d_p scratch field (unused by this path)
-        const size_t tab_bytes = sizeof(rv::AngleEntry) * (size_t)nsel;
-        const size_t need = tab_bytes + sizeof(rv::BpSlice) + 64;
-        if (!grow(ri->d_p, ri->max_d_p, need)) {
-            ri->iStatus = RECONST_INFO_ERROR;
-            rv::log_code(28801);
-            return;
+        bool ok = rv::hip_ok(hipMemcpyAsync(d, h, meta_bytes, hipMemcpyHostToDevice, st), "metadata upload");
+
+        if (ok && bDeconv) {
+            // Gather the selected int16 rows, one H2D, one filter launch.
+            short* h_rows = (short*)(h + meta_bytes);
+            for (int q = 0; q < nsel; q++)
+                std::memcpy(h_rows + (size_t)q * W, ri->iSinogr[sel[q].sidx], sizeof(short) * W);
+            const float* twd = rv::device_twiddles(idev, g.ndimp);
+            ok = twd && rv::hip_ok(hipMemcpyAsync(ri->d_strip, h_rows, sizeof(short) * (size_t)nsel * W,
+                                                  hipMemcpyHostToDevice, st), "row upload");
+            if (ok) {
+                rv::FilterJob J{};
+                J.sino = ri->d_strip;
+                J.in_stride = W;
+                J.row_in = nullptr;
+                J.row_out = d_slots;
+                J.nrows = nsel;
+                J.in_kind = 0;
+                J.W = W; J.z = g.z; J.ic = g.ic; J.n = n; J.np = g.ndimp; J.N = N;
+                J.place = cudafft ? n / 2 : n / 2 - 1;
+                J.pad_mode = cudafft ? 1 : 0;
+                J.crop = cudafft ? n / 2 - g.h : g.ihoffset;
+                J.filt = ri->d_filt;
+                J.tw = twd;
+                J.out = ri->d_px;
+                J.out_stride = N;
+                J.store = 0;
+                rv::launch_filter(J, st);
+                ok = rv::hip_ok(hipGetLastError(), "filter launch");
+            }
+            if (!ok) {
+                (void)hipStreamSynchronize(st);      // the staging buffer is reused by the next call
+                rv::log_code(28804);
+                ri->iStatus = RECONST_INFO_ERROR;
+                return;
+            }
         }
-        unsigned char* base = (unsigned char*)ri->d_p;
-        rv::AngleEntry* d_tab = (rv::AngleEntry*)base;
-        rv::BpSlice* d_slice = (rv::BpSlice*)(base + ((tab_bytes + 63) / 64) * 64);
-        S.ang = d_tab;
-        // synchronous copies: the host table and descriptor live on this stack frame
-        bool ok = rv::hip_ok(hipMemcpy(d_tab, tab.data(), tab_bytes, hipMemcpyHostToDevice),
-                             "angle table") &&
-                  rv::hip_ok(hipMemcpy(d_slice, &S, sizeof(S), hipMemcpyHostToDevice), "slice");
+
+        // Back-projection of every selected projection in one launch.
         if (ok) {
             rv::BpLaunch L{};
             L.slices = d_slice;
@@ -255,7 +278,8 @@ void reconst_body(RECONST_INFO* ri, int idev, bool bReport, bool bEnStream, bool
             ok = rv::hip_ok(hipGetLastError(), "back-projection launch");
         }
         if (!ok) {
-            rv::log_code(28804);
+            (void)hipStreamSynchronize(st);
+            rv::log_code(28806);
             ri->iStatus = RECONST_INFO_ERROR;
             return;
         }
@@ -297,6 +321,7 @@ void CudaReconstMemFree(RECONST_INFO* ri)
     if (ri->d_igp && hipFree(ri->d_igp) == hipSuccess) { ri->d_igp = nullptr; ri->max_d_igp = 0; }
     if (ri->d_p && hipFree(ri->d_p) == hipSuccess) { ri->d_p = nullptr; ri->max_d_p = 0; }
     if (ri->d_filt && hipFree(ri->d_filt) == hipSuccess) { ri->d_filt = nullptr; ri->max_d_filt = 0; }
+    release_stage(ri);
     ri->fftplan = nullptr;
     ri->ifftdim = 0;
 }

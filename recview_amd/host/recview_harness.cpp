@@ -5,10 +5,13 @@
 // DeconvBackProjThread (general.cpp:2172), applies the output stage on the
 // device and writes the 16-bit slice as RecView's TIFF.
 //
-//   recview_harness <sino.i16> <P> <W> <center_user> <fdeg.f32> <out.i32> [out.tif]
+//   recview_harness [--repeat K] <sino.i16> <P> <W> <center_user> <fdeg.f32> <out.i32> [out.tif]
 //
 // sino.i16: P rows of W int16 samples (x3000); fdeg.f32: P angles (degrees).
-// out.i32 receives the raw int32 image (W x W).  Exit code 0 on success.
+// out.i32 receives the raw int32 image (W x W).  --repeat K times K more calls
+// after the first (steady state: buffers already grown) and prints the mean
+// wall time per call.  Exit code 0 on success.
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -27,6 +30,12 @@ static bool read_file(const char* path, void* dst, size_t bytes)
 
 int main(int argc, char** argv)
 {
+    int repeat = 0;
+    if (argc > 2 && std::strcmp(argv[1], "--repeat") == 0) {
+        repeat = std::atoi(argv[2]);
+        argc -= 2;
+        argv += 2;
+    }
     if (argc < 7) {
         std::fprintf(stderr, "usage: %s sino.i16 P W center_user fdeg.f32 out.i32 [out.tif]\n", argv[0]);
         return 2;
@@ -75,6 +84,21 @@ int main(int argc, char** argv)
     if (ri.iStatus == RECONST_INFO_ERROR) {
         std::fprintf(stderr, "CudaReconstHost failed: %s\n", RecviewLastMessage());
         return 4;
+    }
+    if (repeat > 0) {
+        const auto t0 = std::chrono::steady_clock::now();
+        for (int k = 0; k < repeat; k++) {
+            ri.iStatus = RECONST_INFO_BUSY;
+            nsino = 0;
+            CudaReconstHost(&ri, 0, false, true);
+            if (ri.iStatus == RECONST_INFO_ERROR) {
+                std::fprintf(stderr, "CudaReconstHost failed: %s\n", RecviewLastMessage());
+                return 4;
+            }
+        }
+        const double ms =
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / repeat;
+        std::printf("ms_per_call %.3f\n", ms);
     }
     FILE* f = std::fopen(argv[6], "wb");
     if (!f || std::fwrite(image, sizeof(int), (size_t)N * N, f) != (size_t)N * N) return 5;

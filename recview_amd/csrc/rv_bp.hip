@@ -34,6 +34,9 @@
 //                in the reference's FP32 operation order.
 //  BP_LUT_I32  : the reference LUT in int32 (x BACKPROJ_SCALE), int32 sums --
 //                bit-exact with the reference integer path.
+//                Both LUT modes stage one pair (p0, p1p0) per sample (8 B) and
+//                form entry s of sample j as p0 + p1p0 * s at use, the float
+//                operations of the reference's LUT build.
 // This file is compiled with -ffp-contract=off; FMAs are written explicitly.
 #include "rv_internal.h"
 #include <hip/hip_fp16.h>
@@ -63,8 +66,7 @@ constexpr bool is_linear(int mode)
 // LDS bytes per staged angle
 constexpr int entry_bytes(int mode, int seg)
 {
-    return (mode == BP_LINEAR || mode == BP_LINEAR2) ? seg * 8 * bp_slices_per_item(mode)
-                                                                           : (mode == BP_LINEAR_H ? seg * 4 : seg * 16 * 4);
+    return (mode == BP_LINEAR_H) ? seg * 4 : seg * 8 * bp_slices_per_item(mode);
 }
 
 __device__ __forceinline__ float load_row_val(const void* rows, const float* scales, const BpSlice& S, int row,
@@ -107,7 +109,6 @@ __global__ void __launch_bounds__(256) bp_tiled_kernel(BpLaunch L)
     using C = TileCfg<PX, PY>;
     using Acc = typename AccT<MODE>::type;
     constexpr int SEG = C::SEG;
-    constexpr int EPA = is_linear(MODE) ? SEG : 16 * SEG;   // entries per angle
     constexpr int NS = bp_slices_per_item(MODE);             // slices per work item
     constexpr int NQ = NS / 2;                                // float4 per staged sample (multi-slice)
     const int chunk = L.chunk;
@@ -244,12 +245,15 @@ __global__ void __launch_bounds__(256) bp_tiled_kernel(BpLaunch L)
                     pairs[idx] = __halves2half2(__float2half_rn(pv), __float2half_rn(dv));
                 }
             } else {
-                Acc* lut = (Acc*)data;
+                // LUT modes: per sample the pair (p0, p1p0) of the reference LUT
+                // igp[16 j + s] = p0 + p1p0 * s; the entry is formed at use, with
+                // the same float operations, instead of staging 16 entries.
+                float2* lut = (float2*)data;
                 for (int idx = tid; idx < chunk * SEG; idx += C::T) {
                     const int a = idx / SEG, j = idx - a * SEG;
                     const int row = rws[a];
                     const int k = s0s[a] + j;
-                    Acc* dst = lut + (size_t)a * EPA + 16 * j;
+                    float2 e = make_float2(0.f, 0.f);
                     if (row >= 0 && k >= 0 && k < N) {
                         const float q0 = load_row_val(S, row, k);
                         const float q1 = (k < N - 1) ? load_row_val(S, row, k + 1) : 0.f;
@@ -266,15 +270,9 @@ __global__ void __launch_bounds__(256) bp_tiled_kernel(BpLaunch L)
                             p0 = q0;
                             p1p0 = (k == N - 1) ? 0.0f : (q1 - p0) / 16.f;
                         }
-#pragma unroll
-                        for (int s = 0; s < 16; s++) {
-                            const float v = p0 + p1p0 * (float)s;
-                            dst[s] = (MODE == BP_LUT_I32) ? (Acc)(int)v : (Acc)v;
-                        }
-                    } else {
-#pragma unroll
-                        for (int s = 0; s < 16; s++) dst[s] = (Acc)0;
+                        e = make_float2(p0, p1p0);
                     }
+                    lut[idx] = e;
                 }
             }
             __syncthreads();
@@ -351,11 +349,11 @@ __global__ void __launch_bounds__(256) bp_tiled_kernel(BpLaunch L)
 #pragma unroll
                     for (int i = 0; i < PX; i++) acc[j][i] = fmaf(cacc[j][i], cs, acc[j][i]);
             } else {
-                const Acc* lut = (const Acc*)data;
+                const float2* lut = (const float2*)data;
                 for (int a = 0; a < chunk; a++) {
                     const float4 tb = tab[a];
                     const int base = 16 * s0s[a];
-                    const Acc* la = lut + (size_t)a * EPA;
+                    const float2* la = lut + (size_t)a * SEG;
 #pragma unroll
                     for (int j = 0; j < PY; j++) {
                         const float fy = (float)(Y0 + (int)dyl + 8 * j);
@@ -365,7 +363,10 @@ __global__ void __launch_bounds__(256) bp_tiled_kernel(BpLaunch L)
                             const float fx = (float)(X0 + (int)dxl + 8 * i);
                             const float tt = fx * tb.x + rt;            // ix*fcos + (...)
                             const int idx = (L.rule == 0) ? (int)floorf(tt) : (int)tt;
-                            const Acc v = la[idx - base];
+                            const int rel = idx - base;              // >= 0 by the choice of s0
+                            const float2 e = la[rel >> 4];
+                            const float lv = e.x + e.y * (float)(rel & 15);
+                            const Acc v = (MODE == BP_LUT_I32) ? (Acc)(int)lv : (Acc)lv;
                             if (MODE == BP_LUT_I32)
                                 acc[j][i] = (Acc)((unsigned int)acc[j][i] + (unsigned int)v);
                             else
@@ -444,7 +445,7 @@ int bp_default_chunk(int mode, int tile)
     if (mode == BP_LINEAR) return (tile == 2) ? 64 : 32;
     if (mode == BP_LINEAR2) return (tile == 2) ? 32 : 16;
     if (mode == BP_LINEAR_H) return (tile == 2) ? 64 : 32;
-    return (tile == 2) ? 16 : 8;
+    return (tile == 2) ? 64 : 32;
 }
 
 int bp_max_chunk(int mode) { return mode == BP_LINEAR_H ? 64 : 256; }

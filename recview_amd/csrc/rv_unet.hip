@@ -4,7 +4,7 @@
 //   forward / data gradient: M = 16 pixels of an image row, N = 16 output
 //   channels, K = 32 input channels of one tap.  A block owns a 16 x 16 pixel
 //   patch and 16*NF output channels; per 32-channel chunk it stages the 18 x 18
-//   halo (80-byte pixel pitch, conflict-free 16-byte reads) and the chunk's
+//   halo (96-byte pixel pitch, conflict-free 16-byte reads) and the chunk's
 //   weights (already in fragment order) in LDS, then each wave runs
 //   9 taps x 4 rows x NF fragments of MFMA.
 //   weight gradient: M = 16 output channels, N = 16 input channels, K = 32
@@ -34,7 +34,10 @@ static __device__ __forceinline__ uint4 ld16(const uint16_t* p) { return *(const
 static __device__ __forceinline__ void st16(uint16_t* p, uint4 v) { *(uint4*)p = v; }
 
 // ------------------------------------------------------------ conv forward
-constexpr int HALO_PITCH = 40;          // bf16 per staged pixel: 32 channels + 8 pad
+// Row pitches of LDS tiles read as 16-byte fragments (16 rows x 4 groups per
+// wave): measured conflict-free on gfx950 when the pitch is 32 mod 64 bytes
+// (80, 112, 144, 272, 336 bytes conflict 2x, 128 bytes 4x).
+constexpr int HALO_PITCH = 48;          // bf16 per staged pixel: 32 channels + 16 pad
 constexpr int CONV_ROWS = 16;           // patch rows (4 per wave)
 constexpr int HALO_ROWS = CONV_ROWS + 2;
 constexpr int HALO_PIX = HALO_ROWS * 18;
@@ -115,18 +118,20 @@ __global__ __launch_bounds__(256, 2) void conv3x3_kernel(ConvArgs a) {
 #pragma unroll
         for (int tap = 0; tap < 9; tap++) {
             const int ky = tap / 3, kx = tap % 3;
-            bf16x8 av[4];
+            bf16x8 av[4], bv[NF];
 #pragma unroll
             for (int m = 0; m < 4; m++)
                 av[m] = *(const bf16x8*)(in_s + ((4 * w + m + ky) * 18 + (l & 15) + kx) * HALO_PITCH +
                                          8 * (l >> 4));
 #pragma unroll
-            for (int f = 0; f < NF; f++) {
-                const bf16x8 bv = *(const bf16x8*)(w_s + ((f * 9 + tap) * 64 + l) * 8);
+            for (int f = 0; f < NF; f++) bv[f] = *(const bf16x8*)(w_s + ((f * 9 + tap) * 64 + l) * 8);
+            // all fragment reads of the tap are in flight before its first MFMA
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int f = 0; f < NF; f++)
 #pragma unroll
                 for (int m = 0; m < 4; m++)
-                    acc[m][f] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[m], bv, acc[m][f], 0, 0, 0);
-            }
+                    acc[m][f] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[m], bv[f], acc[m][f], 0, 0, 0);
         }
         __syncthreads();
     }
@@ -260,18 +265,19 @@ __global__ __launch_bounds__(256, 2) void conv3x3_c32_kernel(ConvArgs a) {
 #pragma unroll
         for (int tap = 0; tap < 9; tap++) {
             const int ky = tap / 3, kx = tap % 3;
-            bf16x8 av[4];
+            bf16x8 av[4], bv[NF];
 #pragma unroll
             for (int m = 0; m < 4; m++)
                 av[m] = *(const bf16x8*)(in_s + ((4 * w + m + ky) * 18 + (l & 15) + kx) * HALO_PITCH +
                                          8 * (l >> 4));
 #pragma unroll
-            for (int f = 0; f < NF; f++) {
-                const bf16x8 bv = *(const bf16x8*)(w_s + ((f * 9 + tap) * 64 + l) * 8);
+            for (int f = 0; f < NF; f++) bv[f] = *(const bf16x8*)(w_s + ((f * 9 + tap) * 64 + l) * 8);
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int f = 0; f < NF; f++)
 #pragma unroll
                 for (int m = 0; m < 4; m++)
-                    acc[m][f] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[m], bv, acc[m][f], 0, 0, 0);
-            }
+                    acc[m][f] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[m], bv[f], acc[m][f], 0, 0, 0);
         }
         // bias, ReLU, bf16 into the output tile [256 pixels][TC]
 #pragma unroll
@@ -354,8 +360,8 @@ hipError_t launch_conv3x3(const ConvArgs& a, hipStream_t s) {
 }
 
 // ------------------------------------------------------------ weight gradient
-constexpr int DY_PITCH = 136;   // bf16 per staged output channel: 128 pixels + 8 pad
-constexpr int XT_PITCH = 168;   // bf16 per staged (shift, channel): 10 x 16 pixels + 8 pad
+constexpr int DY_PITCH = 144;   // bf16 per staged output channel: 128 pixels + 16 pad
+constexpr int XT_PITCH = 176;   // bf16 per staged (shift, channel): 10 x 16 pixels + 16 pad
 
 template <int NFW>
 __global__ __launch_bounds__(256) void conv3x3_wgrad_kernel(WgradArgs a) {

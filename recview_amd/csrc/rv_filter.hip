@@ -283,17 +283,22 @@ __device__ __forceinline__ float2 cmulf(float2 a, float2 b)
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
-template <int R, int NB>
-__device__ __forceinline__ void stockham_pass(float2* buf, const float2* twf, int n, int Ns, int T, int tid)
+// Forward DFT of buf, n = 2^NP points fixed at compile time (every stride,
+// twiddle step and loop bound is a constant); from 1024 points on T = n/16
+// threads cover each pass's butterflies exactly (no lane guards).
+constexpr int fft_threads(int np) { return (1 << np) / 16 < 64 ? 64 : (1 << np) / 16; }
+
+template <int R, int NP, int NS>
+__device__ __forceinline__ void stockham_pass_ct(float2* buf, const float2* twf, int tid)
 {
-    const int nb = n / R;
-    const int tstep = n / (Ns * R);
+    constexpr int n = 1 << NP, T = fft_threads(NP), nb = n / R, NB = (nb + T - 1) / T;
+    constexpr int tstep = n / (NS * R);
     float2 v[NB][R];
 #pragma unroll
     for (int i = 0; i < NB; i++) {
         const int j = tid + i * T;
-        if (j < nb) {
-            const int jm = j & (Ns - 1);
+        if (nb % T == 0 || j < nb) {
+            const int jm = j & (NS - 1);
 #pragma unroll
             for (int r = 0; r < R; r++) {
                 float2 x = buf[j + r * nb];
@@ -306,41 +311,53 @@ __device__ __forceinline__ void stockham_pass(float2* buf, const float2* twf, in
 #pragma unroll
     for (int i = 0; i < NB; i++) {
         const int j = tid + i * T;
-        if (j < nb) {
-            const int jm = j & (Ns - 1);
+        if (nb % T == 0 || j < nb) {
+            const int jm = j & (NS - 1);
             const int d = (j - jm) * R + jm;
             if (R == 2) {
                 buf[d] = make_float2(v[i][0].x + v[i][1].x, v[i][0].y + v[i][1].y);
-                buf[d + Ns] = make_float2(v[i][0].x - v[i][1].x, v[i][0].y - v[i][1].y);
+                buf[d + NS] = make_float2(v[i][0].x - v[i][1].x, v[i][0].y - v[i][1].y);
             } else {
                 const float2 t0 = make_float2(v[i][0].x + v[i][2].x, v[i][0].y + v[i][2].y);
                 const float2 t1 = make_float2(v[i][0].x - v[i][2].x, v[i][0].y - v[i][2].y);
                 const float2 t2 = make_float2(v[i][1].x + v[i][3].x, v[i][1].y + v[i][3].y);
                 const float2 t3 = make_float2(v[i][1].x - v[i][3].x, v[i][1].y - v[i][3].y);
                 buf[d] = make_float2(t0.x + t2.x, t0.y + t2.y);
-                buf[d + Ns] = make_float2(t1.x + t3.y, t1.y - t3.x);          // t1 - i t3
-                buf[d + 2 * Ns] = make_float2(t0.x - t2.x, t0.y - t2.y);
-                buf[d + 3 * Ns] = make_float2(t1.x - t3.y, t1.y + t3.x);      // t1 + i t3
+                buf[d + NS] = make_float2(t1.x + t3.y, t1.y - t3.x);          // t1 - i t3
+                buf[d + 2 * NS] = make_float2(t0.x - t2.x, t0.y - t2.y);
+                buf[d + 3 * NS] = make_float2(t1.x - t3.y, t1.y + t3.x);      // t1 + i t3
             }
         }
     }
     __syncthreads();
 }
 
-// Forward DFT of buf (n = 2^np points, T threads, n <= 16 T).
-__device__ __forceinline__ void fft_radix4(float2* buf, const float2* twf, int n, int np, int T, int tid)
+template <int NP, int NS>
+__device__ __forceinline__ void radix4_passes_ct(float2* buf, const float2* twf, int tid)
 {
-    int Ns = 1;
-    if (np & 1) {
-        stockham_pass<2, 8>(buf, twf, n, Ns, T, tid);
-        Ns = 2;
+    if constexpr (NS < (1 << NP)) {
+        stockham_pass_ct<4, NP, NS>(buf, twf, tid);
+        radix4_passes_ct<NP, NS * 4>(buf, twf, tid);
     }
-    for (; Ns < n; Ns *= 4) stockham_pass<4, 4>(buf, twf, n, Ns, T, tid);
 }
 
-template <int IN_KIND>
+template <int NP>
+__device__ __forceinline__ void fft_radix4_ct(float2* buf, const float2* twf, int tid)
+{
+    if constexpr (NP & 1) {
+        stockham_pass_ct<2, NP, 1>(buf, twf, tid);
+        radix4_passes_ct<NP, 2>(buf, twf, tid);
+    } else {
+        radix4_passes_ct<NP, 1>(buf, twf, tid);
+    }
+}
+
+// NP: log2 of the transform length (compile time); launched with
+// fft_threads(NP) threads.
+template <int IN_KIND, int NP>
 __global__ void __launch_bounds__(1024) filter_pair_kernel(FilterJob J, FilterBatch FB)
 {
+    constexpr int n = 1 << NP, T = fft_threads(NP);
     extern __shared__ float2 buf[];
     const int ra = 2 * (int)blockIdx.x, bi = blockIdx.y;
     const void* sino = FB.sino[bi];
@@ -349,7 +366,7 @@ __global__ void __launch_bounds__(1024) filter_pair_kernel(FilterJob J, FilterBa
     const int rout = J.row_out ? J.row_out[ra] : ra;
     const int rin_b = hasb ? (J.row_in ? J.row_in[ra + 1] : ra + 1) : 0;
     const int rout_b = hasb ? (J.row_out ? J.row_out[ra + 1] : ra + 1) : 0;
-    const int n = J.n, np = J.np, T = blockDim.x, tid = threadIdx.x;
+    const int tid = threadIdx.x;
     const float2* twf = (const float2*)J.tw;
     const void* row = (IN_KIND == 0)
         ? (const void*)((const int16_t*)sino + (long long)rin * J.in_stride)
@@ -360,14 +377,14 @@ __global__ void __launch_bounds__(1024) filter_pair_kernel(FilterJob J, FilterBa
     for (int m = tid; m < n; m += T)
         buf[m] = make_float2(prep_value<IN_KIND>(J, row, m), hasb ? prep_value<IN_KIND>(J, row_b, m) : 0.f);
     __syncthreads();
-    fft_radix4(buf, twf, n, np, T, tid);
+    fft_radix4_ct<NP>(buf, twf, tid);
     for (int k = tid; k < n; k += T) {            // conj(Hs * X)
         const float h = J.filt[k];
         const float2 x = buf[k];
         buf[k] = make_float2(x.x * h, -(x.y * h));
     }
     __syncthreads();
-    fft_radix4(buf, twf, n, np, T, tid);
+    fft_radix4_ct<NP>(buf, twf, tid);
     // y = conj(buf) / n; rows: real part -> a, imaginary part -> b
     const float a = 1.f / (float)n;
     if (J.store == 0) {
@@ -419,11 +436,20 @@ void prepare_filter(int n)
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     (void)hipFuncSetAttribute((const void*)&filter_rows_kernel<1>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    (void)hipFuncSetAttribute((const void*)&filter_pair_kernel<0>,
+    (void)hipFuncSetAttribute((const void*)&filter_pair_kernel<0, 14>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    (void)hipFuncSetAttribute((const void*)&filter_pair_kernel<1>,
+    (void)hipFuncSetAttribute((const void*)&filter_pair_kernel<1, 14>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     done[dev] = n;
+}
+
+template <int NP>
+static void launch_pair(const FilterJob& job, const FilterBatch& fb, dim3 g, size_t lds, hipStream_t stream)
+{
+    if (job.in_kind == 0)
+        hipLaunchKernelGGL((filter_pair_kernel<0, NP>), g, dim3(fft_threads(NP)), lds, stream, job, fb);
+    else
+        hipLaunchKernelGGL((filter_pair_kernel<1, NP>), g, dim3(fft_threads(NP)), lds, stream, job, fb);
 }
 
 void launch_filter_batch(const FilterJob& job, const FilterBatch& fb, int count, hipStream_t stream)
@@ -437,10 +463,21 @@ void launch_filter_batch(const FilterJob& job, const FilterBatch& fb, int count,
     if (job.nrows <= 0 || count <= 0 || count > kFilterBatch) return;
     if (job.pair) {
         const dim3 g((job.nrows + 1) / 2, count);
-        if (job.in_kind == 0)
-            hipLaunchKernelGGL(filter_pair_kernel<0>, g, dim3(T), lds, stream, job, fb);
-        else
-            hipLaunchKernelGGL(filter_pair_kernel<1>, g, dim3(T), lds, stream, job, fb);
+        switch (job.np) {
+        case 3: launch_pair<3>(job, fb, g, lds, stream); break;
+        case 4: launch_pair<4>(job, fb, g, lds, stream); break;
+        case 5: launch_pair<5>(job, fb, g, lds, stream); break;
+        case 6: launch_pair<6>(job, fb, g, lds, stream); break;
+        case 7: launch_pair<7>(job, fb, g, lds, stream); break;
+        case 8: launch_pair<8>(job, fb, g, lds, stream); break;
+        case 9: launch_pair<9>(job, fb, g, lds, stream); break;
+        case 10: launch_pair<10>(job, fb, g, lds, stream); break;
+        case 11: launch_pair<11>(job, fb, g, lds, stream); break;
+        case 12: launch_pair<12>(job, fb, g, lds, stream); break;
+        case 13: launch_pair<13>(job, fb, g, lds, stream); break;
+        case 14: launch_pair<14>(job, fb, g, lds, stream); break;
+        default: break;
+        }
     } else if (job.in_kind == 0) {
         hipLaunchKernelGGL(filter_rows_kernel<0>, dim3(job.nrows, count), dim3(T), lds, stream, job, fb);
     } else {

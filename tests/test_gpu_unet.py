@@ -134,6 +134,23 @@ def test_network_forward_matches_oracle():
     net.close()
 
 
+def test_inference_forward_equals_training_forward():
+    """Inference lets each decoder convolution read its upsampled half in place;
+    the training forward materialises the concatenation.  Same arithmetic, so
+    with a zero target the training loss mean(y^2) equals that of the
+    inference output."""
+    net = RU.Unet(32, 4, seed=7)
+    rng = np.random.default_rng(3)
+    x = torch.from_numpy(_images(rng, 2, 32, 48).astype(np.float32)).to(DEV)
+    out = net.forward(x, 1.0)
+    torch.cuda.synchronize()
+    y = out.double().cpu().numpy()
+    loss, _ = net.loss_grad(x, torch.zeros_like(x), 1.0)
+    ref = float(np.mean(y * y))
+    assert abs(loss - ref) <= 1e-6 * ref, (loss, ref)
+    net.close()
+
+
 def test_network_gradients_match_oracle():
     net = RU.Unet(32, 4, seed=4)
     rng = np.random.default_rng(1)

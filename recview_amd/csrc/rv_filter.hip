@@ -173,7 +173,11 @@ __device__ void store_lp_row(const FilterJob& J, void* out_lp, float* out_scale,
     __syncthreads();
 }
 
-template <int IN_KIND>
+constexpr int fft_threads(int np) { return (1 << np) / 16 < 64 ? 64 : (1 << np) / 16; }
+
+// NP: log2 of the transform length (compile time, so every stage's strides
+// and twiddle steps are constants); launched with fft_threads(NP) threads.
+template <int IN_KIND, int NP>
 __global__ void __launch_bounds__(1024) filter_rows_kernel(FilterJob J, FilterBatch FB)
 {
     extern __shared__ float2 buf[];
@@ -182,7 +186,8 @@ __global__ void __launch_bounds__(1024) filter_rows_kernel(FilterJob J, FilterBa
     float* out = FB.out[bi];
     const int rin = J.row_in ? J.row_in[r] : r;
     const int rout = J.row_out ? J.row_out[r] : r;
-    const int n = J.n, half = n >> 1, np = J.np, T = blockDim.x, tid = threadIdx.x;
+    constexpr int n = 1 << NP, half = n >> 1, np = NP, T = fft_threads(NP);
+    const int tid = threadIdx.x;
     const float2* tw = (const float2*)J.tw;
     const void* row = (IN_KIND == 0)
         ? (const void*)((const int16_t*)sino + (long long)rin * J.in_stride)
@@ -194,6 +199,7 @@ __global__ void __launch_bounds__(1024) filter_rows_kernel(FilterJob J, FilterBa
 
     // ---- FFT1Rev (cfft.cpp:431-434 -> FFTrev with nmin < 0)
     fft_first_stage<true>(buf, half, T, tid);
+#pragma unroll
     for (int l = 1; l < np - 1; l++) fft_mid_stage(buf, tw, half, l, T, tid);
     {
         // last stage (cfft.cpp:249-268) fused with p[k] *= fFilter[k]
@@ -224,6 +230,7 @@ __global__ void __launch_bounds__(1024) filter_rows_kernel(FilterJob J, FilterBa
 
     // ---- FFT1 (cfft.cpp:412-429 -> FFT with nmin < 0, then reverse and 1/n)
     fft_first_stage<false>(buf, half, T, tid);
+#pragma unroll
     for (int l = 1; l < np - 1; l++) fft_mid_stage(buf, tw, half, l, T, tid);
     {
         float2 A[kMaxBfly], B[kMaxBfly];
@@ -286,8 +293,6 @@ __device__ __forceinline__ float2 cmulf(float2 a, float2 b)
 // Forward DFT of buf, n = 2^NP points fixed at compile time (every stride,
 // twiddle step and loop bound is a constant); from 1024 points on T = n/16
 // threads cover each pass's butterflies exactly (no lane guards).
-constexpr int fft_threads(int np) { return (1 << np) / 16 < 64 ? 64 : (1 << np) / 16; }
-
 template <int R, int NP, int NS>
 __device__ __forceinline__ void stockham_pass_ct(float2* buf, const float2* twf, int tid)
 {
@@ -432,9 +437,9 @@ void prepare_filter(int n)
     (void)hipGetDevice(&dev);
     std::lock_guard<std::mutex> lk(mu);
     if (lds <= 65536 || n <= done[dev]) return;
-    (void)hipFuncSetAttribute((const void*)&filter_rows_kernel<0>,
+    (void)hipFuncSetAttribute((const void*)&filter_rows_kernel<0, 14>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    (void)hipFuncSetAttribute((const void*)&filter_rows_kernel<1>,
+    (void)hipFuncSetAttribute((const void*)&filter_rows_kernel<1, 14>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     (void)hipFuncSetAttribute((const void*)&filter_pair_kernel<0, 14>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -444,44 +449,43 @@ void prepare_filter(int n)
 }
 
 template <int NP>
-static void launch_pair(const FilterJob& job, const FilterBatch& fb, dim3 g, size_t lds, hipStream_t stream)
+static void launch_np(const FilterJob& job, const FilterBatch& fb, int count, size_t lds, hipStream_t stream)
 {
-    if (job.in_kind == 0)
-        hipLaunchKernelGGL((filter_pair_kernel<0, NP>), g, dim3(fft_threads(NP)), lds, stream, job, fb);
-    else
-        hipLaunchKernelGGL((filter_pair_kernel<1, NP>), g, dim3(fft_threads(NP)), lds, stream, job, fb);
+    const dim3 T(fft_threads(NP));
+    if (job.pair) {
+        const dim3 g((job.nrows + 1) / 2, count);
+        if (job.in_kind == 0)
+            hipLaunchKernelGGL((filter_pair_kernel<0, NP>), g, T, lds, stream, job, fb);
+        else
+            hipLaunchKernelGGL((filter_pair_kernel<1, NP>), g, T, lds, stream, job, fb);
+    } else {
+        const dim3 g(job.nrows, count);
+        if (job.in_kind == 0)
+            hipLaunchKernelGGL((filter_rows_kernel<0, NP>), g, T, lds, stream, job, fb);
+        else
+            hipLaunchKernelGGL((filter_rows_kernel<1, NP>), g, T, lds, stream, job, fb);
+    }
 }
 
 void launch_filter_batch(const FilterJob& job, const FilterBatch& fb, int count, hipStream_t stream)
 {
-    const int half = job.n / 2;
-    int T = half / kMaxBfly;
-    if (T < 64) T = 64;
-    if (T > 1024) T = 1024;
     const size_t lds = sizeof(float2) * (size_t)job.n;
     prepare_filter(job.n);
-    if (job.nrows <= 0 || count <= 0 || count > kFilterBatch) return;
-    if (job.pair) {
-        const dim3 g((job.nrows + 1) / 2, count);
-        switch (job.np) {
-        case 3: launch_pair<3>(job, fb, g, lds, stream); break;
-        case 4: launch_pair<4>(job, fb, g, lds, stream); break;
-        case 5: launch_pair<5>(job, fb, g, lds, stream); break;
-        case 6: launch_pair<6>(job, fb, g, lds, stream); break;
-        case 7: launch_pair<7>(job, fb, g, lds, stream); break;
-        case 8: launch_pair<8>(job, fb, g, lds, stream); break;
-        case 9: launch_pair<9>(job, fb, g, lds, stream); break;
-        case 10: launch_pair<10>(job, fb, g, lds, stream); break;
-        case 11: launch_pair<11>(job, fb, g, lds, stream); break;
-        case 12: launch_pair<12>(job, fb, g, lds, stream); break;
-        case 13: launch_pair<13>(job, fb, g, lds, stream); break;
-        case 14: launch_pair<14>(job, fb, g, lds, stream); break;
-        default: break;
-        }
-    } else if (job.in_kind == 0) {
-        hipLaunchKernelGGL(filter_rows_kernel<0>, dim3(job.nrows, count), dim3(T), lds, stream, job, fb);
-    } else {
-        hipLaunchKernelGGL(filter_rows_kernel<1>, dim3(job.nrows, count), dim3(T), lds, stream, job, fb);
+    if (job.nrows <= 0 || count <= 0 || count > kFilterBatch || job.n != (1 << job.np)) return;
+    switch (job.np) {
+    case 3: launch_np<3>(job, fb, count, lds, stream); break;
+    case 4: launch_np<4>(job, fb, count, lds, stream); break;
+    case 5: launch_np<5>(job, fb, count, lds, stream); break;
+    case 6: launch_np<6>(job, fb, count, lds, stream); break;
+    case 7: launch_np<7>(job, fb, count, lds, stream); break;
+    case 8: launch_np<8>(job, fb, count, lds, stream); break;
+    case 9: launch_np<9>(job, fb, count, lds, stream); break;
+    case 10: launch_np<10>(job, fb, count, lds, stream); break;
+    case 11: launch_np<11>(job, fb, count, lds, stream); break;
+    case 12: launch_np<12>(job, fb, count, lds, stream); break;
+    case 13: launch_np<13>(job, fb, count, lds, stream); break;
+    case 14: launch_np<14>(job, fb, count, lds, stream); break;
+    default: break;
     }
 }
 

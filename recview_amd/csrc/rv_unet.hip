@@ -44,18 +44,23 @@ constexpr int HALO_PIX = HALO_ROWS * 18;
 constexpr int HALO_ITEMS = HALO_PIX * 4;          // 16-byte loads per chunk
 constexpr int HALO_PER = (HALO_ITEMS + 255) / 256;
 
-// One block: a 16 x 16 pixel patch of one image and 16*NF output channels.
-// Wave w owns patch rows 4w..4w+3 (four M fragments) and all NF N fragments,
-// so per tap it reads 4 A and NF B fragments for 4*NF MFMAs.  The next
-// 32-channel chunk is fetched into registers while the current one is
+// One block: a 16-wide, 4*MR-row pixel patch of one image and 16*NF output
+// channels.  Wave w owns patch rows MR*w .. MR*w+MR-1 (MR M fragments) and all
+// NF N fragments, so per tap it reads MR A and NF B fragments for MR*NF MFMAs.
+// The next 32-channel chunk is fetched into registers while the current one is
 // multiplied; the epilogue goes through LDS so every store is 16 bytes.
-template <int NF>
+template <int NF, int MR>
 __global__ __launch_bounds__(256, 2) void conv3x3_kernel(ConvArgs a) {
-    constexpr int HSZ = HALO_PIX * HALO_PITCH;
+    constexpr int ROWS = 4 * MR;
+    constexpr int HALO_PIX_K = (ROWS + 2) * 18;
+    constexpr int HALO_ITEMS_K = HALO_PIX_K * 4;
+    constexpr int HALO_PER_K = (HALO_ITEMS_K + 255) / 256;
+    constexpr int NPIX = 16 * ROWS;
+    constexpr int HSZ = HALO_PIX_K * HALO_PITCH;
     constexpr int WSZ = NF * 576 * 8;
     constexpr int TC = 16 * NF;
     constexpr int OPITCH = TC + 8;
-    constexpr int OSZ = 256 * OPITCH;
+    constexpr int OSZ = NPIX * OPITCH;
     constexpr int SMEM = HSZ + WSZ > OSZ ? HSZ + WSZ : OSZ;
     constexpr int W_ITEMS = NF * 576;
     constexpr int W_PER = (W_ITEMS + 255) / 256;
@@ -63,28 +68,28 @@ __global__ __launch_bounds__(256, 2) void conv3x3_kernel(ConvArgs a) {
     uint16_t* in_s = smem;
     uint16_t* w_s = smem + HSZ;
     const int tiles_x = (a.W + 15) >> 4;
-    const int x0 = (blockIdx.x % tiles_x) * 16, y0 = (blockIdx.x / tiles_x) * CONV_ROWS;
+    const int x0 = (blockIdx.x % tiles_x) * 16, y0 = (blockIdx.x / tiles_x) * ROWS;
     const int bb = blockIdx.z, f0 = blockIdx.y * NF;
     const int t = threadIdx.x, l = t & 63, w = t >> 6;
     const int nch = a.Cin >> 5;
     const uint16_t* in = a.in + (long long)bb * a.H * a.W * a.in_stride + a.in_off;
     const uint16_t* up = a.up ? a.up + (long long)bb * (a.H >> 1) * (a.W >> 1) * a.Cup : nullptr;
-    floatx4 acc[4][NF];
+    floatx4 acc[MR][NF];
 #pragma unroll
-    for (int m = 0; m < 4; m++)
+    for (int m = 0; m < MR; m++)
 #pragma unroll
         for (int f = 0; f < NF; f++) acc[m][f] = floatx4{0.f, 0.f, 0.f, 0.f};
 
-    uint4 hreg[HALO_PER], wreg[W_PER];
+    uint4 hreg[HALO_PER_K], wreg[W_PER];
     auto fetch = [&](int cc) {
 #pragma unroll
-        for (int k = 0; k < HALO_PER; k++) {
+        for (int k = 0; k < HALO_PER_K; k++) {
             const int i = t + 256 * k;
             const int hp = i >> 2, g = i & 3;
             const int hy = hp / 18, hx = hp - hy * 18;
             const int y = y0 - 1 + hy, x = x0 - 1 + hx;
             hreg[k] = make_uint4(0, 0, 0, 0);
-            if (i < HALO_ITEMS && y >= 0 && y < a.H && x >= 0 && x < a.W) {
+            if (i < HALO_ITEMS_K && y >= 0 && y < a.H && x >= 0 && x < a.W) {
                 if (cc * 32 < a.Cup)
                     hreg[k] = ld16(up + ((long long)(y >> 1) * (a.W >> 1) + (x >> 1)) * a.Cup + cc * 32 +
                                    g * 8);
@@ -104,9 +109,9 @@ __global__ __launch_bounds__(256, 2) void conv3x3_kernel(ConvArgs a) {
     fetch(0);
     for (int cc = 0; cc < nch; cc++) {
 #pragma unroll
-        for (int k = 0; k < HALO_PER; k++) {
+        for (int k = 0; k < HALO_PER_K; k++) {
             const int i = t + 256 * k;
-            if (i < HALO_ITEMS) st16(in_s + (i >> 2) * HALO_PITCH + (i & 3) * 8, hreg[k]);
+            if (i < HALO_ITEMS_K) st16(in_s + (i >> 2) * HALO_PITCH + (i & 3) * 8, hreg[k]);
         }
 #pragma unroll
         for (int k = 0; k < W_PER; k++) {
@@ -118,10 +123,10 @@ __global__ __launch_bounds__(256, 2) void conv3x3_kernel(ConvArgs a) {
 #pragma unroll
         for (int tap = 0; tap < 9; tap++) {
             const int ky = tap / 3, kx = tap % 3;
-            bf16x8 av[4], bv[NF];
+            bf16x8 av[MR], bv[NF];
 #pragma unroll
-            for (int m = 0; m < 4; m++)
-                av[m] = *(const bf16x8*)(in_s + ((4 * w + m + ky) * 18 + (l & 15) + kx) * HALO_PITCH +
+            for (int m = 0; m < MR; m++)
+                av[m] = *(const bf16x8*)(in_s + ((MR * w + m + ky) * 18 + (l & 15) + kx) * HALO_PITCH +
                                          8 * (l >> 4));
 #pragma unroll
             for (int f = 0; f < NF; f++) bv[f] = *(const bf16x8*)(w_s + ((f * 9 + tap) * 64 + l) * 8);
@@ -130,31 +135,31 @@ __global__ __launch_bounds__(256, 2) void conv3x3_kernel(ConvArgs a) {
 #pragma unroll
             for (int f = 0; f < NF; f++)
 #pragma unroll
-                for (int m = 0; m < 4; m++)
+                for (int m = 0; m < MR; m++)
                     acc[m][f] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[m], bv[f], acc[m][f], 0, 0, 0);
         }
         __syncthreads();
     }
-    // epilogue: bias, ReLU, bf16 into an LDS tile [256 pixels][TC], then 16-byte stores
+    // epilogue: bias, ReLU, bf16 into an LDS tile [NPIX pixels][TC], then 16-byte stores
     uint16_t* o_s = smem;
 #pragma unroll
     for (int f = 0; f < NF; f++) {
         const int co = f * 16 + (l & 15);
         const float bias = a.bias ? a.bias[(f0 * 16) + co] : 0.f;
 #pragma unroll
-        for (int m = 0; m < 4; m++)
+        for (int m = 0; m < MR; m++)
 #pragma unroll
             for (int i = 0; i < 4; i++) {
                 float v = acc[m][f][i] + bias;
                 if (a.relu) v = fmaxf(v, 0.f);
-                o_s[((4 * w + m) * 16 + 4 * (l >> 4) + i) * OPITCH + co] = f2bf(v);
+                o_s[((MR * w + m) * 16 + 4 * (l >> 4) + i) * OPITCH + co] = f2bf(v);
             }
     }
     __syncthreads();
     if (a.pool_out) {
-        // 8 x 8 pooled pixels of this patch, from the rounded outputs
+        // 8 x ROWS/2 pooled pixels of this patch, from the rounded outputs
         const int Ho = a.H >> 1, Wo = a.W >> 1;
-        for (int it = t; it < 64 * (TC / 8); it += 256) {
+        for (int it = t; it < 4 * ROWS * (TC / 8); it += 256) {
             const int q = it / (TC / 8), g = it - q * (TC / 8);
             const int py = q >> 3, pxx = q & 7;
             const int yo = (y0 >> 1) + py, xo = (x0 >> 1) + pxx;
@@ -175,7 +180,7 @@ __global__ __launch_bounds__(256, 2) void conv3x3_kernel(ConvArgs a) {
             st16(a.pool_out + (((long long)bb * Ho + yo) * Wo + xo) * a.Cout + f0 * 16 + 8 * g, r);
         }
     }
-    for (int it = t; it < 256 * (TC / 8); it += 256) {
+    for (int it = t; it < NPIX * (TC / 8); it += 256) {
         const int px = it / (TC / 8), g = it - px * (TC / 8);
         const int y = y0 + (px >> 4), x = x0 + (px & 15);
         if (y >= a.H || x >= a.W) continue;
@@ -348,14 +353,16 @@ hipError_t launch_conv3x3(const ConvArgs& a, hipStream_t s) {
         (a.pool_out && (a.H % 2 || a.W % 2)))
         return hipErrorInvalidValue;
     const int tiles = ((a.W + 15) / 16) * ((a.H + CONV_ROWS - 1) / CONV_ROWS);
+    const int tiles24 = ((a.W + 15) / 16) * ((a.H + 23) / 24);
     // 32 -> 32 layers (one chunk, 64 output channels of work per 16 KB of halo)
-    // walk four patches per block with the weights staged once.
+    // walk four patches per block with the weights staged once; other 32-output
+    // layers use 24-row patches (6 M fragments per wave against 2 N fragments).
     if (a.Cout % 64 == 0)
-        hipLaunchKernelGGL((conv3x3_kernel<4>), dim3(tiles, a.Cout / 64, a.B), dim3(256), 0, s, a);
+        hipLaunchKernelGGL((conv3x3_kernel<4, 4>), dim3(tiles, a.Cout / 64, a.B), dim3(256), 0, s, a);
     else if (a.Cin == 32)
         hipLaunchKernelGGL((conv3x3_c32_kernel<2, 4>), dim3((tiles + 3) / 4, a.Cout / 32, a.B), dim3(256), 0, s, a);
     else
-        hipLaunchKernelGGL((conv3x3_kernel<2>), dim3(tiles, a.Cout / 32, a.B), dim3(256), 0, s, a);
+        hipLaunchKernelGGL((conv3x3_kernel<2, 6>), dim3(tiles24, a.Cout / 32, a.B), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

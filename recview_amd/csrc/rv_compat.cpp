@@ -251,8 +251,7 @@ void reconst_body(RECONST_INFO* ri, int idev, bool bReport, bool bEnStream, bool
                 J.out = ri->d_px;
                 J.out_stride = N;
                 J.store = 0;
-                rv::launch_filter(J, st);
-                ok = rv::hip_ok(hipGetLastError(), "filter launch");
+                ok = rv::hip_ok(rv::launch_filter(J, st), "filter launch");
             }
             if (!ok) {
                 (void)hipStreamSynchronize(st);      // the staging buffer is reused by the next call
@@ -557,7 +556,10 @@ void CudaDeconv(int ixdim, int iIntpDim, int ndim, float center, float* d_filt, 
     J.tw = tw;
     J.out = (float*)d_p;
     J.out_stride = N;
-    rv::launch_filter(J, nullptr);
+    if (!rv::hip_ok(rv::launch_filter(J, nullptr), "filter launch")) {
+        rv::log_code(28804);
+        return;
+    }
     rv::launch_px2igp((const float*)d_p, d_igp, N, 1, 100.f, blocksize, nullptr);
 }
 

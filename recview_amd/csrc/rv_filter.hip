@@ -467,11 +467,13 @@ static void launch_np(const FilterJob& job, const FilterBatch& fb, int count, si
     }
 }
 
-void launch_filter_batch(const FilterJob& job, const FilterBatch& fb, int count, hipStream_t stream)
+hipError_t launch_filter_batch(const FilterJob& job, const FilterBatch& fb, int count, hipStream_t stream)
 {
+    if (job.nrows <= 0) return hipSuccess;
+    if (count <= 0 || count > kFilterBatch || job.np < 3 || job.np > 14 || job.n != (1 << job.np))
+        return hipErrorInvalidValue;
     const size_t lds = sizeof(float2) * (size_t)job.n;
     prepare_filter(job.n);
-    if (job.nrows <= 0 || count <= 0 || count > kFilterBatch || job.n != (1 << job.np)) return;
     switch (job.np) {
     case 3: launch_np<3>(job, fb, count, lds, stream); break;
     case 4: launch_np<4>(job, fb, count, lds, stream); break;
@@ -487,16 +489,17 @@ void launch_filter_batch(const FilterJob& job, const FilterBatch& fb, int count,
     case 14: launch_np<14>(job, fb, count, lds, stream); break;
     default: break;
     }
+    return hipGetLastError();
 }
 
-void launch_filter(const FilterJob& job, hipStream_t stream)
+hipError_t launch_filter(const FilterJob& job, hipStream_t stream)
 {
     FilterBatch fb{};
     fb.sino[0] = job.sino;
     fb.out[0] = job.out;
     fb.out_lp[0] = job.out_lp;
     fb.out_scale[0] = job.out_scale;
-    launch_filter_batch(job, fb, 1, stream);
+    return launch_filter_batch(job, fb, 1, stream);
 }
 
 // ------------------------------------------------ compatibility kernels

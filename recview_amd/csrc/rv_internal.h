@@ -35,7 +35,8 @@ struct FilterJob {
                            //    filter, tw = full table exp(-2 pi i k / n), k < n
 };
 
-void launch_filter(const FilterJob& job, hipStream_t stream);
+// Both return hipErrorInvalidValue for a job they cannot run (no launch).
+hipError_t launch_filter(const FilterJob& job, hipStream_t stream);
 // Several slices of identical filter geometry in one launch (grid y = slice):
 // the per-slice pointers come from the batch, every other field from the job.
 constexpr int kFilterBatch = 16;
@@ -45,7 +46,7 @@ struct FilterBatch {
     void* out_lp[kFilterBatch];
     float* out_scale[kFilterBatch];
 };
-void launch_filter_batch(const FilterJob& job, const FilterBatch& batch, int count, hipStream_t stream);
+hipError_t launch_filter_batch(const FilterJob& job, const FilterBatch& batch, int count, hipStream_t stream);
 void prepare_filter(int n);   // kernel attributes for an n-point transform (not capturable)
 
 // LUT build (cudaReconst.cu:230-246) and single-projection LUT back-projection

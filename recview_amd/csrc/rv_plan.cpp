@@ -176,8 +176,9 @@ int launch_filters(rv_plan_t p, const void* const* d_sino, hipStream_t st)
     rv::FilterJob head{};
     rv::FilterBatch fb{};
     int count = 0, cls = -1;
+    hipError_t err = hipSuccess;
     auto flush = [&]() {
-        if (count) rv::launch_filter_batch(head, fb, count, st);
+        if (count && err == hipSuccess) err = rv::launch_filter_batch(head, fb, count, st);
         count = 0;
     };
     for (size_t s = 0; s < p->sl.size(); s++) {
@@ -214,7 +215,7 @@ int launch_filters(rv_plan_t p, const void* const* d_sino, hipStream_t st)
         count++;
     }
     flush();
-    return rv::hip_ok(hipGetLastError(), "filter launch") ? 0 : -1;
+    return rv::hip_ok(err, "filter launch") && rv::hip_ok(hipGetLastError(), "filter launch") ? 0 : -1;
 }
 
 int launch_backprojection(rv_plan_t p, hipStream_t st)

@@ -4,9 +4,13 @@ load_output_log   -- SPring-8 output.log (gazoDoc.cpp:2298-2380): one line per
                      frame, "name exposure angle flag"; flag '1' = sample frame,
                      anything else = incident (white) frame; the dark frame is
                      appended by the caller (iLenSinogr = frames + 1).
-frames_to_exclude -- the default exclusion list for a log (gazoDoc.cpp:2411-2520,
-                     non-HDF5 branch): sample frames beyond one 180 (or 360)
-                     degree turn, as tokens " sNNNN".
+load_hdf5_log     -- APS DXchange HDF5 (gazoDoc.cpp:2229-2312): a step scan
+                     takes its angles from /exchange/theta, a fly scan (no
+                     usable theta) spreads 180 degrees over the frames of
+                     /exchange/data; both get a white entry before and after.
+frames_to_exclude -- the default exclusion list for a log (gazoDoc.cpp:2411-2520):
+                     sample frames beyond one 180 (or 360) degree turn, as
+                     tokens " sNNNN" (HDF5: frame numbers of the data block).
 apply_exclusions  -- turns an exclusion list into bInc flags and, for bare frame
                      numbers of sample frames, replaces the frame by the mean of
                      its nearest kept neighbours (gazoDoc.cpp:3372-3467).
@@ -32,6 +36,8 @@ class ScanLog:
     fdeg: np.ndarray          # float32 angles (degrees)
     binc: np.ndarray          # int8 flags
     len_sinogr: int = field(default=0)   # frames + dark
+    hdf5_frames: int = field(default=0)  # HDF5 logs: frames in /exchange/data
+    step_scan: bool = field(default=False)   # HDF5 logs: angles from /exchange/theta
 
 
 def _scan_line(line):
@@ -72,8 +78,42 @@ def load_output_log(source):
     return ScanLog(fname, fexp, fdeg, np.array(binc, np.int8), len(lines) + 1)
 
 
+def load_hdf5_log(path):
+    """Scan log of an APS DXchange HDF5 file (LoadLogFile, gazoDoc.cpp:2229-2312)."""
+    from .hdf5 import H5File, H5Error
+    with H5File(path) as f:
+        try:
+            dims = f.shape("/exchange/data")
+        except KeyError:
+            raise LogError(16052221, "no /exchange/data in the HDF5 file") from None
+        nframes = int(dims[0])
+        theta = None
+        try:
+            if f.exists("/exchange/theta") and len(f.shape("/exchange/theta")) == 1:
+                theta = f.read("/exchange/theta")
+        except H5Error:
+            theta = None                  # unreadable theta: handled as a fly scan, as the reference
+    if theta is not None:
+        n = len(theta) + 2
+        fdeg = np.zeros(n, np.float32)
+        fdeg[1:n - 1] = theta.astype(np.float32)              # (float) of float / double elements
+    else:
+        n = nframes + 2
+        fdeg = np.zeros(n, np.float32)
+        i = np.arange(1, n - 1)
+        fdeg[1:n - 1] = (np.float32(180) * i.astype(np.float32)) / np.float32(n - 3)   # 180.f*i/(ipos-2-1)
+    fdeg[0] = fdeg[1] - np.float32(1)
+    fdeg[n - 1] = fdeg[n - 2] + np.float32(1)
+    binc = np.full(n, BINC_SAMPLE, np.int8)
+    binc[0] = binc[n - 1] = BINC_WHITE
+    names = ["%05d" % i for i in range(n)]
+    return ScanLog(names, fdeg.copy(), fdeg, binc, n + 1, nframes, theta is not None)
+
+
 def frames_to_exclude(log):
     """(exclusion string, first sample frame, last sample frame) for a log."""
+    if log.hdf5_frames or log.step_scan:
+        return _frames_to_exclude_hdf5(log)
     fdeg, binc = log.fdeg, log.binc
     ipos = len(fdeg)
     idigit = int(math.log10(ipos)) + 1
@@ -105,6 +145,48 @@ def frames_to_exclude(log):
             else:
                 first = i
                 break
+    return out, first, last
+
+
+def _sample_span(log):
+    ipos = log.len_sinogr - 1
+    sample = (log.binc[:ipos] & BINC_SAMPLE) != 0
+    idx = np.nonzero(sample)[0]
+    start = log.fdeg[idx[0]] if len(idx) else np.float32(0)
+    end = log.fdeg[idx[-1]] if len(idx) else np.float32(180)
+    span = np.float32(np.float32(end) - np.float32(start)) - np.float32(360)
+    return ipos, sample, np.float32(start), (359.99 if abs(float(span)) < 10 else 179.99)
+
+
+def _frames_to_exclude_hdf5(log):
+    """The HDF5 branch of SetFramesToExclude (gazoDoc.cpp:2438-2468): frame numbers
+    of the data block; a fly scan also lists the averaged white frame (" b0")."""
+    ipos, sample, start, d_end = _sample_span(log)
+    fmt = " s%0" + str(int(math.log10(ipos)) + 1) + "d"
+    out, first, last = " ", 0, 0
+    if log.step_scan and log.hdf5_frames > 0:
+        first = log.hdf5_frames - (ipos - 2)
+        for i in range(first):
+            out += fmt % i
+        for i in range(1, ipos):
+            if not sample[i]:
+                continue
+            if float(np.float32(log.fdeg[i] - start)) >= d_end:
+                out += fmt % (i - 1 + first)
+            else:
+                last = i - 1 + first
+    else:
+        for i in range(ipos):
+            if sample[i] and log.fdeg[i] == start:
+                first = i - 1
+                break
+        for i in range(1, ipos):
+            if sample[i]:
+                if float(np.float32(log.fdeg[i] - start)) >= d_end:
+                    out += fmt % (i - 1)
+                else:
+                    last = i - 1
+        out += (" b%0" + str(int(math.log10(ipos)) + 1) + "d") % 0
     return out, first, last
 
 

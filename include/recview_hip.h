@@ -279,10 +279,11 @@ typedef struct {
     int exact_filter;    /* LINEAR modes: 1 keeps CFft's operation order (filtered rows bit-identical to
                           * the reference CPU filter); 0 (default) filters two rows per transform with
                           * the symmetrised filter, the same operator at half the cost */
-    int slices_per_item; /* LINEAR with FP32 pairs in LDS: slices of identical geometry back-projected
-                          * two per work item, sharing each pixel's coordinate and 16-byte LDS reads
-                          * (results identical to one slice per item): 0 (default) or 2 = pairs,
-                          * 1 = one slice per item */
+    int slices_per_item; /* LINEAR modes: slices of identical geometry back-projected together, sharing
+                          * each pixel's coordinate and LDS reads: 0 (default) = two per work item
+                          * (FP32 pairs in LDS: results identical to one per item; FP16 pairs with
+                          * lds_half: one FP16 dot product per update and one power-of-two scale per
+                          * work item); or 1, 2, 4 (4 only with FP16 pairs) */
 } RvOptions;
 
 typedef struct rv_plan_opaque* rv_plan_t;

@@ -29,6 +29,12 @@
 //  BP_LINEAR_H : FP16/BF16 filtered sinogram.  LDS holds FP16 pairs (p_k, d_k)
 //                rescaled to one power-of-two scale per chunk of angles, value =
 //                p + fract(u)*d (mixed-precision FMA), FP32 sums.  4 B/update.
+//  BP_LINEAR_H2/H4: BP_LINEAR_H for two / four slices of identical geometry:
+//                LDS holds the slices' FP16 pairs side by side (8 / 16 B per
+//                sample), rescaled by one power-of-two scale per work item (the
+//                largest row scale of its slices), and each update is one
+//                v_dot2_f32_f16: sum += (p, d) . (1, fract(u)) in FP32.  One
+//                coordinate, one fract and one (1, f) pack serve 2 / 4 updates.
 //  BP_LUT_F32  : the reference's 16-sub-sample LUT (cudaReconst.cu:230-240)
 //                in float, float sums; index t = ix*fcos + (iy*fsin + foffset)
 //                in the reference's FP32 operation order.
@@ -39,10 +45,13 @@
 //                operations of the reference's LUT build.
 // This file is compiled with -ffp-contract=off; FMAs are written explicitly.
 #include "rv_internal.h"
+#include <type_traits>
 #include <hip/hip_fp16.h>
 #include <hip/hip_bf16.h>
 
 namespace rv {
+
+typedef _Float16 half2v __attribute__((ext_vector_type(2)));
 
 template <int PX, int PY>
 struct TileCfg {
@@ -58,15 +67,18 @@ struct AccT { using type = float; };
 template <>
 struct AccT<BP_LUT_I32> { using type = int; };
 
+constexpr bool is_half_multi(int mode) { return mode == BP_LINEAR_H2 || mode == BP_LINEAR_H4; }
+
 constexpr bool is_linear(int mode)
 {
-    return mode == BP_LINEAR || mode == BP_LINEAR_H || mode == BP_LINEAR2;
+    return mode == BP_LINEAR || mode == BP_LINEAR_H || mode == BP_LINEAR2 || is_half_multi(mode);
 }
 
 // LDS bytes per staged angle
 constexpr int entry_bytes(int mode, int seg)
 {
-    return (mode == BP_LINEAR_H) ? seg * 4 : seg * 8 * bp_slices_per_item(mode);
+    return (mode == BP_LINEAR_H || is_half_multi(mode)) ? seg * 4 * bp_slices_per_item(mode)
+                                                         : seg * 8 * bp_slices_per_item(mode);
 }
 
 __device__ __forceinline__ float load_row_val(const void* rows, const float* scales, const BpSlice& S, int row,
@@ -77,6 +89,14 @@ __device__ __forceinline__ float load_row_val(const void* rows, const float* sca
     const float sc = scales[row];
     if (S.store == 1) return __half2float(((const __half*)rows)[off]) * sc;
     return __bfloat162float(((const __hip_bfloat16*)rows)[off]) * sc;
+}
+
+// 16-bit row value without its row scale
+__device__ __forceinline__ float raw_row_val(const void* rows, const BpSlice& S, int row, int k)
+{
+    const long long off = (long long)row * S.row_stride + k;
+    if (S.store == 1) return __half2float(((const __half*)rows)[off]);
+    return __bfloat162float(((const __hip_bfloat16*)rows)[off]);
 }
 
 __device__ __forceinline__ float load_row_val(const BpSlice& S, int row, int k)
@@ -122,6 +142,9 @@ __global__ void __launch_bounds__(256) bp_tiled_kernel(BpLaunch L)
     void* data = (void*)(lds4 + chunk + (2 * chunk + 3) / 4);
     __shared__ int item_sh;
     __shared__ float chunk_scale;
+    __shared__ float red_sh[4];
+    // FP16 multi-slice modes: per staged angle and slice, row scale / item scale
+    __shared__ float hmul[is_half_multi(MODE) ? 256 * 4 : 1];
 
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
@@ -167,6 +190,23 @@ __global__ void __launch_bounds__(256) bp_tiled_kernel(BpLaunch L)
                 for (int e = 0; e + 1 < NS; e++) accx[e][j][i] = (Acc)0;
             }
 
+        float gscale = 1.f;
+        if (is_half_multi(MODE)) {
+            // one power-of-two scale for the item: the largest row scale of its slices
+            float m = 0.f;
+            for (int a = tid; a < S.nang; a += C::T) {
+                const int row = S.ang[a].row;
+                m = fmaxf(m, S.scales[row]);
+#pragma unroll
+                for (int e = 0; e + 1 < NS; e++) m = fmaxf(m, S.scales_x[e][row]);
+            }
+            for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+            if (lane == 0) red_sh[wave] = m;
+            __syncthreads();
+            m = fmaxf(fmaxf(red_sh[0], red_sh[1]), fmaxf(red_sh[2], red_sh[3]));
+            gscale = (m > 0.f) ? m : 1.f;
+        }
+
         for (int a0 = 0; a0 < S.nang; a0 += chunk) {
             // ---- per-angle tile geometry
             if (tid < chunk) {
@@ -180,6 +220,12 @@ __global__ void __launch_bounds__(256) bp_tiled_kernel(BpLaunch L)
                         const float sc = (MODE == BP_LINEAR_H) ? S.scales[e.row] : 0.f;
                         tab[tid] = make_float4((float)e.dux, (float)e.duy, (float)(ut - (double)s0), sc);
                         s0s[tid] = s0;
+                        if (is_half_multi(MODE)) {
+                            const float inv = 1.f / gscale;        // powers of two: exact
+                            hmul[tid * 4] = S.scales[e.row] * inv;
+#pragma unroll
+                            for (int q = 0; q + 1 < NS; q++) hmul[tid * 4 + q + 1] = S.scales_x[q][e.row] * inv;
+                        }
                     } else {
                         const double fc = e.fcos, fs = e.fsin;
                         const double tt = (double)X0 * fc + (double)Y0 * fs + (double)e.foffset;
@@ -213,7 +259,7 @@ __global__ void __launch_bounds__(256) bp_tiled_kernel(BpLaunch L)
                     const int a = idx / SEG, j = idx - a * SEG;
                     pairs[idx] = linear_pair(S.rows, S.scales, S, rws[a], s0s[a] + j, j);
                 }
-            } else if (NS > 1) {
+            } else if (MODE == BP_LINEAR2) {
                 float4* quads = (float4*)data;
                 for (int idx = tid; idx < chunk * SEG; idx += C::T) {
                     const int a = idx / SEG, j = idx - a * SEG;
@@ -243,6 +289,38 @@ __global__ void __launch_bounds__(256) bp_tiled_kernel(BpLaunch L)
                         }
                     }
                     pairs[idx] = __halves2half2(__float2half_rn(pv), __float2half_rn(dv));
+                }
+            } else if (is_half_multi(MODE)) {
+                // Every lane takes part in each round (lanes past the end stage
+                // nothing), so a sample's right neighbour comes from the next
+                // lane; the last sample of a segment and lane 63 load it.
+                const int total = chunk * SEG;
+                for (int b0 = 0; b0 < total; b0 += C::T) {
+                    const int idx = b0 + tid;
+                    const bool valid = idx < total;
+                    const int a = valid ? idx / SEG : 0, j = idx - a * SEG;
+                    const int row = valid ? rws[a] : -1;
+                    const int k = s0s[a] + j;
+                    const bool in = row >= 0 && k >= 0 && k < N;
+                    const bool own_next = (j == SEG - 1) || (lane == 63);
+                    uint32_t wv[NS];
+#pragma unroll
+                    for (int e = 0; e < NS; e++) {
+                        const void* rows = e == 0 ? S.rows : S.rows_x[e - 1];
+                        const float mul = hmul[a * 4 + e];
+                        const float pv = in ? __half2float(__float2half_rn(raw_row_val(rows, S, row, k) * mul)) : 0.f;
+                        float p1 = __shfl_down(pv, 1);
+                        if (in && k < N - 1 && own_next)
+                            p1 = __half2float(__float2half_rn(raw_row_val(rows, S, row, k + 1) * mul));
+                        const float dv = (in && k < N - 1) ? p1 - pv : 0.f;
+                        wv[e] = __builtin_bit_cast(uint32_t, __halves2half2(__float2half_rn(pv), __float2half_rn(dv)));
+                    }
+                    if (valid) {
+                        if (NS == 4)
+                            ((uint4*)data)[idx] = make_uint4(wv[0], wv[NS > 1 ? 1 : 0], wv[NS > 2 ? 2 : 0], wv[NS > 3 ? 3 : 0]);
+                        else
+                            ((uint2*)data)[idx] = make_uint2(wv[0], wv[NS > 1 ? 1 : 0]);
+                    }
                 }
             } else {
                 // LUT modes: per sample the pair (p0, p1p0) of the reference LUT
@@ -297,7 +375,7 @@ __global__ void __launch_bounds__(256) bp_tiled_kernel(BpLaunch L)
                         }
                     }
                 }
-            } else if (NS > 1) {
+            } else if (MODE == BP_LINEAR2) {
                 const float4* quads = (const float4*)data;
                 for (int a = 0; a < chunk; a++) {
                     const float4 tb = tab[a];
@@ -319,6 +397,38 @@ __global__ void __launch_bounds__(256) bp_tiled_kernel(BpLaunch L)
                             }
                         }
                     }
+                }
+            } else if (is_half_multi(MODE)) {
+                using QT = typename std::conditional<NS == 4, uint4, uint2>::type;
+                const QT* ents = (const QT*)data;
+                for (int a = 0; a < chunk; a++) {
+                    const float4 tb = tab[a];
+                    const float base = fmaf(dyl, tb.y, fmaf(dxl, tb.x, tb.z));
+                    // all entries of the angle in flight first, then the dot products
+                    QT q[PY][PX];
+                    half2v w[PY][PX];
+#pragma unroll
+                    for (int j = 0; j < PY; j++) {
+                        const float bj = (j == 0) ? base : fmaf((float)(8 * j), tb.y, base);
+#pragma unroll
+                        for (int i = 0; i < PX; i++) {
+                            const float u = (i == 0) ? bj : fmaf((float)(8 * i), tb.x, bj);
+                            q[j][i] = ents[a * SEG + (int)u];
+                            // weights (1, fract(u)) in FP16 (round toward zero)
+                            w[j][i] = __builtin_bit_cast(half2v, __builtin_amdgcn_cvt_pkrtz(1.f, __builtin_amdgcn_fractf(u)));
+                        }
+                    }
+#pragma unroll
+                    for (int j = 0; j < PY; j++)
+#pragma unroll
+                        for (int i = 0; i < PX; i++) {
+                            const uint32_t* qe = (const uint32_t*)&q[j][i];
+                            acc[j][i] = __builtin_amdgcn_fdot2(__builtin_bit_cast(half2v, qe[0]), w[j][i], acc[j][i], false);
+#pragma unroll
+                            for (int e = 0; e + 1 < NS; e++)
+                                accx[e][j][i] = __builtin_amdgcn_fdot2(__builtin_bit_cast(half2v, qe[e + 1]), w[j][i],
+                                                                       accx[e][j][i], false);
+                        }
                 }
             } else if (MODE == BP_LINEAR_H) {
                 const __half2* pairs = (const __half2*)data;
@@ -394,12 +504,12 @@ __global__ void __launch_bounds__(256) bp_tiled_kernel(BpLaunch L)
                     out[o] = S.accumulate ? out[o] + v : v;
                 } else {
                     float* out = (float*)S.out;
-                    const float v = (float)acc[j][i] * S.out_scale;
+                    const float v = (float)acc[j][i] * gscale * S.out_scale;
                     out[o] = S.accumulate ? out[o] + v : v;
 #pragma unroll
                     for (int e = 0; e + 1 < NS; e++) {
                         float* outx = (float*)S.out_x[e];
-                        const float vx = (float)accx[e][j][i] * S.out_scale;
+                        const float vx = (float)accx[e][j][i] * gscale * S.out_scale;
                         outx[o] = S.accumulate ? outx[o] + vx : vx;
                     }
                 }
@@ -445,6 +555,8 @@ int bp_default_chunk(int mode, int tile)
     if (mode == BP_LINEAR) return (tile == 2) ? 64 : 32;
     if (mode == BP_LINEAR2) return (tile == 2) ? 32 : 16;
     if (mode == BP_LINEAR_H) return (tile == 2) ? 64 : 32;
+    if (mode == BP_LINEAR_H2) return (tile == 2) ? 64 : 32;
+    if (mode == BP_LINEAR_H4) return (tile == 2) ? 32 : 16;
     return (tile == 2) ? 64 : 32;
 }
 
@@ -473,6 +585,8 @@ static const void* kernel_for(int mode, int tile)
     if (mode == BP_LINEAR) return kernel_ptr<BP_LINEAR>(tile);
     if (mode == BP_LINEAR2) return kernel_ptr<BP_LINEAR2>(tile);
     if (mode == BP_LINEAR_H) return kernel_ptr<BP_LINEAR_H>(tile);
+    if (mode == BP_LINEAR_H2) return kernel_ptr<BP_LINEAR_H2>(tile);
+    if (mode == BP_LINEAR_H4) return kernel_ptr<BP_LINEAR_H4>(tile);
     if (mode == BP_LUT_F32) return kernel_ptr<BP_LUT_F32>(tile);
     return kernel_ptr<BP_LUT_I32>(tile);
 }
@@ -519,6 +633,8 @@ void launch_bp(const BpLaunch& L, int grid, hipStream_t stream)
     if (L.mode == BP_LINEAR) launch_mode<BP_LINEAR>(L, grid, stream);
     else if (L.mode == BP_LINEAR2) launch_mode<BP_LINEAR2>(L, grid, stream);
     else if (L.mode == BP_LINEAR_H) launch_mode<BP_LINEAR_H>(L, grid, stream);
+    else if (L.mode == BP_LINEAR_H2) launch_mode<BP_LINEAR_H2>(L, grid, stream);
+    else if (L.mode == BP_LINEAR_H4) launch_mode<BP_LINEAR_H4>(L, grid, stream);
     else if (L.mode == BP_LUT_F32) launch_mode<BP_LUT_F32>(L, grid, stream);
     else launch_mode<BP_LUT_I32>(L, grid, stream);
 }

@@ -92,11 +92,15 @@ void launch_output_stage(const void* img, int is_float, int N, int irad2, uint16
                          hipStream_t s);
 
 // ------------------------------------------------------- back-projection
-// BP_LINEAR2: BP_LINEAR over two slices of identical geometry per work item
-// (the second in BpSlice::rows_x[0] / scales_x[0] / out_x[0]).
-enum BpMode { BP_LINEAR = 0, BP_LUT_F32 = 1, BP_LUT_I32 = 2, BP_LINEAR_H = 3, BP_LINEAR2 = 4 };
-constexpr int bp_slices_per_item(int mode) { return mode == 4 ? 2 : 1; }
-constexpr int BP_MAX_SLICES_PER_ITEM = 2;
+// BP_LINEAR2: BP_LINEAR over two slices of identical geometry per work item;
+// BP_LINEAR_H2 / H4: BP_LINEAR_H over two / four (the others in
+// BpSlice::rows_x / scales_x / out_x).
+enum BpMode {
+    BP_LINEAR = 0, BP_LUT_F32 = 1, BP_LUT_I32 = 2, BP_LINEAR_H = 3, BP_LINEAR2 = 4,
+    BP_LINEAR_H2 = 5, BP_LINEAR_H4 = 6
+};
+constexpr int bp_slices_per_item(int mode) { return (mode == 4 || mode == 5) ? 2 : mode == 6 ? 4 : 1; }
+constexpr int BP_MAX_SLICES_PER_ITEM = 4;
 
 // Per-angle table entry.  LINEAR: detector coordinate of pixel (ix, iy) of the
 // output grid is u = u0 + ix*dux + iy*duy (all double).  LUT16: the reference

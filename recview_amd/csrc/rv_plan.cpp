@@ -368,13 +368,18 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
         }
     }
 
-    // LINEAR with FP32 pairs in LDS: slices whose back-projection geometry is
-    // identical (angle table, sizes, tile) share work items two at a time
-    // (classes of equal geometry, in slice order).
+    // LINEAR modes: slices whose back-projection geometry is identical (angle
+    // table, sizes, tile) share work items -- two at a time, or up to four with
+    // FP16 pairs in LDS (classes of equal geometry, in slice order).
     std::vector<std::vector<int>> classes;
-    const int per_item = opt->slices_per_item == 0 ? 2 : opt->slices_per_item;
-    if (per_item != 1 && per_item != 2) return fail(p, "slices_per_item must be 0, 1 or 2");
-    if (p->mode == rv::BP_LINEAR && per_item > 1) {
+    if (opt->slices_per_item != 0 && opt->slices_per_item != 1 && opt->slices_per_item != 2 &&
+        opt->slices_per_item != 4)
+        return fail(p, "slices_per_item must be 0, 1, 2 or 4");
+    // default two per item in both LINEAR modes (four FP16 slices per item
+    // need twice the registers and were measured slower than two)
+    const int max_per_item = (p->mode == rv::BP_LINEAR_H) ? 4 : 2;
+    const int per_item = std::min(opt->slices_per_item == 0 ? 2 : opt->slices_per_item, max_per_item);
+    if ((p->mode == rv::BP_LINEAR || p->mode == rv::BP_LINEAR_H) && per_item > 1) {
         auto same = [&](int a, int b) {
             const SliceState& A = p->sl[a];
             const SliceState& B = p->sl[b];
@@ -417,8 +422,11 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
         size_t i = 0;
         while (i < cls.size()) {
             const size_t left = cls.size() - i;
-            const int k = (per_item >= 2 && left >= 2) ? 2 : 1;
-            const int mode = k == 2 ? (int)rv::BP_LINEAR2 : p->mode;
+            const int k = (per_item >= 4 && left >= 4) ? 4 : (per_item >= 2 && left >= 2) ? 2 : 1;
+            const bool half = (p->mode == rv::BP_LINEAR_H);
+            const int mode = k == 4 ? (int)rv::BP_LINEAR_H4
+                           : k == 2 ? (half ? (int)rv::BP_LINEAR_H2 : (int)rv::BP_LINEAR2)
+                                    : p->mode;
             const int s = cls[i];
             const auto gk = std::make_pair(p->sl[s].tile, mode);
             if (gidx.find(gk) == gidx.end()) {

@@ -81,9 +81,9 @@ class FbpPlan:
         """normalize: images in attenuation per detector sample (x 2 pi / projections)
         instead of raw back-projected sums.  exact_filter: LINEAR modes keep CFft's
         operation order (bit-identical filtered rows) instead of two rows per transform.
-        slices_per_item: LINEAR with FP32 LDS pairs back-projects slices of identical
-        geometry two per work item (0 or 2; 1 = one per item) -- same results,
-        fewer instructions per update."""
+        slices_per_item: LINEAR modes back-project slices of identical geometry
+        together (0: two per work item -- with FP32 LDS pairs the same results as
+        one per item; or 1, 2, 4, where 4 needs FP16 pairs, lds_half)."""
         self.slices = list(slices)
         L = _lib.lib()
         specs = (_lib.RvSliceSpec * len(self.slices))(*[s.to_c() for s in self.slices])

@@ -202,6 +202,37 @@ def test_plan_filter_batches_equal_single(store, dtype, exact):
         assert np.array_equal(batch[i], one), i
 
 
+@pytest.mark.parametrize("store,rms_tol", [("f16", 1e-4), ("bf16", 2e-3)])
+def test_plan_half_pairs_grouped(store, rms_tol):
+    """FP16 pairs in LDS (lds_half) with slices of one geometry back-projected two
+    (default) or four per work item (one item-wide power-of-two scale, FP16 dot
+    products):
+    every image matches the FP64 oracle to the store's tolerance, and the one
+    slice per item path to a fraction of it."""
+    print(f"case store={store}", flush=True)
+    W, P = 256, 180
+    specs, sinos, raw = [], [], []
+    for i in range(7):                               # a group of four, a pair, one left
+        s, fdeg, c = _sino(W, P, 1300 + i, photons=1e4)
+        specs.append(fbp.SliceSpec(W, fdeg, c))
+        sinos.append(torch.from_numpy(s))
+        raw.append((s, fdeg, c))
+    _, single = _run(specs, sinos, store=store, lds_half=True, slices_per_item=1)
+    m = _circle(W)
+    refs = []
+    for s, fdeg, c in raw:
+        g = O.geometry(W, 2, c)
+        refs.append(O.recon_of64(s, fdeg, g, O.set_filter(g.ndim, 0)))
+    for k in (0, 4):
+        _, grouped = _run(specs, sinos, store=store, lds_half=True, slices_per_item=k)
+        for i, ref in enumerate(refs):
+            rms, _ = _errors(grouped[i], ref, m)
+            rms1, _ = _errors(single[i], ref, m)
+            diff, _ = _errors(grouped[i], single[i].astype(np.float64), m)
+            print(f"k={k} slice {i}: grouped rms {rms:.2e}, single rms {rms1:.2e}, grouped - single {diff:.2e}")
+            assert rms <= rms_tol and diff <= 0.5 * rms_tol, (k, i, rms, diff)
+
+
 def test_plan_normalize_gives_attenuation():
     """normalize = 1 multiplies each image by 2 pi / P (per slice, also in a
     batch of different P), so the output is attenuation per detector sample."""

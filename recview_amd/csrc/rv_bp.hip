@@ -22,7 +22,9 @@
 //                segment-relative coordinate u: one FMA, no fract.  8 B/update.
 //  BP_LINEAR2  : BP_LINEAR for two slices of identical geometry at once: LDS
 //                holds (a_j, d_j, a'_j, d'_j), so one coordinate and one
-//                16-byte LDS read serve two updates.  On gfx950 ds_read_b128
+//                16-byte LDS read serve two updates.  Its rows come packed by
+//                pack_pairs_kernel as (p_k, d_k, p'_k, d'_k), so staging a
+//                sample is one 16-byte load and two FMAs.  On gfx950 ds_read_b128
 //                returns ~1.7x the bytes per clock of ds_read_b64 (measured
 //                ~0.2 vs ~0.23 instructions per clock per CU), and the shared
 //                coordinate cuts VALU work from ~5.3 to ~3.6 per update.
@@ -260,18 +262,16 @@ __global__ void __launch_bounds__(256) bp_tiled_kernel(BpLaunch L)
                     pairs[idx] = linear_pair(S.rows, S.scales, S, rws[a], s0s[a] + j, j);
                 }
             } else if (MODE == BP_LINEAR2) {
+                // a_j = p_k - j d_k as linear_pair forms it (d = 0 at the last sample)
                 float4* quads = (float4*)data;
+                const float4* packed = (const float4*)S.rows;
                 for (int idx = tid; idx < chunk * SEG; idx += C::T) {
                     const int a = idx / SEG, j = idx - a * SEG;
                     const int row = rws[a], k = s0s[a] + j;
-                    // plane q of angle a holds slices 2q, 2q+1 (16-byte stride per sample)
-#pragma unroll
-                    for (int q = 0; q < NQ; q++) {
-                        const float2 q0 = (q == 0) ? linear_pair(S.rows, S.scales, S, row, k, j)
-                                                   : linear_pair(S.rows_x[2 * q - 1], S.scales_x[2 * q - 1], S, row, k, j);
-                        const float2 q1 = linear_pair(S.rows_x[2 * q], S.scales_x[2 * q], S, row, k, j);
-                        quads[(a * NQ + q) * SEG + j] = make_float4(q0.x, q0.y, q1.x, q1.y);
-                    }
+                    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+                    if (row >= 0 && k >= 0 && k < N) v = packed[(long long)row * S.row_stride + k];
+                    const float fj = -(float)j;
+                    quads[idx] = make_float4(fmaf(fj, v.y, v.x), v.y, fmaf(fj, v.w, v.z), v.w);
                 }
             } else if (MODE == BP_LINEAR_H) {
                 __half2* pairs = (__half2*)data;
@@ -517,6 +517,44 @@ __global__ void __launch_bounds__(256) bp_tiled_kernel(BpLaunch L)
         }
         if (!L.counter) break;
     }
+}
+
+__global__ void __launch_bounds__(256) pack_pairs_kernel(const PackJob* jobs, long long max_elems)
+{
+    const PackJob J = jobs[blockIdx.y];
+    const long long n = (long long)J.nrows * J.N;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+        const int row = (int)(i / J.N), k = (int)(i - (long long)row * J.N);
+        float v[4];
+#pragma unroll
+        for (int e = 0; e < 2; e++) {
+            const void* rows = e ? J.rows1 : J.rows0;
+            const float* sc = e ? J.scales1 : J.scales0;
+            float p0, p1 = 0.f;
+            if (J.store == 0) {
+                p0 = ((const float*)rows)[i];
+                if (k < J.N - 1) p1 = ((const float*)rows)[i + 1];
+            } else if (J.store == 1) {
+                p0 = __half2float(((const __half*)rows)[i]) * sc[row];
+                if (k < J.N - 1) p1 = __half2float(((const __half*)rows)[i + 1]) * sc[row];
+            } else {
+                p0 = __bfloat162float(((const __hip_bfloat16*)rows)[i]) * sc[row];
+                if (k < J.N - 1) p1 = __bfloat162float(((const __hip_bfloat16*)rows)[i + 1]) * sc[row];
+            }
+            v[2 * e] = p0;
+            v[2 * e + 1] = (k < J.N - 1) ? p1 - p0 : 0.f;
+        }
+        J.out[i] = make_float4(v[0], v[1], v[2], v[3]);
+    }
+    (void)max_elems;
+}
+
+void launch_pack_pairs(const PackJob* d_jobs, int njobs, long long max_elems, hipStream_t stream)
+{
+    if (njobs <= 0 || max_elems <= 0) return;
+    const long long blocks = (max_elems + 255) / 256;
+    const dim3 g((unsigned)(blocks < 2048 ? blocks : 2048), (unsigned)njobs);
+    hipLaunchKernelGGL(pack_pairs_kernel, g, dim3(256), 0, stream, d_jobs, max_elems);
 }
 
 // Tile configurations: id -> (PX, PY).  0: 64x64, 1: 64x128 (W x H), 2: 32x32, 3: 128x64.

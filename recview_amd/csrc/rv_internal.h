@@ -145,6 +145,20 @@ struct BpLaunch {
 
 void launch_bp(const BpLaunch& L, int grid, hipStream_t stream);
 
+// BP_LINEAR2 input: the filtered rows of a slice pair packed as one float4 per
+// sample, (p_k, d_k, p'_k, d'_k) with d_k = p_{k+1} - p_k (0 at the last sample),
+// so the back-projection stages a sample with one 16-byte load.
+struct PackJob {
+    const void* rows0;      // filtered rows of the two slices (float / half / bf16)
+    const void* rows1;
+    const float* scales0;   // per-row scales (16-bit stores), else nullptr
+    const float* scales1;
+    int store;              // 0 f32, 1 f16, 2 bf16
+    int nrows, N;           // rows of N samples, row stride N
+    float4* out;            // [nrows][N]
+};
+void launch_pack_pairs(const PackJob* d_jobs, int njobs, long long max_elems, hipStream_t stream);
+
 // ------------------------------------------------------- offset-CT stitch
 // gazoDoc.cpp:3536-3665 for one multiplex row of every frame below 180 deg.
 struct StitchJob {

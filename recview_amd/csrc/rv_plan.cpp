@@ -73,6 +73,10 @@ struct rv_plan_opaque {
     unsigned int* d_counter = nullptr;   // one per group
     void* d_stitch = nullptr;            // stitched offset-CT rows
     int* d_pairs = nullptr;
+    float4* d_packed = nullptr;          // BP_LINEAR2 work items: packed slice-pair rows
+    rv::PackJob* d_pack_jobs = nullptr;
+    int n_pack = 0;
+    long long pack_max = 0;
     std::vector<void*> cur_out;
     hipStream_t cap_stream = nullptr;
     hipGraph_t graph = nullptr;
@@ -101,6 +105,8 @@ void plan_free(rv_plan_t p)
     (void)hipFree(p->d_counter);
     (void)hipFree(p->d_stitch);
     (void)hipFree(p->d_pairs);
+    (void)hipFree(p->d_packed);
+    (void)hipFree(p->d_pack_jobs);
     delete p;
 }
 
@@ -220,6 +226,7 @@ int launch_filters(rv_plan_t p, const void* const* d_sino, hipStream_t st)
 
 int launch_backprojection(rv_plan_t p, hipStream_t st)
 {
+    if (p->n_pack) rv::launch_pack_pairs(p->d_pack_jobs, p->n_pack, p->pack_max, st);
     for (size_t gi = 0; gi < p->groups.size(); gi++) {
         const Group& G = p->groups[gi];
         rv::BpLaunch L{};
@@ -469,6 +476,17 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
         !rv::hip_ok(hipMemcpy(p->d_ang, ha.data(), sizeof(rv::AngleEntry) * nang_total, hipMemcpyHostToDevice), "angles"))
         return fail(p, std::string("plan upload failed: ") + RecviewLastMessage());
 
+    // packed rows of the slice pairs (BP_LINEAR2): one float4 per sample
+    size_t packed_elems = 0;
+    for (const auto& G : p->groups)
+        if (G.mode == rv::BP_LINEAR2)
+            for (int s : G.slices) packed_elems += (size_t)p->sl[s].nang * p->sl[s].g.N;
+    std::vector<rv::PackJob> pack_jobs;
+    if (packed_elems &&
+        !rv::hip_ok(hipMalloc(&p->d_packed, sizeof(float4) * packed_elems), "packed slice-pair rows"))
+        return fail(p, std::string("plan allocation failed: ") + RecviewLastMessage());
+    size_t packed_off = 0;
+
     for (auto& G : p->groups) {
         // slice order inside the group: longest projection lists first
         std::vector<int> ord = G.slices;
@@ -512,6 +530,22 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
                 B.rows_x[e] = (const unsigned char*)p->d_rows + S2.rows_off * esz;
                 B.scales_x[e] = (p->store == 0) ? nullptr : p->d_scales + S2.ang_off;
             }
+            if (G.mode == rv::BP_LINEAR2) {
+                rv::PackJob J{};
+                J.rows0 = B.rows;
+                J.rows1 = B.rows_x[0];
+                J.scales0 = B.scales;
+                J.scales1 = B.scales_x[0];
+                J.store = p->store;
+                J.nrows = S.nang;
+                J.N = S.g.N;
+                J.out = p->d_packed + packed_off;
+                pack_jobs.push_back(J);
+                p->pack_max = std::max(p->pack_max, (long long)S.nang * S.g.N);
+                B.rows = J.out;                    // the kernel reads the packed rows
+                B.row_stride = S.g.N;
+                packed_off += (size_t)S.nang * S.g.N;
+            }
         }
         if (!rv::hip_ok(hipMalloc(&G.d_slices, sizeof(rv::BpSlice) * G.h.size()), "slice table") ||
             !rv::hip_ok(hipMalloc(&G.d_order, sizeof(int) * std::max<size_t>(order.size(), 1)), "work list") ||
@@ -528,6 +562,13 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
         }
     }
     p->cur_out.assign(nslices, nullptr);
+    if (!pack_jobs.empty()) {
+        p->n_pack = (int)pack_jobs.size();
+        if (!rv::hip_ok(hipMalloc(&p->d_pack_jobs, sizeof(rv::PackJob) * pack_jobs.size()), "pack jobs") ||
+            !rv::hip_ok(hipMemcpy(p->d_pack_jobs, pack_jobs.data(), sizeof(rv::PackJob) * pack_jobs.size(),
+                                  hipMemcpyHostToDevice), "pack jobs"))
+            return fail(p, std::string("plan upload failed: ") + RecviewLastMessage());
+    }
 
     // Everything that allocates or sets kernel attributes happens here, so that
     // execution (and hipGraph capture of it) only launches kernels.

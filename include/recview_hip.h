@@ -281,9 +281,9 @@ typedef struct {
                           * the symmetrised filter, the same operator at half the cost */
     int slices_per_item; /* LINEAR modes: slices of identical geometry back-projected together, sharing
                           * each pixel's coordinate and LDS reads: 0 (default) = two per work item
-                          * (FP32 pairs in LDS: results identical to one per item; FP16 pairs with
-                          * lds_half: one FP16 dot product per update and one power-of-two scale per
-                          * work item); or 1, 2, 4 (4 only with FP16 pairs) */
+                          * with FP32 pairs in LDS (results identical to one per item), four with
+                          * FP16 pairs (lds_half: one FP16 dot product per update and one
+                          * power-of-two scale per work item); or 1, 2, 4 (4 only with FP16 pairs) */
 } RvOptions;
 
 typedef struct rv_plan_opaque* rv_plan_t;

@@ -23,7 +23,7 @@
 //  BP_LINEAR2  : BP_LINEAR for two slices of identical geometry at once: LDS
 //                holds (a_j, d_j, a'_j, d'_j), so one coordinate and one
 //                16-byte LDS read serve two updates.  Its rows come packed by
-//                pack_pairs_kernel as (p_k, d_k, p'_k, d'_k), so staging a
+//                pack_kernel as (p_k, d_k, p'_k, d'_k), so staging a
 //                sample is one 16-byte load and two FMAs.  On gfx950 ds_read_b128
 //                returns ~1.7x the bytes per clock of ds_read_b64 (measured
 //                ~0.2 vs ~0.23 instructions per clock per CU), and the shared
@@ -37,6 +37,8 @@
 //                largest row scale of its slices), and each update is one
 //                v_dot2_f32_f16: sum += (p, d) . (1, fract(u)) in FP32.  One
 //                coordinate, one fract and one (1, f) pack serve 2 / 4 updates.
+//                pack_kernel forms the rescaled FP16 pairs once per execution,
+//                so staging a sample is one 8- / 16-byte load.
 //  BP_LUT_F32  : the reference's 16-sub-sample LUT (cudaReconst.cu:230-240)
 //                in float, float sums; index t = ix*fcos + (iy*fsin + foffset)
 //                in the reference's FP32 operation order.
@@ -144,9 +146,6 @@ __global__ void __launch_bounds__(256) bp_tiled_kernel(BpLaunch L)
     void* data = (void*)(lds4 + chunk + (2 * chunk + 3) / 4);
     __shared__ int item_sh;
     __shared__ float chunk_scale;
-    __shared__ float red_sh[4];
-    // FP16 multi-slice modes: per staged angle and slice, row scale / item scale
-    __shared__ float hmul[is_half_multi(MODE) ? 256 * 4 : 1];
 
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
@@ -192,22 +191,8 @@ __global__ void __launch_bounds__(256) bp_tiled_kernel(BpLaunch L)
                 for (int e = 0; e + 1 < NS; e++) accx[e][j][i] = (Acc)0;
             }
 
-        float gscale = 1.f;
-        if (is_half_multi(MODE)) {
-            // one power-of-two scale for the item: the largest row scale of its slices
-            float m = 0.f;
-            for (int a = tid; a < S.nang; a += C::T) {
-                const int row = S.ang[a].row;
-                m = fmaxf(m, S.scales[row]);
-#pragma unroll
-                for (int e = 0; e + 1 < NS; e++) m = fmaxf(m, S.scales_x[e][row]);
-            }
-            for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
-            if (lane == 0) red_sh[wave] = m;
-            __syncthreads();
-            m = fmaxf(fmaxf(red_sh[0], red_sh[1]), fmaxf(red_sh[2], red_sh[3]));
-            gscale = (m > 0.f) ? m : 1.f;
-        }
+        // FP16 multi-slice modes: the item scale the packed pairs were divided by
+        const float gscale = is_half_multi(MODE) ? *S.gscale : 1.f;
 
         for (int a0 = 0; a0 < S.nang; a0 += chunk) {
             // ---- per-angle tile geometry
@@ -222,12 +207,6 @@ __global__ void __launch_bounds__(256) bp_tiled_kernel(BpLaunch L)
                         const float sc = (MODE == BP_LINEAR_H) ? S.scales[e.row] : 0.f;
                         tab[tid] = make_float4((float)e.dux, (float)e.duy, (float)(ut - (double)s0), sc);
                         s0s[tid] = s0;
-                        if (is_half_multi(MODE)) {
-                            const float inv = 1.f / gscale;        // powers of two: exact
-                            hmul[tid * 4] = S.scales[e.row] * inv;
-#pragma unroll
-                            for (int q = 0; q + 1 < NS; q++) hmul[tid * 4 + q + 1] = S.scales_x[q][e.row] * inv;
-                        }
                     } else {
                         const double fc = e.fcos, fs = e.fsin;
                         const double tt = (double)X0 * fc + (double)Y0 * fs + (double)e.foffset;
@@ -291,35 +270,20 @@ __global__ void __launch_bounds__(256) bp_tiled_kernel(BpLaunch L)
                     pairs[idx] = __halves2half2(__float2half_rn(pv), __float2half_rn(dv));
                 }
             } else if (is_half_multi(MODE)) {
-                // Every lane takes part in each round (lanes past the end stage
-                // nothing), so a sample's right neighbour comes from the next
-                // lane; the last sample of a segment and lane 63 load it.
-                const int total = chunk * SEG;
-                for (int b0 = 0; b0 < total; b0 += C::T) {
-                    const int idx = b0 + tid;
-                    const bool valid = idx < total;
-                    const int a = valid ? idx / SEG : 0, j = idx - a * SEG;
-                    const int row = valid ? rws[a] : -1;
-                    const int k = s0s[a] + j;
+                // packed FP16 pairs of the item's slices: one 8- / 16-byte load per sample
+                for (int idx = tid; idx < chunk * SEG; idx += C::T) {
+                    const int a = idx / SEG, j = idx - a * SEG;
+                    const int row = rws[a], k = s0s[a] + j;
                     const bool in = row >= 0 && k >= 0 && k < N;
-                    const bool own_next = (j == SEG - 1) || (lane == 63);
-                    uint32_t wv[NS];
-#pragma unroll
-                    for (int e = 0; e < NS; e++) {
-                        const void* rows = e == 0 ? S.rows : S.rows_x[e - 1];
-                        const float mul = hmul[a * 4 + e];
-                        const float pv = in ? __half2float(__float2half_rn(raw_row_val(rows, S, row, k) * mul)) : 0.f;
-                        float p1 = __shfl_down(pv, 1);
-                        if (in && k < N - 1 && own_next)
-                            p1 = __half2float(__float2half_rn(raw_row_val(rows, S, row, k + 1) * mul));
-                        const float dv = (in && k < N - 1) ? p1 - pv : 0.f;
-                        wv[e] = __builtin_bit_cast(uint32_t, __halves2half2(__float2half_rn(pv), __float2half_rn(dv)));
-                    }
-                    if (valid) {
-                        if (NS == 4)
-                            ((uint4*)data)[idx] = make_uint4(wv[0], wv[NS > 1 ? 1 : 0], wv[NS > 2 ? 2 : 0], wv[NS > 3 ? 3 : 0]);
-                        else
-                            ((uint2*)data)[idx] = make_uint2(wv[0], wv[NS > 1 ? 1 : 0]);
+                    const long long off = (long long)row * S.row_stride + k;
+                    if (NS == 4) {
+                        uint4 v = make_uint4(0u, 0u, 0u, 0u);
+                        if (in) v = ((const uint4*)S.rows)[off];
+                        ((uint4*)data)[idx] = v;
+                    } else {
+                        uint2 v = make_uint2(0u, 0u);
+                        if (in) v = ((const uint2*)S.rows)[off];
+                        ((uint2*)data)[idx] = v;
                     }
                 }
             } else {
@@ -519,42 +483,82 @@ __global__ void __launch_bounds__(256) bp_tiled_kernel(BpLaunch L)
     }
 }
 
-__global__ void __launch_bounds__(256) pack_pairs_kernel(const PackJob* jobs, long long max_elems)
+__device__ __forceinline__ float stored_val(const void* rows, int store, long long i)
 {
-    const PackJob J = jobs[blockIdx.y];
-    const long long n = (long long)J.nrows * J.N;
-    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
-        const int row = (int)(i / J.N), k = (int)(i - (long long)row * J.N);
-        float v[4];
-#pragma unroll
-        for (int e = 0; e < 2; e++) {
-            const void* rows = e ? J.rows1 : J.rows0;
-            const float* sc = e ? J.scales1 : J.scales0;
-            float p0, p1 = 0.f;
-            if (J.store == 0) {
-                p0 = ((const float*)rows)[i];
-                if (k < J.N - 1) p1 = ((const float*)rows)[i + 1];
-            } else if (J.store == 1) {
-                p0 = __half2float(((const __half*)rows)[i]) * sc[row];
-                if (k < J.N - 1) p1 = __half2float(((const __half*)rows)[i + 1]) * sc[row];
-            } else {
-                p0 = __bfloat162float(((const __hip_bfloat16*)rows)[i]) * sc[row];
-                if (k < J.N - 1) p1 = __bfloat162float(((const __hip_bfloat16*)rows)[i + 1]) * sc[row];
-            }
-            v[2 * e] = p0;
-            v[2 * e + 1] = (k < J.N - 1) ? p1 - p0 : 0.f;
-        }
-        J.out[i] = make_float4(v[0], v[1], v[2], v[3]);
-    }
-    (void)max_elems;
+    if (store == 0) return ((const float*)rows)[i];
+    if (store == 1) return __half2float(((const __half*)rows)[i]);
+    return __bfloat162float(((const __hip_bfloat16*)rows)[i]);
 }
 
-void launch_pack_pairs(const PackJob* d_jobs, int njobs, long long max_elems, hipStream_t stream)
+// Item scale of the FP16-pair jobs: the largest row scale of the job's slices
+// over its angle list (row scales are powers of two, so 1 / scale is exact).
+__global__ void __launch_bounds__(256) pack_scale_kernel(const PackJob* jobs)
+{
+    __shared__ float red[4];
+    const PackJob& J = jobs[blockIdx.x];
+    if (!J.half) return;
+    float m = 0.f;
+    for (int a = threadIdx.x; a < J.nang; a += 256) {
+        const int row = J.ang[a].row;
+        for (int e = 0; e < J.ns; e++) m = fmaxf(m, J.scales[e][row]);
+    }
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        m = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+        *J.gscale = (m > 0.f) ? m : 1.f;
+    }
+}
+
+__global__ void __launch_bounds__(256) pack_kernel(const PackJob* jobs)
+{
+    const PackJob& J = jobs[blockIdx.y];
+    const long long n = (long long)J.nrows * J.N;
+    const float inv = J.half ? 1.f / *J.gscale : 1.f;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+        const int row = (int)(i / J.N), k = (int)(i - (long long)row * J.N);
+        const bool last = k == J.N - 1;
+        if (!J.half) {
+            // the FP32 values and difference of the single-slice staging
+            float v[4];
+#pragma unroll
+            for (int e = 0; e < 2; e++) {
+                const float sc = J.store ? J.scales[e][row] : 1.f;
+                float p0 = stored_val(J.rows[e], J.store, i), p1 = 0.f;
+                if (J.store) p0 = p0 * sc;
+                if (!last) {
+                    p1 = stored_val(J.rows[e], J.store, i + 1);
+                    if (J.store) p1 = p1 * sc;
+                }
+                v[2 * e] = p0;
+                v[2 * e + 1] = last ? 0.f : p1 - p0;
+            }
+            ((float4*)J.out)[i] = make_float4(v[0], v[1], v[2], v[3]);
+        } else {
+            uint32_t w[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                if (e >= J.ns) break;
+                const float mul = J.scales[e][row] * inv;
+                const float pv = __half2float(__float2half_rn(stored_val(J.rows[e], J.store, i) * mul));
+                float dv = 0.f;
+                if (!last) dv = __half2float(__float2half_rn(stored_val(J.rows[e], J.store, i + 1) * mul)) - pv;
+                w[e] = __builtin_bit_cast(uint32_t, __halves2half2(__float2half_rn(pv), __float2half_rn(dv)));
+            }
+            if (J.ns == 4) ((uint4*)J.out)[i] = make_uint4(w[0], w[1], w[2], w[3]);
+            else ((uint2*)J.out)[i] = make_uint2(w[0], w[1]);
+        }
+    }
+}
+
+void launch_pack(const PackJob* d_jobs, int njobs, bool any_half, long long max_elems, hipStream_t stream)
 {
     if (njobs <= 0 || max_elems <= 0) return;
+    if (any_half) hipLaunchKernelGGL(pack_scale_kernel, dim3((unsigned)njobs), dim3(256), 0, stream, d_jobs);
     const long long blocks = (max_elems + 255) / 256;
     const dim3 g((unsigned)(blocks < 2048 ? blocks : 2048), (unsigned)njobs);
-    hipLaunchKernelGGL(pack_pairs_kernel, g, dim3(256), 0, stream, d_jobs, max_elems);
+    hipLaunchKernelGGL(pack_kernel, g, dim3(256), 0, stream, d_jobs);
 }
 
 // Tile configurations: id -> (PX, PY).  0: 64x64, 1: 64x128 (W x H), 2: 32x32, 3: 128x64.

@@ -129,6 +129,7 @@ struct BpSlice {
     const void* rows_x[BP_MAX_SLICES_PER_ITEM - 1];   // BP_LINEAR2: rows, scales and image
     const float* scales_x[BP_MAX_SLICES_PER_ITEM - 1]; // of the item's other slices
     void* out_x[BP_MAX_SLICES_PER_ITEM - 1];
+    const float* gscale;    // FP16 multi-slice modes: the item scale (device, set by the pack pass)
 };
 
 struct BpLaunch {
@@ -145,19 +146,28 @@ struct BpLaunch {
 
 void launch_bp(const BpLaunch& L, int grid, hipStream_t stream);
 
-// BP_LINEAR2 input: the filtered rows of a slice pair packed as one float4 per
-// sample, (p_k, d_k, p'_k, d'_k) with d_k = p_{k+1} - p_k (0 at the last sample),
-// so the back-projection stages a sample with one 16-byte load.
+// Packed input of the multi-slice modes, built once per execution from the
+// filtered rows so that the back-projection stages a sample with one 8- or
+// 16-byte load:
+//   BP_LINEAR2      float4 (p_k, d_k, p'_k, d'_k), d_k = p_{k+1} - p_k (0 at the
+//                   last sample);
+//   BP_LINEAR_H2/H4 two / four FP16 pairs (p_k, d_k), values multiplied by
+//                   row scale / item scale, where the item scale is the largest
+//                   row scale of the slices over their angle list (a power of two).
 struct PackJob {
-    const void* rows0;      // filtered rows of the two slices (float / half / bf16)
-    const void* rows1;
-    const float* scales0;   // per-row scales (16-bit stores), else nullptr
-    const float* scales1;
+    const void* rows[BP_MAX_SLICES_PER_ITEM];      // filtered rows (float / half / bf16)
+    const float* scales[BP_MAX_SLICES_PER_ITEM];   // per-row scales (16-bit stores), else nullptr
+    const AngleEntry* ang;  // FP16 pairs: the angle list whose rows set the item scale
+    int nang;
+    int ns;                 // slices: 2 or 4 (FP16 pairs only)
+    int half;               // 1: FP16 pairs
     int store;              // 0 f32, 1 f16, 2 bf16
     int nrows, N;           // rows of N samples, row stride N
-    float4* out;            // [nrows][N]
+    void* out;              // [nrows][N] packed samples
+    float* gscale;          // FP16 pairs: the item scale, written by the scale pass
 };
-void launch_pack_pairs(const PackJob* d_jobs, int njobs, long long max_elems, hipStream_t stream);
+// Scale pass (FP16 jobs) then packing, in stream order.
+void launch_pack(const PackJob* d_jobs, int njobs, bool any_half, long long max_elems, hipStream_t stream);
 
 // ------------------------------------------------------- offset-CT stitch
 // gazoDoc.cpp:3536-3665 for one multiplex row of every frame below 180 deg.

@@ -73,9 +73,11 @@ struct rv_plan_opaque {
     unsigned int* d_counter = nullptr;   // one per group
     void* d_stitch = nullptr;            // stitched offset-CT rows
     int* d_pairs = nullptr;
-    float4* d_packed = nullptr;          // BP_LINEAR2 work items: packed slice-pair rows
+    unsigned char* d_packed = nullptr;   // multi-slice work items: packed rows
+    float* d_gscale = nullptr;           // FP16-pair items: item scales
     rv::PackJob* d_pack_jobs = nullptr;
     int n_pack = 0;
+    bool pack_half = false;
     long long pack_max = 0;
     std::vector<void*> cur_out;
     hipStream_t cap_stream = nullptr;
@@ -86,6 +88,8 @@ struct rv_plan_opaque {
 };
 
 namespace {
+
+size_t align_up(size_t b, size_t a) { return (b + a - 1) / a * a; }
 
 void plan_free(rv_plan_t p)
 {
@@ -106,6 +110,7 @@ void plan_free(rv_plan_t p)
     (void)hipFree(p->d_stitch);
     (void)hipFree(p->d_pairs);
     (void)hipFree(p->d_packed);
+    (void)hipFree(p->d_gscale);
     (void)hipFree(p->d_pack_jobs);
     delete p;
 }
@@ -226,7 +231,7 @@ int launch_filters(rv_plan_t p, const void* const* d_sino, hipStream_t st)
 
 int launch_backprojection(rv_plan_t p, hipStream_t st)
 {
-    if (p->n_pack) rv::launch_pack_pairs(p->d_pack_jobs, p->n_pack, p->pack_max, st);
+    if (p->n_pack) rv::launch_pack(p->d_pack_jobs, p->n_pack, p->pack_half, p->pack_max, st);
     for (size_t gi = 0; gi < p->groups.size(); gi++) {
         const Group& G = p->groups[gi];
         rv::BpLaunch L{};
@@ -382,10 +387,11 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
     if (opt->slices_per_item != 0 && opt->slices_per_item != 1 && opt->slices_per_item != 2 &&
         opt->slices_per_item != 4)
         return fail(p, "slices_per_item must be 0, 1, 2 or 4");
-    // default two per item in both LINEAR modes (four FP16 slices per item
-    // need twice the registers and were measured slower than two)
+    // default: two FP32 slices or four FP16-pair slices per item (packed rows;
+    // measured at 2048^2 x 1800: FP32 pairs 40.3 ms, FP16 pairs 46.9 ms with two
+    // and 32.7 ms with four slices per item, for 64 slices)
     const int max_per_item = (p->mode == rv::BP_LINEAR_H) ? 4 : 2;
-    const int per_item = std::min(opt->slices_per_item == 0 ? 2 : opt->slices_per_item, max_per_item);
+    const int per_item = std::min(opt->slices_per_item == 0 ? 4 : opt->slices_per_item, max_per_item);
     if ((p->mode == rv::BP_LINEAR || p->mode == rv::BP_LINEAR_H) && per_item > 1) {
         auto same = [&](int a, int b) {
             const SliceState& A = p->sl[a];
@@ -476,16 +482,25 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
         !rv::hip_ok(hipMemcpy(p->d_ang, ha.data(), sizeof(rv::AngleEntry) * nang_total, hipMemcpyHostToDevice), "angles"))
         return fail(p, std::string("plan upload failed: ") + RecviewLastMessage());
 
-    // packed rows of the slice pairs (BP_LINEAR2): one float4 per sample
-    size_t packed_elems = 0;
+    // packed rows of the multi-slice work items: FP32 pairs 16 B, FP16 pairs 8 B
+    // (two slices) or 16 B (four slices) per sample
+    auto packed_size = [](int mode) -> size_t {
+        return mode == rv::BP_LINEAR2 || mode == rv::BP_LINEAR_H4 ? 16 : mode == rv::BP_LINEAR_H2 ? 8 : 0;
+    };
+    size_t packed_bytes = 0, n_half = 0;
     for (const auto& G : p->groups)
-        if (G.mode == rv::BP_LINEAR2)
-            for (int s : G.slices) packed_elems += (size_t)p->sl[s].nang * p->sl[s].g.N;
+        if (packed_size(G.mode))
+            for (int s : G.slices) {
+                packed_bytes += align_up((size_t)p->sl[s].nang * p->sl[s].g.N * packed_size(G.mode), 256);
+                n_half += G.mode != rv::BP_LINEAR2;
+            }
     std::vector<rv::PackJob> pack_jobs;
-    if (packed_elems &&
-        !rv::hip_ok(hipMalloc(&p->d_packed, sizeof(float4) * packed_elems), "packed slice-pair rows"))
+    if (packed_bytes &&
+        !rv::hip_ok(hipMalloc(&p->d_packed, packed_bytes), "packed multi-slice rows"))
         return fail(p, std::string("plan allocation failed: ") + RecviewLastMessage());
-    size_t packed_off = 0;
+    if (n_half && !rv::hip_ok(hipMalloc(&p->d_gscale, sizeof(float) * n_half), "item scales"))
+        return fail(p, std::string("plan allocation failed: ") + RecviewLastMessage());
+    size_t packed_off = 0, half_idx = 0;
 
     for (auto& G : p->groups) {
         // slice order inside the group: longest projection lists first
@@ -530,21 +545,31 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
                 B.rows_x[e] = (const unsigned char*)p->d_rows + S2.rows_off * esz;
                 B.scales_x[e] = (p->store == 0) ? nullptr : p->d_scales + S2.ang_off;
             }
-            if (G.mode == rv::BP_LINEAR2) {
+            if (packed_size(G.mode)) {
+                const int ns = rv::bp_slices_per_item(G.mode);
                 rv::PackJob J{};
-                J.rows0 = B.rows;
-                J.rows1 = B.rows_x[0];
-                J.scales0 = B.scales;
-                J.scales1 = B.scales_x[0];
+                J.rows[0] = B.rows;
+                J.scales[0] = B.scales;
+                for (int e = 1; e < ns; e++) {
+                    J.rows[e] = B.rows_x[e - 1];
+                    J.scales[e] = B.scales_x[e - 1];
+                }
+                J.ang = B.ang;
+                J.nang = S.nang;
+                J.ns = ns;
+                J.half = G.mode != rv::BP_LINEAR2;
                 J.store = p->store;
                 J.nrows = S.nang;
                 J.N = S.g.N;
                 J.out = p->d_packed + packed_off;
+                J.gscale = J.half ? p->d_gscale + half_idx++ : nullptr;
                 pack_jobs.push_back(J);
                 p->pack_max = std::max(p->pack_max, (long long)S.nang * S.g.N);
+                p->pack_half = p->pack_half || J.half;
                 B.rows = J.out;                    // the kernel reads the packed rows
                 B.row_stride = S.g.N;
-                packed_off += (size_t)S.nang * S.g.N;
+                B.gscale = J.gscale;
+                packed_off += align_up((size_t)S.nang * S.g.N * packed_size(G.mode), 256);
             }
         }
         if (!rv::hip_ok(hipMalloc(&G.d_slices, sizeof(rv::BpSlice) * G.h.size()), "slice table") ||

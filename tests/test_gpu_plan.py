@@ -205,8 +205,8 @@ def test_plan_filter_batches_equal_single(store, dtype, exact):
 @pytest.mark.parametrize("store,rms_tol", [("f16", 1e-4), ("bf16", 2e-3)])
 def test_plan_half_pairs_grouped(store, rms_tol):
     """FP16 pairs in LDS (lds_half) with slices of one geometry back-projected two
-    (default) or four per work item (one item-wide power-of-two scale, FP16 dot
-    products):
+    or four (default) per work item (packed rows, one item-wide power-of-two
+    scale, FP16 dot products):
     every image matches the FP64 oracle to the store's tolerance, and the one
     slice per item path to a fraction of it."""
     print(f"case store={store}", flush=True)
@@ -223,7 +223,7 @@ def test_plan_half_pairs_grouped(store, rms_tol):
     for s, fdeg, c in raw:
         g = O.geometry(W, 2, c)
         refs.append(O.recon_of64(s, fdeg, g, O.set_filter(g.ndim, 0)))
-    for k in (0, 4):
+    for k in (0, 2, 4):
         _, grouped = _run(specs, sinos, store=store, lds_half=True, slices_per_item=k)
         for i, ref in enumerate(refs):
             rms, _ = _errors(grouped[i], ref, m)

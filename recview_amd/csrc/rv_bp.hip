@@ -365,34 +365,46 @@ __global__ void __launch_bounds__(256) bp_tiled_kernel(BpLaunch L)
             } else if (is_half_multi(MODE)) {
                 using QT = typename std::conditional<NS == 4, uint4, uint2>::type;
                 const QT* ents = (const QT*)data;
+                // Entries of GY pixel rows are put in flight, then their dot
+                // products issued.  Four slices: one row at a time (118 VGPRs,
+                // four waves per SIMD; the whole angle at once needs 151 and
+                // allows three, measured 6% slower); two slices: the whole angle.
+                constexpr int GY = (NS == 4) ? 1 : PY;
                 for (int a = 0; a < chunk; a++) {
                     const float4 tb = tab[a];
                     const float base = fmaf(dyl, tb.y, fmaf(dxl, tb.x, tb.z));
-                    // all entries of the angle in flight first, then the dot products
-                    QT q[PY][PX];
-                    half2v w[PY][PX];
 #pragma unroll
-                    for (int j = 0; j < PY; j++) {
-                        const float bj = (j == 0) ? base : fmaf((float)(8 * j), tb.y, base);
+                    for (int j0 = 0; j0 < PY; j0 += GY) {
+                        QT q[GY][PX];
+                        half2v w[GY][PX];
 #pragma unroll
-                        for (int i = 0; i < PX; i++) {
-                            const float u = (i == 0) ? bj : fmaf((float)(8 * i), tb.x, bj);
-                            q[j][i] = ents[a * SEG + (int)u];
-                            // weights (1, fract(u)) in FP16 (round toward zero)
-                            w[j][i] = __builtin_bit_cast(half2v, __builtin_amdgcn_cvt_pkrtz(1.f, __builtin_amdgcn_fractf(u)));
+                        for (int jj = 0; jj < GY; jj++) {
+                            const int j = j0 + jj;
+                            const float bj = (j == 0) ? base : fmaf((float)(8 * j), tb.y, base);
+#pragma unroll
+                            for (int i = 0; i < PX; i++) {
+                                const float u = (i == 0) ? bj : fmaf((float)(8 * i), tb.x, bj);
+                                q[jj][i] = ents[a * SEG + (int)u];
+                                // weights (1, fract(u)) in FP16 (round toward zero)
+                                w[jj][i] = __builtin_bit_cast(half2v,
+                                                              __builtin_amdgcn_cvt_pkrtz(1.f, __builtin_amdgcn_fractf(u)));
+                            }
                         }
+#pragma unroll
+                        for (int jj = 0; jj < GY; jj++)
+#pragma unroll
+                            for (int i = 0; i < PX; i++) {
+                                const int j = j0 + jj;
+                                const uint32_t* qe = (const uint32_t*)&q[jj][i];
+                                acc[j][i] = __builtin_amdgcn_fdot2(__builtin_bit_cast(half2v, qe[0]), w[jj][i], acc[j][i],
+                                                                   false);
+#pragma unroll
+                                for (int e = 0; e + 1 < NS; e++)
+                                    accx[e][j][i] = __builtin_amdgcn_fdot2(__builtin_bit_cast(half2v, qe[e + 1]), w[jj][i],
+                                                                           accx[e][j][i], false);
+                            }
+                        __builtin_amdgcn_sched_barrier(0);     // keep the row groups apart
                     }
-#pragma unroll
-                    for (int j = 0; j < PY; j++)
-#pragma unroll
-                        for (int i = 0; i < PX; i++) {
-                            const uint32_t* qe = (const uint32_t*)&q[j][i];
-                            acc[j][i] = __builtin_amdgcn_fdot2(__builtin_bit_cast(half2v, qe[0]), w[j][i], acc[j][i], false);
-#pragma unroll
-                            for (int e = 0; e + 1 < NS; e++)
-                                accx[e][j][i] = __builtin_amdgcn_fdot2(__builtin_bit_cast(half2v, qe[e + 1]), w[j][i],
-                                                                       accx[e][j][i], false);
-                        }
                 }
             } else if (MODE == BP_LINEAR_H) {
                 const __half2* pairs = (const __half2*)data;

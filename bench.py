@@ -329,7 +329,7 @@ def main():
     ap.add_argument("--persistent", type=int, default=1)
     ap.add_argument("--slices-per-item", type=int, default=0,
                     help="LINEAR: slices of identical geometry per work item (0 = two with FP32 "
-                         "pairs, four with FP16 pairs; or 1, 2, 4)")
+                         "pairs, eight with FP16 pairs; or 1, 2, 4, 8)")
     ap.add_argument("--graph", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)

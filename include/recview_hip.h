@@ -281,9 +281,10 @@ typedef struct {
                           * the symmetrised filter, the same operator at half the cost */
     int slices_per_item; /* LINEAR modes: slices of identical geometry back-projected together, sharing
                           * each pixel's coordinate and LDS reads: 0 (default) = two per work item
-                          * with FP32 pairs in LDS (results identical to one per item), four with
+                          * with FP32 pairs in LDS (results identical to one per item), eight with
                           * FP16 pairs (lds_half: one FP16 dot product per update and one
-                          * power-of-two scale per work item); or 1, 2, 4 (4 only with FP16 pairs) */
+                          * power-of-two scale per work item); or 1, 2, 4, 8 (4 and 8 only with FP16
+                          * pairs; 8 on the 64x64 and 32x32 tiles, else 4) */
 } RvOptions;
 
 typedef struct rv_plan_opaque* rv_plan_t;

@@ -71,7 +71,10 @@ struct AccT { using type = float; };
 template <>
 struct AccT<BP_LUT_I32> { using type = int; };
 
-constexpr bool is_half_multi(int mode) { return mode == BP_LINEAR_H2 || mode == BP_LINEAR_H4; }
+constexpr bool is_half_multi(int mode)
+{
+    return mode == BP_LINEAR_H2 || mode == BP_LINEAR_H4 || mode == BP_LINEAR_H8;
+}
 
 constexpr bool is_linear(int mode)
 {
@@ -276,7 +279,16 @@ __global__ void __launch_bounds__(256) bp_tiled_kernel(BpLaunch L)
                     const int row = rws[a], k = s0s[a] + j;
                     const bool in = row >= 0 && k >= 0 && k < N;
                     const long long off = (long long)row * S.row_stride + k;
-                    if (NS == 4) {
+                    if (NS == 8) {
+                        // two 16-byte planes per angle: slices 0-3, then 4-7
+                        uint4 v0 = make_uint4(0u, 0u, 0u, 0u), v1 = v0;
+                        if (in) {
+                            v0 = ((const uint4*)S.rows)[2 * off];
+                            v1 = ((const uint4*)S.rows)[2 * off + 1];
+                        }
+                        ((uint4*)data)[(2 * a) * SEG + j] = v0;
+                        ((uint4*)data)[(2 * a + 1) * SEG + j] = v1;
+                    } else if (NS == 4) {
                         uint4 v = make_uint4(0u, 0u, 0u, 0u);
                         if (in) v = ((const uint4*)S.rows)[off];
                         ((uint4*)data)[idx] = v;
@@ -363,19 +375,22 @@ __global__ void __launch_bounds__(256) bp_tiled_kernel(BpLaunch L)
                     }
                 }
             } else if (is_half_multi(MODE)) {
-                using QT = typename std::conditional<NS == 4, uint4, uint2>::type;
-                const QT* ents = (const QT*)data;
                 // Entries of GY pixel rows are put in flight, then their dot
-                // products issued.  Four slices: one row at a time (118 VGPRs,
-                // four waves per SIMD; the whole angle at once needs 151 and
-                // allows three, measured 6% slower); two slices: the whole angle.
-                constexpr int GY = (NS == 4) ? 1 : PY;
+                // products issued.  Four or eight slices: one row at a time (four
+                // slices: 118 VGPRs, four waves per SIMD; the whole angle at once
+                // needs 151 and allows three, measured 6% slower); two slices:
+                // the whole angle.  Eight slices read two 16-byte planes.
+                using QT = typename std::conditional<NS >= 4, uint4, uint2>::type;
+                constexpr int NPL = NS >= 8 ? 2 : 1;
+                constexpr int GY = (NS >= 4) ? 1 : PY;
+                const QT* ents = (const QT*)data;
                 for (int a = 0; a < chunk; a++) {
                     const float4 tb = tab[a];
                     const float base = fmaf(dyl, tb.y, fmaf(dxl, tb.x, tb.z));
+                    const QT* seg = ents + a * NPL * SEG;
 #pragma unroll
                     for (int j0 = 0; j0 < PY; j0 += GY) {
-                        QT q[GY][PX];
+                        QT q[GY][PX][NPL];
                         half2v w[GY][PX];
 #pragma unroll
                         for (int jj = 0; jj < GY; jj++) {
@@ -384,7 +399,8 @@ __global__ void __launch_bounds__(256) bp_tiled_kernel(BpLaunch L)
 #pragma unroll
                             for (int i = 0; i < PX; i++) {
                                 const float u = (i == 0) ? bj : fmaf((float)(8 * i), tb.x, bj);
-                                q[jj][i] = ents[a * SEG + (int)u];
+#pragma unroll
+                                for (int pl = 0; pl < NPL; pl++) q[jj][i][pl] = seg[pl * SEG + (int)u];
                                 // weights (1, fract(u)) in FP16 (round toward zero)
                                 w[jj][i] = __builtin_bit_cast(half2v,
                                                               __builtin_amdgcn_cvt_pkrtz(1.f, __builtin_amdgcn_fractf(u)));
@@ -395,13 +411,18 @@ __global__ void __launch_bounds__(256) bp_tiled_kernel(BpLaunch L)
 #pragma unroll
                             for (int i = 0; i < PX; i++) {
                                 const int j = j0 + jj;
-                                const uint32_t* qe = (const uint32_t*)&q[jj][i];
-                                acc[j][i] = __builtin_amdgcn_fdot2(__builtin_bit_cast(half2v, qe[0]), w[jj][i], acc[j][i],
-                                                                   false);
 #pragma unroll
-                                for (int e = 0; e + 1 < NS; e++)
-                                    accx[e][j][i] = __builtin_amdgcn_fdot2(__builtin_bit_cast(half2v, qe[e + 1]), w[jj][i],
-                                                                           accx[e][j][i], false);
+                                for (int pl = 0; pl < NPL; pl++) {
+                                    const uint32_t* qe = (const uint32_t*)&q[jj][i][pl];
+                                    constexpr int PER = (NS >= 4) ? 4 : 2;
+#pragma unroll
+                                    for (int e4 = 0; e4 < PER; e4++) {
+                                        const int e = pl * 4 + e4;
+                                        const half2v h = __builtin_bit_cast(half2v, qe[e4]);
+                                        if (e == 0) acc[j][i] = __builtin_amdgcn_fdot2(h, w[jj][i], acc[j][i], false);
+                                        else accx[e - 1][j][i] = __builtin_amdgcn_fdot2(h, w[jj][i], accx[e - 1][j][i], false);
+                                    }
+                                }
                             }
                         __builtin_amdgcn_sched_barrier(0);     // keep the row groups apart
                     }
@@ -548,9 +569,9 @@ __global__ void __launch_bounds__(256) pack_kernel(const PackJob* jobs)
             }
             ((float4*)J.out)[i] = make_float4(v[0], v[1], v[2], v[3]);
         } else {
-            uint32_t w[4] = {0u, 0u, 0u, 0u};
+            uint32_t w[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
 #pragma unroll
-            for (int e = 0; e < 4; e++) {
+            for (int e = 0; e < 8; e++) {
                 if (e >= J.ns) break;
                 const float mul = J.scales[e][row] * inv;
                 const float pv = __half2float(__float2half_rn(stored_val(J.rows[e], J.store, i) * mul));
@@ -558,8 +579,14 @@ __global__ void __launch_bounds__(256) pack_kernel(const PackJob* jobs)
                 if (!last) dv = __half2float(__float2half_rn(stored_val(J.rows[e], J.store, i + 1) * mul)) - pv;
                 w[e] = __builtin_bit_cast(uint32_t, __halves2half2(__float2half_rn(pv), __float2half_rn(dv)));
             }
-            if (J.ns == 4) ((uint4*)J.out)[i] = make_uint4(w[0], w[1], w[2], w[3]);
-            else ((uint2*)J.out)[i] = make_uint2(w[0], w[1]);
+            if (J.ns == 8) {
+                ((uint4*)J.out)[2 * i] = make_uint4(w[0], w[1], w[2], w[3]);
+                ((uint4*)J.out)[2 * i + 1] = make_uint4(w[4], w[5], w[6], w[7]);
+            } else if (J.ns == 4) {
+                ((uint4*)J.out)[i] = make_uint4(w[0], w[1], w[2], w[3]);
+            } else {
+                ((uint2*)J.out)[i] = make_uint2(w[0], w[1]);
+            }
         }
     }
 }
@@ -611,6 +638,7 @@ int bp_default_chunk(int mode, int tile)
     if (mode == BP_LINEAR_H) return (tile == 2) ? 64 : 32;
     if (mode == BP_LINEAR_H2) return (tile == 2) ? 64 : 32;
     if (mode == BP_LINEAR_H4) return (tile == 2) ? 32 : 16;
+    if (mode == BP_LINEAR_H8) return (tile == 2) ? 32 : 24;
     return (tile == 2) ? 64 : 32;
 }
 
@@ -623,14 +651,21 @@ size_t bp_lds_bytes(int mode, int tile, int chunk)
     return head + (size_t)chunk * entry_bytes(mode, seg);
 }
 
+// Eight slices per item exist for tiles 0 and 2 only (the larger tiles would
+// need 256 accumulators per thread); the plan forms no such items otherwise.
 template <int MODE>
 static const void* kernel_ptr(int tile)
 {
-    switch (tile) {
-    case 1: return (const void*)&bp_tiled_kernel<MODE, 4, 8>;
-    case 2: return (const void*)&bp_tiled_kernel<MODE, 2, 2>;
-    case 3: return (const void*)&bp_tiled_kernel<MODE, 8, 4>;
-    default: return (const void*)&bp_tiled_kernel<MODE, 4, 4>;
+    if constexpr (MODE == BP_LINEAR_H8) {
+        if (tile == 2) return (const void*)&bp_tiled_kernel<MODE, 2, 2>;
+        return tile == 0 ? (const void*)&bp_tiled_kernel<MODE, 4, 4> : nullptr;
+    } else {
+        switch (tile) {
+        case 1: return (const void*)&bp_tiled_kernel<MODE, 4, 8>;
+        case 2: return (const void*)&bp_tiled_kernel<MODE, 2, 2>;
+        case 3: return (const void*)&bp_tiled_kernel<MODE, 8, 4>;
+        default: return (const void*)&bp_tiled_kernel<MODE, 4, 4>;
+        }
     }
 }
 
@@ -641,6 +676,7 @@ static const void* kernel_for(int mode, int tile)
     if (mode == BP_LINEAR_H) return kernel_ptr<BP_LINEAR_H>(tile);
     if (mode == BP_LINEAR_H2) return kernel_ptr<BP_LINEAR_H2>(tile);
     if (mode == BP_LINEAR_H4) return kernel_ptr<BP_LINEAR_H4>(tile);
+    if (mode == BP_LINEAR_H8) return kernel_ptr<BP_LINEAR_H8>(tile);
     if (mode == BP_LUT_F32) return kernel_ptr<BP_LUT_F32>(tile);
     return kernel_ptr<BP_LUT_I32>(tile);
 }
@@ -674,11 +710,16 @@ static void launch_mode(const BpLaunch& L, int grid, hipStream_t stream)
 {
     const size_t lds = bp_lds_bytes(MODE, L.tile, L.chunk);
     allow_lds(kernel_ptr<MODE>(L.tile), lds);
-    switch (L.tile) {
-    case 1: hipLaunchKernelGGL((bp_tiled_kernel<MODE, 4, 8>), dim3(grid), dim3(256), lds, stream, L); break;
-    case 2: hipLaunchKernelGGL((bp_tiled_kernel<MODE, 2, 2>), dim3(grid), dim3(256), lds, stream, L); break;
-    case 3: hipLaunchKernelGGL((bp_tiled_kernel<MODE, 8, 4>), dim3(grid), dim3(256), lds, stream, L); break;
-    default: hipLaunchKernelGGL((bp_tiled_kernel<MODE, 4, 4>), dim3(grid), dim3(256), lds, stream, L); break;
+    if constexpr (MODE == BP_LINEAR_H8) {
+        if (L.tile == 2) hipLaunchKernelGGL((bp_tiled_kernel<MODE, 2, 2>), dim3(grid), dim3(256), lds, stream, L);
+        else if (L.tile == 0) hipLaunchKernelGGL((bp_tiled_kernel<MODE, 4, 4>), dim3(grid), dim3(256), lds, stream, L);
+    } else {
+        switch (L.tile) {
+        case 1: hipLaunchKernelGGL((bp_tiled_kernel<MODE, 4, 8>), dim3(grid), dim3(256), lds, stream, L); break;
+        case 2: hipLaunchKernelGGL((bp_tiled_kernel<MODE, 2, 2>), dim3(grid), dim3(256), lds, stream, L); break;
+        case 3: hipLaunchKernelGGL((bp_tiled_kernel<MODE, 8, 4>), dim3(grid), dim3(256), lds, stream, L); break;
+        default: hipLaunchKernelGGL((bp_tiled_kernel<MODE, 4, 4>), dim3(grid), dim3(256), lds, stream, L); break;
+        }
     }
 }
 
@@ -689,6 +730,7 @@ void launch_bp(const BpLaunch& L, int grid, hipStream_t stream)
     else if (L.mode == BP_LINEAR_H) launch_mode<BP_LINEAR_H>(L, grid, stream);
     else if (L.mode == BP_LINEAR_H2) launch_mode<BP_LINEAR_H2>(L, grid, stream);
     else if (L.mode == BP_LINEAR_H4) launch_mode<BP_LINEAR_H4>(L, grid, stream);
+    else if (L.mode == BP_LINEAR_H8) launch_mode<BP_LINEAR_H8>(L, grid, stream);
     else if (L.mode == BP_LUT_F32) launch_mode<BP_LUT_F32>(L, grid, stream);
     else launch_mode<BP_LUT_I32>(L, grid, stream);
 }

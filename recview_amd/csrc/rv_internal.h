@@ -93,14 +93,17 @@ void launch_output_stage(const void* img, int is_float, int N, int irad2, uint16
 
 // ------------------------------------------------------- back-projection
 // BP_LINEAR2: BP_LINEAR over two slices of identical geometry per work item;
-// BP_LINEAR_H2 / H4: BP_LINEAR_H over two / four (the others in
+// BP_LINEAR_H2 / H4 / H8: BP_LINEAR_H over two / four / eight (the others in
 // BpSlice::rows_x / scales_x / out_x).
 enum BpMode {
     BP_LINEAR = 0, BP_LUT_F32 = 1, BP_LUT_I32 = 2, BP_LINEAR_H = 3, BP_LINEAR2 = 4,
-    BP_LINEAR_H2 = 5, BP_LINEAR_H4 = 6
+    BP_LINEAR_H2 = 5, BP_LINEAR_H4 = 6, BP_LINEAR_H8 = 7
 };
-constexpr int bp_slices_per_item(int mode) { return (mode == 4 || mode == 5) ? 2 : mode == 6 ? 4 : 1; }
-constexpr int BP_MAX_SLICES_PER_ITEM = 4;
+constexpr int bp_slices_per_item(int mode)
+{
+    return (mode == 4 || mode == 5) ? 2 : mode == 6 ? 4 : mode == 7 ? 8 : 1;
+}
+constexpr int BP_MAX_SLICES_PER_ITEM = 8;
 
 // Per-angle table entry.  LINEAR: detector coordinate of pixel (ix, iy) of the
 // output grid is u = u0 + ix*dux + iy*duy (all double).  LUT16: the reference
@@ -151,7 +154,7 @@ void launch_bp(const BpLaunch& L, int grid, hipStream_t stream);
 // 16-byte load:
 //   BP_LINEAR2      float4 (p_k, d_k, p'_k, d'_k), d_k = p_{k+1} - p_k (0 at the
 //                   last sample);
-//   BP_LINEAR_H2/H4 two / four FP16 pairs (p_k, d_k), values multiplied by
+//   BP_LINEAR_H2/H4/H8 two / four / eight FP16 pairs (p_k, d_k), values multiplied by
 //                   row scale / item scale, where the item scale is the largest
 //                   row scale of the slices over their angle list (a power of two).
 struct PackJob {
@@ -159,7 +162,7 @@ struct PackJob {
     const float* scales[BP_MAX_SLICES_PER_ITEM];   // per-row scales (16-bit stores), else nullptr
     const AngleEntry* ang;  // FP16 pairs: the angle list whose rows set the item scale
     int nang;
-    int ns;                 // slices: 2 or 4 (FP16 pairs only)
+    int ns;                 // slices: 2, or 4 / 8 (FP16 pairs only)
     int half;               // 1: FP16 pairs
     int store;              // 0 f32, 1 f16, 2 bf16
     int nrows, N;           // rows of N samples, row stride N

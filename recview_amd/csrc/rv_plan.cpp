@@ -385,13 +385,13 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
     // FP16 pairs in LDS (classes of equal geometry, in slice order).
     std::vector<std::vector<int>> classes;
     if (opt->slices_per_item != 0 && opt->slices_per_item != 1 && opt->slices_per_item != 2 &&
-        opt->slices_per_item != 4)
-        return fail(p, "slices_per_item must be 0, 1, 2 or 4");
-    // default: two FP32 slices or four FP16-pair slices per item (packed rows;
-    // measured at 2048^2 x 1800: FP32 pairs 40.3 ms, FP16 pairs 46.9 ms with two
-    // and 32.7 ms with four slices per item, for 64 slices)
-    const int max_per_item = (p->mode == rv::BP_LINEAR_H) ? 4 : 2;
-    const int per_item = std::min(opt->slices_per_item == 0 ? 4 : opt->slices_per_item, max_per_item);
+        opt->slices_per_item != 4 && opt->slices_per_item != 8)
+        return fail(p, "slices_per_item must be 0, 1, 2, 4 or 8");
+    // default: two FP32 slices or eight FP16-pair slices per item (packed rows;
+    // measured for 64 slices at 2048^2 x 1800: FP32 pairs 40.3 ms; FP16 pairs
+    // 46.0 / 30.9 / 28.3 ms with two / four / eight slices per item)
+    const int max_per_item = (p->mode == rv::BP_LINEAR_H) ? 8 : 2;
+    const int per_item = std::min(opt->slices_per_item == 0 ? 8 : opt->slices_per_item, max_per_item);
     if ((p->mode == rv::BP_LINEAR || p->mode == rv::BP_LINEAR_H) && per_item > 1) {
         auto same = [&](int a, int b) {
             const SliceState& A = p->sl[a];
@@ -435,9 +435,13 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
         size_t i = 0;
         while (i < cls.size()) {
             const size_t left = cls.size() - i;
-            const int k = (per_item >= 4 && left >= 4) ? 4 : (per_item >= 2 && left >= 2) ? 2 : 1;
+            const int tile = p->sl[cls[i]].tile;
+            const bool tile8 = tile == 0 || tile == 2;       // eight per item: 64^2 and 32^2 tiles
+            const int k = (per_item >= 8 && left >= 8 && tile8) ? 8
+                        : (per_item >= 4 && left >= 4) ? 4 : (per_item >= 2 && left >= 2) ? 2 : 1;
             const bool half = (p->mode == rv::BP_LINEAR_H);
-            const int mode = k == 4 ? (int)rv::BP_LINEAR_H4
+            const int mode = k == 8 ? (int)rv::BP_LINEAR_H8
+                           : k == 4 ? (int)rv::BP_LINEAR_H4
                            : k == 2 ? (half ? (int)rv::BP_LINEAR_H2 : (int)rv::BP_LINEAR2)
                                     : p->mode;
             const int s = cls[i];
@@ -485,7 +489,8 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
     // packed rows of the multi-slice work items: FP32 pairs 16 B, FP16 pairs 8 B
     // (two slices) or 16 B (four slices) per sample
     auto packed_size = [](int mode) -> size_t {
-        return mode == rv::BP_LINEAR2 || mode == rv::BP_LINEAR_H4 ? 16 : mode == rv::BP_LINEAR_H2 ? 8 : 0;
+        return mode == rv::BP_LINEAR_H8 ? 32
+             : mode == rv::BP_LINEAR2 || mode == rv::BP_LINEAR_H4 ? 16 : mode == rv::BP_LINEAR_H2 ? 8 : 0;
     };
     size_t packed_bytes = 0, n_half = 0;
     for (const auto& G : p->groups)

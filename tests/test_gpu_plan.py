@@ -204,15 +204,15 @@ def test_plan_filter_batches_equal_single(store, dtype, exact):
 
 @pytest.mark.parametrize("store,rms_tol", [("f16", 1e-4), ("bf16", 2e-3)])
 def test_plan_half_pairs_grouped(store, rms_tol):
-    """FP16 pairs in LDS (lds_half) with slices of one geometry back-projected two
-    or four (default) per work item (packed rows, one item-wide power-of-two
-    scale, FP16 dot products):
+    """FP16 pairs in LDS (lds_half) with slices of one geometry back-projected two,
+    four or eight (default) per work item (packed rows, one item-wide
+    power-of-two scale, FP16 dot products; eight on 64^2 and 32^2 tiles):
     every image matches the FP64 oracle to the store's tolerance, and the one
     slice per item path to a fraction of it."""
     print(f"case store={store}", flush=True)
     W, P = 256, 180
     specs, sinos, raw = [], [], []
-    for i in range(7):                               # a group of four, a pair, one left
+    for i in range(11):                              # groups of 8 / 4+4 / 2x5, a pair, one left
         s, fdeg, c = _sino(W, P, 1300 + i, photons=1e4)
         specs.append(fbp.SliceSpec(W, fdeg, c))
         sinos.append(torch.from_numpy(s))
@@ -223,14 +223,15 @@ def test_plan_half_pairs_grouped(store, rms_tol):
     for s, fdeg, c in raw:
         g = O.geometry(W, 2, c)
         refs.append(O.recon_of64(s, fdeg, g, O.set_filter(g.ndim, 0)))
-    for k in (0, 2, 4):
-        _, grouped = _run(specs, sinos, store=store, lds_half=True, slices_per_item=k)
+    for k, tile in ((0, 0), (2, 0), (4, 0), (8, 2)):
+        _, grouped = _run(specs, sinos, store=store, lds_half=True, slices_per_item=k, tile=tile)
         for i, ref in enumerate(refs):
             rms, _ = _errors(grouped[i], ref, m)
             rms1, _ = _errors(single[i], ref, m)
             diff, _ = _errors(grouped[i], single[i].astype(np.float64), m)
-            print(f"k={k} slice {i}: grouped rms {rms:.2e}, single rms {rms1:.2e}, grouped - single {diff:.2e}")
-            assert rms <= rms_tol and diff <= 0.5 * rms_tol, (k, i, rms, diff)
+            print(f"k={k} tile={tile} slice {i}: grouped rms {rms:.2e}, single rms {rms1:.2e}, "
+                  f"grouped - single {diff:.2e}")
+            assert rms <= rms_tol and diff <= 0.5 * rms_tol, (k, tile, i, rms, diff)
 
 
 def test_plan_normalize_gives_attenuation():

@@ -516,10 +516,11 @@ __global__ void __launch_bounds__(256) bp_tiled_kernel(BpLaunch L)
     }
 }
 
-__device__ __forceinline__ float stored_val(const void* rows, int store, long long i)
+template <int STORE>
+__device__ __forceinline__ float stored_val(const void* rows, long long i)
 {
-    if (store == 0) return ((const float*)rows)[i];
-    if (store == 1) return __half2float(((const __half*)rows)[i]);
+    if (STORE == 0) return ((const float*)rows)[i];
+    if (STORE == 1) return __half2float(((const __half*)rows)[i]);
     return __bfloat162float(((const __hip_bfloat16*)rows)[i]);
 }
 
@@ -544,60 +545,86 @@ __global__ void __launch_bounds__(256) pack_scale_kernel(const PackJob* jobs)
     }
 }
 
+// KIND 0: FP32 pairs of two slices; KIND 2 / 4 / 8: FP16 pairs of that many
+// slices.  Blocks stride over rows, threads over samples (no index division;
+// each row's scales read once); every slice's loads are issued together.
+template <int KIND, int STORE>
 __global__ void __launch_bounds__(256) pack_kernel(const PackJob* jobs)
 {
-    const PackJob& J = jobs[blockIdx.y];
-    const long long n = (long long)J.nrows * J.N;
-    const float inv = J.half ? 1.f / *J.gscale : 1.f;
-    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
-        const int row = (int)(i / J.N), k = (int)(i - (long long)row * J.N);
-        const bool last = k == J.N - 1;
-        if (!J.half) {
-            // the FP32 values and difference of the single-slice staging
-            float v[4];
+    constexpr int NS = KIND == 0 ? 2 : KIND;
+    const PackJob J = jobs[blockIdx.y];
+    const int N = J.N;
+    const float inv = KIND ? 1.f / *J.gscale : 1.f;
+    for (int row = blockIdx.x; row < J.nrows; row += gridDim.x) {
+        float mul[NS];
 #pragma unroll
-            for (int e = 0; e < 2; e++) {
-                const float sc = J.store ? J.scales[e][row] : 1.f;
-                float p0 = stored_val(J.rows[e], J.store, i), p1 = 0.f;
-                if (J.store) p0 = p0 * sc;
-                if (!last) {
-                    p1 = stored_val(J.rows[e], J.store, i + 1);
-                    if (J.store) p1 = p1 * sc;
+        for (int e = 0; e < NS; e++) mul[e] = STORE ? J.scales[e][row] * inv : 1.f;
+        const long long r0 = (long long)row * N;
+        for (int k = threadIdx.x; k < N; k += 256) {
+            const long long i = r0 + k;
+            const bool last = k == N - 1;
+            const long long i1 = last ? i : i + 1;      // in-row neighbour (unused at the last sample)
+            float p0[NS], p1[NS];
+#pragma unroll
+            for (int e = 0; e < NS; e++) {
+                p0[e] = stored_val<STORE>(J.rows[e], i);
+                p1[e] = stored_val<STORE>(J.rows[e], i1);
+            }
+            if (KIND == 0) {
+                // the FP32 values and difference of the single-slice staging
+                float v[4];
+#pragma unroll
+                for (int e = 0; e < 2; e++) {
+                    const float a0 = STORE ? p0[e] * mul[e] : p0[e];
+                    const float a1 = STORE ? p1[e] * mul[e] : p1[e];
+                    v[2 * e] = a0;
+                    v[2 * e + 1] = last ? 0.f : a1 - a0;
                 }
-                v[2 * e] = p0;
-                v[2 * e + 1] = last ? 0.f : p1 - p0;
-            }
-            ((float4*)J.out)[i] = make_float4(v[0], v[1], v[2], v[3]);
-        } else {
-            uint32_t w[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
-#pragma unroll
-            for (int e = 0; e < 8; e++) {
-                if (e >= J.ns) break;
-                const float mul = J.scales[e][row] * inv;
-                const float pv = __half2float(__float2half_rn(stored_val(J.rows[e], J.store, i) * mul));
-                float dv = 0.f;
-                if (!last) dv = __half2float(__float2half_rn(stored_val(J.rows[e], J.store, i + 1) * mul)) - pv;
-                w[e] = __builtin_bit_cast(uint32_t, __halves2half2(__float2half_rn(pv), __float2half_rn(dv)));
-            }
-            if (J.ns == 8) {
-                ((uint4*)J.out)[2 * i] = make_uint4(w[0], w[1], w[2], w[3]);
-                ((uint4*)J.out)[2 * i + 1] = make_uint4(w[4], w[5], w[6], w[7]);
-            } else if (J.ns == 4) {
-                ((uint4*)J.out)[i] = make_uint4(w[0], w[1], w[2], w[3]);
+                ((float4*)J.out)[i] = make_float4(v[0], v[1], v[2], v[3]);
             } else {
-                ((uint2*)J.out)[i] = make_uint2(w[0], w[1]);
+                uint32_t w[NS];
+#pragma unroll
+                for (int e = 0; e < NS; e++) {
+                    const float pv = __half2float(__float2half_rn(p0[e] * mul[e]));
+                    const float dv = last ? 0.f : __half2float(__float2half_rn(p1[e] * mul[e])) - pv;
+                    w[e] = __builtin_bit_cast(uint32_t, __halves2half2(__float2half_rn(pv), __float2half_rn(dv)));
+                }
+                if (NS == 8) {
+                    ((uint4*)J.out)[2 * i] = make_uint4(w[0], w[1], w[2], w[3]);
+                    ((uint4*)J.out)[2 * i + 1] = make_uint4(w[4 % NS], w[5 % NS], w[6 % NS], w[7 % NS]);
+                } else if (NS == 4) {
+                    ((uint4*)J.out)[i] = make_uint4(w[0], w[1], w[2 % NS], w[3 % NS]);
+                } else {
+                    ((uint2*)J.out)[i] = make_uint2(w[0], w[1]);
+                }
             }
         }
     }
 }
 
-void launch_pack(const PackJob* d_jobs, int njobs, bool any_half, long long max_elems, hipStream_t stream)
+template <int KIND>
+static void launch_pack_kind(const PackJob* d_jobs, int njobs, int store, int max_rows, hipStream_t stream)
 {
-    if (njobs <= 0 || max_elems <= 0) return;
-    if (any_half) hipLaunchKernelGGL(pack_scale_kernel, dim3((unsigned)njobs), dim3(256), 0, stream, d_jobs);
-    const long long blocks = (max_elems + 255) / 256;
-    const dim3 g((unsigned)(blocks < 2048 ? blocks : 2048), (unsigned)njobs);
-    hipLaunchKernelGGL(pack_kernel, g, dim3(256), 0, stream, d_jobs);
+    int bx = 16384 / njobs;
+    bx = bx < 1 ? 1 : bx > max_rows ? max_rows : bx;
+    const dim3 g((unsigned)bx, (unsigned)njobs);
+    if (store == 0) hipLaunchKernelGGL((pack_kernel<KIND, 0>), g, dim3(256), 0, stream, d_jobs);
+    else if (store == 1) hipLaunchKernelGGL((pack_kernel<KIND, 1>), g, dim3(256), 0, stream, d_jobs);
+    else hipLaunchKernelGGL((pack_kernel<KIND, 2>), g, dim3(256), 0, stream, d_jobs);
+}
+
+void launch_pack_scales(const PackJob* d_jobs, int njobs, hipStream_t stream)
+{
+    if (njobs > 0) hipLaunchKernelGGL(pack_scale_kernel, dim3((unsigned)njobs), dim3(256), 0, stream, d_jobs);
+}
+
+void launch_pack(const PackJob* d_jobs, int njobs, int kind, int store, int max_rows, hipStream_t stream)
+{
+    if (njobs <= 0 || max_rows <= 0) return;
+    if (kind == 0) launch_pack_kind<0>(d_jobs, njobs, store, max_rows, stream);
+    else if (kind == 2) launch_pack_kind<2>(d_jobs, njobs, store, max_rows, stream);
+    else if (kind == 4) launch_pack_kind<4>(d_jobs, njobs, store, max_rows, stream);
+    else launch_pack_kind<8>(d_jobs, njobs, store, max_rows, stream);
 }
 
 // Tile configurations: id -> (PX, PY).  0: 64x64, 1: 64x128 (W x H), 2: 32x32, 3: 128x64.

@@ -169,8 +169,11 @@ struct PackJob {
     void* out;              // [nrows][N] packed samples
     float* gscale;          // FP16 pairs: the item scale, written by the scale pass
 };
-// Scale pass (FP16 jobs) then packing, in stream order.
-void launch_pack(const PackJob* d_jobs, int njobs, bool any_half, long long max_elems, hipStream_t stream);
+// Item scales of FP16-pair jobs (jobs of other kinds are skipped), then the
+// packing of njobs jobs of one kind (0: FP32 pairs; 2 / 4 / 8: FP16 pairs of
+// that many slices) and one store, in stream order.
+void launch_pack_scales(const PackJob* d_jobs, int njobs, hipStream_t stream);
+void launch_pack(const PackJob* d_jobs, int njobs, int kind, int store, int max_rows, hipStream_t stream);
 
 // ------------------------------------------------------- offset-CT stitch
 // gazoDoc.cpp:3536-3665 for one multiplex row of every frame below 180 deg.

@@ -78,7 +78,8 @@ struct rv_plan_opaque {
     rv::PackJob* d_pack_jobs = nullptr;
     int n_pack = 0;
     bool pack_half = false;
-    long long pack_max = 0;
+    struct PackRange { int first, count, kind, rows; };
+    std::vector<PackRange> pack_ranges;  // one per group: jobs of one kind
     std::vector<void*> cur_out;
     hipStream_t cap_stream = nullptr;
     hipGraph_t graph = nullptr;
@@ -231,7 +232,9 @@ int launch_filters(rv_plan_t p, const void* const* d_sino, hipStream_t st)
 
 int launch_backprojection(rv_plan_t p, hipStream_t st)
 {
-    if (p->n_pack) rv::launch_pack(p->d_pack_jobs, p->n_pack, p->pack_half, p->pack_max, st);
+    if (p->pack_half) rv::launch_pack_scales(p->d_pack_jobs, p->n_pack, st);
+    for (const auto& R : p->pack_ranges)
+        rv::launch_pack(p->d_pack_jobs + R.first, R.count, R.kind, p->store, R.rows, st);
     for (size_t gi = 0; gi < p->groups.size(); gi++) {
         const Group& G = p->groups[gi];
         rv::BpLaunch L{};
@@ -568,8 +571,11 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
                 J.N = S.g.N;
                 J.out = p->d_packed + packed_off;
                 J.gscale = J.half ? p->d_gscale + half_idx++ : nullptr;
+                if (q == 0) p->pack_ranges.push_back({(int)pack_jobs.size(), 0, J.half ? ns : 0, 0});
+                auto& R = p->pack_ranges.back();
+                R.count++;
+                R.rows = std::max(R.rows, S.nang);
                 pack_jobs.push_back(J);
-                p->pack_max = std::max(p->pack_max, (long long)S.nang * S.g.N);
                 p->pack_half = p->pack_half || J.half;
                 B.rows = J.out;                    // the kernel reads the packed rows
                 B.row_stride = S.g.N;

@@ -375,56 +375,76 @@ __global__ void __launch_bounds__(256) bp_tiled_kernel(BpLaunch L)
                     }
                 }
             } else if (is_half_multi(MODE)) {
-                // Entries of GY pixel rows are put in flight, then their dot
-                // products issued.  Four or eight slices: one row at a time (four
-                // slices: 118 VGPRs, four waves per SIMD; the whole angle at once
-                // needs 151 and allows three, measured 6% slower); two slices:
-                // the whole angle.  Eight slices read two 16-byte planes.
+                // FP16 pairs of 2 / 4 / 8 slices per sample (8: two 16-byte planes).
                 using QT = typename std::conditional<NS >= 4, uint4, uint2>::type;
                 constexpr int NPL = NS >= 8 ? 2 : 1;
-                constexpr int GY = (NS >= 4) ? 1 : PY;
                 const QT* ents = (const QT*)data;
+                // entries of one pixel row and their FP16 weights (1, fract(u)),
+                // rounded toward zero
+                auto load_row = [&](const float4& tb, float base, const QT* seg, int j, QT (&q)[PX][NPL],
+                                    half2v (&w)[PX]) {
+                    const float bj = (j == 0) ? base : fmaf((float)(8 * j), tb.y, base);
+#pragma unroll
+                    for (int i = 0; i < PX; i++) {
+                        const float u = (i == 0) ? bj : fmaf((float)(8 * i), tb.x, bj);
+#pragma unroll
+                        for (int pl = 0; pl < NPL; pl++) q[i][pl] = seg[pl * SEG + (int)u];
+                        w[i] = __builtin_bit_cast(half2v, __builtin_amdgcn_cvt_pkrtz(1.f, __builtin_amdgcn_fractf(u)));
+                    }
+                };
+                auto dot_row = [&](int j, const QT (&q)[PX][NPL], const half2v (&w)[PX]) {
+                    constexpr int PER = (NS >= 4) ? 4 : 2;
+#pragma unroll
+                    for (int i = 0; i < PX; i++)
+#pragma unroll
+                        for (int pl = 0; pl < NPL; pl++) {
+                            const uint32_t* qe = (const uint32_t*)&q[i][pl];
+#pragma unroll
+                            for (int e4 = 0; e4 < PER; e4++) {
+                                const int e = pl * 4 + e4;
+                                const half2v h = __builtin_bit_cast(half2v, qe[e4]);
+                                if (e == 0) acc[j][i] = __builtin_amdgcn_fdot2(h, w[i], acc[j][i], false);
+                                else accx[e - 1][j][i] = __builtin_amdgcn_fdot2(h, w[i], accx[e - 1][j][i], false);
+                            }
+                        }
+                };
                 for (int a = 0; a < chunk; a++) {
                     const float4 tb = tab[a];
                     const float base = fmaf(dyl, tb.y, fmaf(dxl, tb.x, tb.z));
                     const QT* seg = ents + a * NPL * SEG;
+                    if constexpr (NS == 8) {
+                        // software pipeline over pixel rows: the next row's entries
+                        // are in flight while this row's dot products issue (242
+                        // VGPRs, two waves per SIMD; measured 3% faster than one
+                        // row at a time)
+                        QT q[2][PX][NPL];
+                        half2v w[2][PX];
+                        load_row(tb, base, seg, 0, q[0], w[0]);
 #pragma unroll
-                    for (int j0 = 0; j0 < PY; j0 += GY) {
-                        QT q[GY][PX][NPL];
-                        half2v w[GY][PX];
-#pragma unroll
-                        for (int jj = 0; jj < GY; jj++) {
-                            const int j = j0 + jj;
-                            const float bj = (j == 0) ? base : fmaf((float)(8 * j), tb.y, base);
-#pragma unroll
-                            for (int i = 0; i < PX; i++) {
-                                const float u = (i == 0) ? bj : fmaf((float)(8 * i), tb.x, bj);
-#pragma unroll
-                                for (int pl = 0; pl < NPL; pl++) q[jj][i][pl] = seg[pl * SEG + (int)u];
-                                // weights (1, fract(u)) in FP16 (round toward zero)
-                                w[jj][i] = __builtin_bit_cast(half2v,
-                                                              __builtin_amdgcn_cvt_pkrtz(1.f, __builtin_amdgcn_fractf(u)));
-                            }
+                        for (int j = 0; j < PY; j++) {
+                            if (j + 1 < PY) load_row(tb, base, seg, j + 1, q[(j + 1) & 1], w[(j + 1) & 1]);
+                            dot_row(j, q[j & 1], w[j & 1]);
                         }
+                    } else if constexpr (NS == 4) {
+                        // one row at a time (118 VGPRs, four waves per SIMD; the whole
+                        // angle at once needs 151 and allows three, measured 6% slower)
 #pragma unroll
-                        for (int jj = 0; jj < GY; jj++)
+                        for (int j = 0; j < PY; j++) {
+                            QT q[PX][NPL];
+                            half2v w[PX];
+                            load_row(tb, base, seg, j, q, w);
+                            dot_row(j, q, w);
+                            __builtin_amdgcn_sched_barrier(0);     // keep the rows apart
+                        }
+                    } else {
+                        // two slices: every entry of the angle in flight, then the dot products
+                        QT q[PY][PX][NPL];
+                        half2v w[PY][PX];
 #pragma unroll
-                            for (int i = 0; i < PX; i++) {
-                                const int j = j0 + jj;
+                        for (int j = 0; j < PY; j++) load_row(tb, base, seg, j, q[j], w[j]);
 #pragma unroll
-                                for (int pl = 0; pl < NPL; pl++) {
-                                    const uint32_t* qe = (const uint32_t*)&q[jj][i][pl];
-                                    constexpr int PER = (NS >= 4) ? 4 : 2;
-#pragma unroll
-                                    for (int e4 = 0; e4 < PER; e4++) {
-                                        const int e = pl * 4 + e4;
-                                        const half2v h = __builtin_bit_cast(half2v, qe[e4]);
-                                        if (e == 0) acc[j][i] = __builtin_amdgcn_fdot2(h, w[jj][i], acc[j][i], false);
-                                        else accx[e - 1][j][i] = __builtin_amdgcn_fdot2(h, w[jj][i], accx[e - 1][j][i], false);
-                                    }
-                                }
-                            }
-                        __builtin_amdgcn_sched_barrier(0);     // keep the row groups apart
+                        for (int j = 0; j < PY; j++) dot_row(j, q[j], w[j]);
+                        __builtin_amdgcn_sched_barrier(0);
                     }
                 }
             } else if (MODE == BP_LINEAR_H) {

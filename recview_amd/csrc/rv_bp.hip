@@ -57,11 +57,11 @@ namespace rv {
 
 typedef _Float16 half2v __attribute__((ext_vector_type(2)));
 
-template <int PX, int PY>
+template <int PX, int PY, int NW = 4>
 struct TileCfg {
     static constexpr int WX = 8 * PX, WY = 8 * PY;   // wave block
-    static constexpr int TW = 2 * WX, TH = 2 * WY;   // 2 x 2 waves
-    static constexpr int T = 256;
+    static constexpr int TW = 2 * WX, TH = (NW / 2) * WY;   // 2 x NW/2 waves
+    static constexpr int T = 64 * NW;
     // detector footprint of a tile: (TW-1)|cos| + (TH-1)|sin| + 3 margin samples
     static constexpr int SEG = ((int)(1.0 + 1.41421356 * (TW > TH ? TW : TH)) + 3 + 3) / 4 * 4;
 };
@@ -130,10 +130,10 @@ __device__ __forceinline__ float2 linear_pair(const void* rows, const float* sca
     return make_float2(av, dv);
 }
 
-template <int MODE, int PX, int PY>
-__global__ void __launch_bounds__(256) bp_tiled_kernel(BpLaunch L)
+template <int MODE, int PX, int PY, int NW = 4>
+__global__ void __launch_bounds__(64 * NW, NW == 8 ? 4 : 1) bp_tiled_kernel(BpLaunch L)
 {
-    using C = TileCfg<PX, PY>;
+    using C = TileCfg<PX, PY, NW>;
     using Acc = typename AccT<MODE>::type;
     constexpr int SEG = C::SEG;
     constexpr int NS = bp_slices_per_item(MODE);             // slices per work item
@@ -412,7 +412,7 @@ __global__ void __launch_bounds__(256) bp_tiled_kernel(BpLaunch L)
                     const float4 tb = tab[a];
                     const float base = fmaf(dyl, tb.y, fmaf(dxl, tb.x, tb.z));
                     const QT* seg = ents + a * NPL * SEG;
-                    if constexpr (NS == 8) {
+                    if constexpr (NS == 8 && NW == 4) {
                         // software pipeline over pixel rows: the next row's entries
                         // are in flight while this row's dot products issue (242
                         // VGPRs, two waves per SIMD; measured 3% faster than one
@@ -425,9 +425,10 @@ __global__ void __launch_bounds__(256) bp_tiled_kernel(BpLaunch L)
                             if (j + 1 < PY) load_row(tb, base, seg, j + 1, q[(j + 1) & 1], w[(j + 1) & 1]);
                             dot_row(j, q[j & 1], w[j & 1]);
                         }
-                    } else if constexpr (NS == 4) {
-                        // one row at a time (118 VGPRs, four waves per SIMD; the whole
-                        // angle at once needs 151 and allows three, measured 6% slower)
+                    } else if constexpr (NS >= 4) {
+                        // one row at a time (four slices: 118 VGPRs, four waves per SIMD;
+                        // the whole angle at once needs 151 and allows three, measured 6%
+                        // slower)
 #pragma unroll
                         for (int j = 0; j < PY; j++) {
                             QT q[PX][NPL];
@@ -700,12 +701,14 @@ size_t bp_lds_bytes(int mode, int tile, int chunk)
 
 // Eight slices per item exist for tiles 0 and 2 only (the larger tiles would
 // need 256 accumulators per thread); the plan forms no such items otherwise.
+// On the 64x64 tile they run eight waves per workgroup (8 pixels x 8 slices per
+// lane) so that four waves fit per SIMD.
 template <int MODE>
 static const void* kernel_ptr(int tile)
 {
     if constexpr (MODE == BP_LINEAR_H8) {
         if (tile == 2) return (const void*)&bp_tiled_kernel<MODE, 2, 2>;
-        return tile == 0 ? (const void*)&bp_tiled_kernel<MODE, 4, 4> : nullptr;
+        return tile == 0 ? (const void*)&bp_tiled_kernel<MODE, 4, 2, 8> : nullptr;
     } else {
         switch (tile) {
         case 1: return (const void*)&bp_tiled_kernel<MODE, 4, 8>;
@@ -748,7 +751,8 @@ int bp_blocks_per_cu(int mode, int tile, int chunk)
     const size_t lds = bp_lds_bytes(mode, tile, chunk);
     allow_lds(f, lds);
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, 256, lds) != hipSuccess || nb < 1) nb = 1;
+    const int threads = (mode == BP_LINEAR_H8 && tile == 0) ? 512 : 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, threads, lds) != hipSuccess || nb < 1) nb = 1;
     return nb;
 }
 
@@ -759,7 +763,8 @@ static void launch_mode(const BpLaunch& L, int grid, hipStream_t stream)
     allow_lds(kernel_ptr<MODE>(L.tile), lds);
     if constexpr (MODE == BP_LINEAR_H8) {
         if (L.tile == 2) hipLaunchKernelGGL((bp_tiled_kernel<MODE, 2, 2>), dim3(grid), dim3(256), lds, stream, L);
-        else if (L.tile == 0) hipLaunchKernelGGL((bp_tiled_kernel<MODE, 4, 4>), dim3(grid), dim3(256), lds, stream, L);
+        else if (L.tile == 0)
+            hipLaunchKernelGGL((bp_tiled_kernel<MODE, 4, 2, 8>), dim3(grid), dim3(512), lds, stream, L);
     } else {
         switch (L.tile) {
         case 1: hipLaunchKernelGGL((bp_tiled_kernel<MODE, 4, 8>), dim3(grid), dim3(256), lds, stream, L); break;

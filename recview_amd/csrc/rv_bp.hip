@@ -478,30 +478,37 @@ __global__ void __launch_bounds__(64 * NW, NW == 8 ? 4 : 1) bp_tiled_kernel(BpLa
                     for (int i = 0; i < PX; i++) acc[j][i] = fmaf(cacc[j][i], cs, acc[j][i]);
             } else {
                 const float2* lut = (const float2*)data;
-                for (int a = 0; a < chunk; a++) {
-                    const float4 tb = tab[a];
-                    const int base = 16 * s0s[a];
-                    const float2* la = lut + (size_t)a * SEG;
+                // the index rule is fixed per launch: one loop per rule, so an
+                // update converts once (no per-update select of floor and trunc)
+                auto lut_loop = [&](auto rule_tag) {
+                    constexpr int RULE = decltype(rule_tag)::value;
+                    for (int a = 0; a < chunk; a++) {
+                        const float4 tb = tab[a];
+                        const int base = 16 * s0s[a];
+                        const float2* la = lut + (size_t)a * SEG;
 #pragma unroll
-                    for (int j = 0; j < PY; j++) {
-                        const float fy = (float)(Y0 + (int)dyl + 8 * j);
-                        const float rt = fy * tb.y + tb.z;              // iy*fsin + foffset
+                        for (int j = 0; j < PY; j++) {
+                            const float fy = (float)(Y0 + (int)dyl + 8 * j);
+                            const float rt = fy * tb.y + tb.z;              // iy*fsin + foffset
 #pragma unroll
-                        for (int i = 0; i < PX; i++) {
-                            const float fx = (float)(X0 + (int)dxl + 8 * i);
-                            const float tt = fx * tb.x + rt;            // ix*fcos + (...)
-                            const int idx = (L.rule == 0) ? (int)floorf(tt) : (int)tt;
-                            const int rel = idx - base;              // >= 0 by the choice of s0
-                            const float2 e = la[rel >> 4];
-                            const float lv = e.x + e.y * (float)(rel & 15);
-                            const Acc v = (MODE == BP_LUT_I32) ? (Acc)(int)lv : (Acc)lv;
-                            if (MODE == BP_LUT_I32)
-                                acc[j][i] = (Acc)((unsigned int)acc[j][i] + (unsigned int)v);
-                            else
-                                acc[j][i] += v;
+                            for (int i = 0; i < PX; i++) {
+                                const float fx = (float)(X0 + (int)dxl + 8 * i);
+                                const float tt = fx * tb.x + rt;            // ix*fcos + (...)
+                                const int idx = (RULE == 0) ? (int)floorf(tt) : (int)tt;
+                                const int rel = idx - base;              // >= 0 by the choice of s0
+                                const float2 e = la[rel >> 4];
+                                const float lv = e.x + e.y * (float)(rel & 15);
+                                const Acc v = (MODE == BP_LUT_I32) ? (Acc)(int)lv : (Acc)lv;
+                                if (MODE == BP_LUT_I32)
+                                    acc[j][i] = (Acc)((unsigned int)acc[j][i] + (unsigned int)v);
+                                else
+                                    acc[j][i] += v;
+                            }
                         }
                     }
-                }
+                };
+                if (L.rule == 0) lut_loop(std::integral_constant<int, 0>{});
+                else lut_loop(std::integral_constant<int, 1>{});
             }
             __syncthreads();
         }

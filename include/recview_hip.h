@@ -290,7 +290,9 @@ typedef struct {
 typedef struct rv_plan_opaque* rv_plan_t;
 
 /* Builds a plan for nslices slices on device `device`: geometry, filter
- * tables, angle tables, filtered-sinogram workspace and the tile work list.
+ * tables, angle tables, filtered-sinogram workspace, the packed rows of
+ * multi-slice work items (FP32: 16 B per sample and slice pair; FP16 pairs:
+ * 4 B per sample and slice) and the tile work list.
  * Returns 0 or a negative error (message in RecviewLastMessage). */
 int rvPlanCreate(rv_plan_t* plan, int device, const RvSliceSpec* specs, int nslices,
                  const RvOptions* opt);
@@ -309,7 +311,7 @@ int rvPlanBackproject(rv_plan_t plan, void* const* d_out, rv_stream_t stream);
 int rvPlanFilteredRows(rv_plan_t plan, int s, void** rows, float** scales, int* row_len);
 /* Uploads filtered FP32 rows [P][row_len] for slice s (BP-only testing). */
 int rvPlanSetFilteredRows(rv_plan_t plan, int s, const float* d_rows, rv_stream_t stream);
-/* Plan facts: image side, number of tiles, tile shape, workspace bytes. */
+/* Plan facts: image side, transform length, number of tiles, tile shape. */
 int rvPlanInfo(rv_plan_t plan, int s, int* out_n, int* n_fft, int* tiles, int* tile_w, int* tile_h);
 /* Rows and width that slice s reconstructs (after offset-CT stitching). */
 int rvPlanRows(rv_plan_t plan, int s, int* nrows, int* width);

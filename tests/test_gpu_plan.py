@@ -234,6 +234,38 @@ def test_plan_half_pairs_grouped(store, rms_tol):
             assert rms <= rms_tol and diff <= 0.5 * rms_tol, (k, tile, i, rms, diff)
 
 
+@pytest.mark.parametrize("persistent,graph", [(True, False), (False, True)])
+def test_plan_half_groups_mixed_geometries_and_tiles(persistent, graph):
+    """One FP16-pair plan holding two geometries on different tiles: nine slices
+    on 64^2 tiles (an eight-slice item and a single) and five on 64x128 tiles
+    (eight per item is not built there: a four-slice item and a single), so one
+    execution packs rows of several kinds.  Every image equals the one slice per
+    item FP16 path to a fraction of the FP16 tolerance, through plain and graph
+    execution."""
+    print(f"case persistent={persistent} graph={graph}", flush=True)
+    specs, sinos, refs = [], [], []
+    for i in range(9):
+        s, fdeg, c = _sino(128, 90, 1500 + i, photons=1e4)
+        specs.append(fbp.SliceSpec(128, fdeg, c, tile=0))
+        sinos.append(torch.from_numpy(s))
+    for i in range(5):
+        s, fdeg, c = _sino(96, 60, 1600 + i, photons=1e4)
+        specs.append(fbp.SliceSpec(96, fdeg, c, tile=1))
+        sinos.append(torch.from_numpy(s))
+    _, single = _run(specs, sinos, store="f16", lds_half=True, slices_per_item=1, persistent=persistent)
+    plan = fbp.FbpPlan(specs, store="f16", lds_half=True, persistent=persistent)
+    outs = [torch.empty(plan.out_n(s), plan.out_n(s), device="cuda") for s in range(len(specs))]
+    for _ in range(2):                                   # a second execution repacks the rows
+        plan.execute([x.cuda() for x in sinos], outs, graph=graph)
+        torch.cuda.synchronize()
+        for i, o in enumerate(outs):
+            got = o.cpu().numpy()
+            m = _circle(got.shape[0])
+            diff, _ = _errors(got, single[i].astype(np.float64), m)
+            assert diff <= 5e-5, (i, diff)
+    plan.close()
+
+
 def test_plan_normalize_gives_attenuation():
     """normalize = 1 multiplies each image by 2 pi / P (per slice, also in a
     batch of different P), so the output is attenuation per detector sample."""

@@ -5,6 +5,9 @@ C1: one slice, 900 projections x 1024 detector samples (float32 analytic
     float against the reference CPU path R-f32 (projx64 index rule, CPU LUT
     formula): max |diff| <= 1e-5 of the dynamic range; LINEAR against the FP64
     oracle: RMS and max reported.
+C2: 16 slices of one geometry at 1800 x 2048 -> 2048^2, grouped as the plan
+    does (FP32 slice pairs, FP16 pairs eight per work item), against FP64
+    filtering + FP64 linear back-projection on sampled rows.
 C4: one 4096^2 slice from 3600 projections x 4096 samples with 1e5-photon
     Poisson noise, and a 4x zoomed 1024^2 off-centre sub-region; filtered
     sinograms stored as FP32, FP16 and BF16 (per-row scale, FP32 accumulation).
@@ -70,6 +73,43 @@ def _errors(got, ref):
     rng = ref.max() - ref.min()
     d = got.astype(np.float64) - ref
     return float(np.sqrt(np.mean(d * d)) / rng), float(np.abs(d).max() / rng)
+
+
+def test_c2_grouped_parity():
+    """C2 geometry (1800 projections x 2048 samples -> 2048^2) with 16 slices of
+    one geometry and different noisy phantoms, as the plan groups them: FP32
+    slice pairs (the default) and FP16 pairs eight per work item (lds_half).
+    Every slice against FP64 filtering + FP64 linear back-projection on six rows
+    across the reconstruction circle."""
+    W, P, S = 2048, 1800, 16
+    c_det = W / 2 + 0.37
+    fdeg = phantom.angles_deg(P)
+    g = O.geometry(W, 2, c_det)
+    filt = O.set_filter(g.ndim, 0)
+    N = g.N
+    rows = np.linspace(0.1 * N, 0.9 * N, 6).astype(int)
+    ys, xs = np.meshgrid(rows, np.arange(N), indexing="ij")
+    inside = ((xs - N / 2) ** 2 + (ys - N / 2) ** 2) <= (0.45 * N) ** 2
+    sinos, refs = [], []
+    for i in range(S):
+        ell = phantom.random_ellipses(1100 + i, count=10, mu_range=(0.0, 2.0 / W))
+        clean = phantom.sinogram(ell, W, fdeg, c_det, scale=W, device="cuda")
+        gen = torch.Generator(device="cuda").manual_seed(1100 + i)
+        sino = phantom.poisson(clean, 1e4, generator=gen).contiguous()
+        px64 = O.filter_rows_f64(sino.cpu().numpy(), g, filt)
+        refs.append(O.bp_points_linear_f64(px64, fdeg, g, xs[inside], ys[inside]))
+        sinos.append(sino)
+    specs = [fbp.SliceSpec(W, fdeg, c_det) for _ in range(S)]
+    for store, lds_half, tol in (("f32", False, 2e-6), ("f16", True, 1e-4)):
+        plan, outs = _run(specs, sinos, store=store, lds_half=lds_half, persistent=True)
+        worst = 0.0
+        for i in range(S):
+            rms, mx = _errors(outs[i].cpu().numpy()[ys[inside], xs[inside]], refs[i])
+            worst = max(worst, rms)
+            assert rms <= tol, (store, i, rms, mx)
+        print(f"C2 grouped {store}{'+lds_half' if lds_half else ''}: worst slice rms {worst:.2e} of range",
+              flush=True)
+        plan.close()
 
 
 def test_c4_precision_report():

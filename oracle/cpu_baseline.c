@@ -1,7 +1,7 @@
 /*
  * cpu_baseline.c -- the reference CPU reconstruction, restated for timing.
  *
- * TEST / BASELINE INFRASTRUCTURE ONLY (loaded by tests/ and by the cpu_baseline
+ * TEST AND CPU-BASELINE INFRASTRUCTURE ONLY (loaded by tests/ and by the cpu_baseline
  * leg of bench.py).  It is timed beside the GPU path as a baseline, never used
  * as a parity reference.
  *

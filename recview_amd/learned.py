@@ -7,7 +7,8 @@ row, P/8 angles) the input.  Both reconstructions run through one FbpPlan
 (LINEAR interpolation, FP32), so the network learns to remove exactly the
 streaks this engine's sparse-view FBP produces.  Both images are normalised
 to attenuation per detector sample (2 pi / projections used), the units of the
-phantom.  Full-angle FBP stays the fallback and the judge of the restored slice.
+phantom.  Full-angle FBP stays the fallback and the error reference for the
+restored slice.
 
   python -m recview_amd.learned train --out weights.npz   (GPU)
   python -m recview_amd.learned eval --weights weights.npz

@@ -266,6 +266,25 @@ def test_plan_half_groups_mixed_geometries_and_tiles(persistent, graph):
     plan.close()
 
 
+def test_plan_slices_per_item_options():
+    """slices_per_item: values other than 0, 1, 2, 4, 8 are rejected at plan
+    creation; FP32 plans clamp 4 and 8 to pairs (the FP32 pair kernel), so their
+    images equal the default's exactly."""
+    specs, sinos = [], []
+    for i in range(4):
+        s, fdeg, c = _sino(96, 60, 1700 + i)
+        specs.append(fbp.SliceSpec(96, fdeg, c))
+        sinos.append(torch.from_numpy(s))
+    for bad in (3, 16, -1):
+        with pytest.raises(RuntimeError):
+            fbp.FbpPlan(specs, slices_per_item=bad)
+    _, ref = _run(specs, sinos)
+    for k in (4, 8):
+        _, got = _run(specs, sinos, slices_per_item=k)
+        for i in range(len(specs)):
+            assert np.array_equal(got[i], ref[i]), (k, i)
+
+
 def test_plan_normalize_gives_attenuation():
     """normalize = 1 multiplies each image by 2 pi / P (per slice, also in a
     batch of different P), so the output is attenuation per detector sample."""

@@ -171,6 +171,12 @@ __global__ void __launch_bounds__(64 * NW, NW == 8 ? 4 : 1) bp_tiled_kernel(BpLa
         iter++;
         if (item >= L.total_items) break;
         if (L.item_order) item = L.item_order[item];
+        const int parts = (MODE == BP_LUT_I32 && L.angle_parts > 1) ? L.angle_parts : 1;
+        int part = 0;
+        if (parts > 1) {
+            part = item % parts;
+            item /= parts;
+        }
 
         // slice lookup: last slice with tile_base <= item
         int lo = 0, hi = L.nslices - 1;
@@ -197,11 +203,18 @@ __global__ void __launch_bounds__(64 * NW, NW == 8 ? 4 : 1) bp_tiled_kernel(BpLa
         // FP16 multi-slice modes: the item scale the packed pairs were divided by
         const float gscale = is_half_multi(MODE) ? *S.gscale : 1.f;
 
-        for (int a0 = 0; a0 < S.nang; a0 += chunk) {
+        // angles of this work item: all, or one chunk-aligned part of them
+        int a_begin = 0, a_end = S.nang;
+        if (parts > 1) {
+            const int per = ((S.nang + parts - 1) / parts + chunk - 1) / chunk * chunk;
+            a_begin = min(S.nang, part * per);
+            a_end = min(S.nang, a_begin + per);
+        }
+        for (int a0 = a_begin; a0 < a_end; a0 += chunk) {
             // ---- per-angle tile geometry
             if (tid < chunk) {
                 const int ga = a0 + tid;
-                if (ga < S.nang) {
+                if (ga < a_end) {
                     const AngleEntry e = S.ang[ga];
                     if (is_linear(MODE)) {
                         const double ut = e.u0 + (double)X0 * e.dux + (double)Y0 * e.duy;
@@ -526,7 +539,8 @@ __global__ void __launch_bounds__(64 * NW, NW == 8 ? 4 : 1) bp_tiled_kernel(BpLa
                 if (MODE == BP_LUT_I32) {
                     unsigned int* out = (unsigned int*)S.out;
                     const unsigned int v = (unsigned int)acc[j][i];
-                    out[o] = S.accumulate ? out[o] + v : v;
+                    if (parts > 1) atomicAdd(out + o, v);       // the parts of a tile meet here
+                    else out[o] = S.accumulate ? out[o] + v : v;
                 } else {
                     float* out = (float*)S.out;
                     const float v = (float)acc[j][i] * gscale * S.out_scale;

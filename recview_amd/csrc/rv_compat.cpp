@@ -273,6 +273,13 @@ void reconst_body(RECONST_INFO* ri, int idev, bool bReport, bool bEnStream, bool
             L.rule = cudafft ? 1 : g_index_rule;
             L.tile = tile;
             L.chunk = rv::bp_default_chunk(rv::BP_LUT_I32, tile);
+            // Each tile's projections are split over four work items whose integer
+            // sums meet in the zeroed image (one 2048^2 slice is 1024 tiles, one
+            // round of resident workgroups that finish unevenly; 4096 smaller
+            // items let the dispatcher balance them).
+            L.angle_parts = 4;
+            L.total_items *= L.angle_parts;
+            rv::launch_zero_u32((unsigned int*)ri->d_ifp, N * N, st);
             rv::launch_bp(L, L.total_items, st);
             ok = rv::hip_ok(hipGetLastError(), "back-projection launch");
         }

@@ -145,6 +145,9 @@ struct BpLaunch {
     int rule;               // LUT: 0 floor (float mask), 1 trunc
     int chunk;              // angles per LDS chunk
     int tile;               // tile config id
+    int angle_parts;        // LUT16 integer mode: a tile's angles split over this many work
+                            // items, their integer sums added atomically into a zeroed
+                            // image (order-free, so bit-exact); 0 or 1: one item per tile
 };
 
 void launch_bp(const BpLaunch& L, int grid, hipStream_t stream);

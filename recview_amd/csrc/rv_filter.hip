@@ -401,9 +401,12 @@ __global__ void __launch_bounds__(1024) filter_pair_kernel(FilterJob J, FilterBa
             if (hasb) ob[c] = -y.y * a;
         }
     } else {
-        float va[8], vb[8];
+        // The cropped width N is below n, so n / T values per thread cover
+        // every column (N > n/2 for any width that is not a power of two).
+        constexpr int NV = (n + T - 1) / T;
+        float va[NV], vb[NV];
 #pragma unroll
-        for (int i = 0; i < 8; i++) {
+        for (int i = 0; i < NV; i++) {
             const int c = tid + i * T;
             if (c < J.N) {
                 const float2 y = buf[c + J.crop];
@@ -415,7 +418,7 @@ __global__ void __launch_bounds__(1024) filter_pair_kernel(FilterJob J, FilterBa
         float* outf = (float*)buf;
         float* outf_b = outf + J.N;
 #pragma unroll
-        for (int i = 0; i < 8; i++) {
+        for (int i = 0; i < NV; i++) {
             const int c = tid + i * T;
             if (c < J.N) { outf[c] = va[i]; outf_b[c] = vb[i]; }
         }

@@ -385,6 +385,23 @@ def test_plan_graph_replay():
         for o, e in zip(outs, eager):
             assert torch.equal(o, e), f"replay {rep}"
         del junk
+    # eager executions into other images between replays, on another stream: each
+    # replay writes its own images again, and no eager result lands in them
+    side = torch.cuda.Stream()
+    others = [torch.empty_like(o) for o in outs]
+    for rep in range(3):
+        for o in outs + others:
+            o.fill_(float("nan"))
+        torch.cuda.synchronize()
+        with torch.cuda.stream(side):
+            plan.execute(sinos, others, stream=side)
+        plan.execute(sinos, outs, graph=True, stream=side)
+        with torch.cuda.stream(side):
+            plan.execute(sinos, others, stream=side)
+        torch.cuda.synchronize()
+        for o, x, e in zip(outs, others, eager):
+            assert torch.equal(o, e), f"replay {rep} after eager executions"
+            assert torch.equal(x, e), f"eager execution {rep} between replays"
     # a plan whose very first execution is a capture (nothing cached beforehand)
     spec = fbp.SliceSpec(96, phantom.angles_deg(30), 48.3)
     s, _, _ = _sino(96, 30, 61, c_off=0.3)
@@ -471,3 +488,55 @@ def test_plan_edge_cases():
     with pytest.raises(Exception):
         plan.execute([], [])
     plan.close()
+
+
+@pytest.mark.parametrize("W,interp", [(600, 2), (1000, 2), (2000, 2), (3000, 2), (600, 3), (1500, 3)])
+def test_plan_filtered_rows_non_pow2(W, interp):
+    """Widths that are not powers of two keep more than half of the transform
+    (n/2 < N < n): every cropped column of both rows of a pair, in FP32 and in
+    the 16-bit stores with their per-row scales, against the FP64 filter; an odd
+    row count leaves the last transform with one row."""
+    P = 13
+    s, fdeg, c = _sino(W, P, 1700 + W + interp)
+    g = O.geometry(W, interp, c)
+    assert g.ndim // 2 < g.N < g.ndim
+    ref64 = O.filter_rows_f64(s, g, O.set_filter(g.ndim, 0))
+    rmax = np.abs(ref64).max(axis=1, keepdims=True)
+    spec = fbp.SliceSpec(W, fdeg, c, iInterpolation=interp)
+    for exact in (False, True):
+        for store, tol in (("f32", 5e-6), ("f16", 1e-3), ("bf16", 6e-3)):
+            if exact and store == "f32":
+                continue
+            plan = fbp.FbpPlan([spec], store=store, exact_filter=exact)
+            plan.filter([torch.from_numpy(s).cuda()])
+            torch.cuda.synchronize()
+            got = plan.filtered_rows_tensor(0).cpu().numpy()
+            plan.close()
+            err = np.abs(got - ref64) / rmax
+            print(f"W={W} z={g.z} n={g.ndim} N={g.N} store={store} exact={exact}: "
+                  f"max {err.max():.2e} of the row maximum (column {int(err.max(axis=0).argmax())})")
+            assert err.max() <= tol, (W, store, exact)
+
+
+@pytest.mark.parametrize("W,nsl,store,lds_half,rms_tol", [
+    (1000, 8, "f16", True, 1e-4), (1000, 8, "bf16", True, 2e-3), (1000, 4, "f16", False, 1e-4),
+    (2000, 2, "f16", True, 1e-4), (2000, 2, "bf16", False, 2e-3), (2000, 2, "f32", False, 2e-6),
+    (3000, 2, "f16", True, 1e-4)])
+def test_plan_non_pow2_grouped_vs_fp64(W, nsl, store, lds_half, rms_tol):
+    """Whole pipeline at detector widths that are not powers of two, slices of one
+    geometry grouped per work item (eight / two FP16 pairs, FP32 pairs), against
+    the FP64 oracle inside the reconstruction circle."""
+    P = 96
+    specs, sinos = [], []
+    for i in range(nsl):
+        s, fdeg, c = _sino(W, P, 1800 + 31 * i + W, photons=1e4)
+        specs.append(fbp.SliceSpec(W, fdeg, c))
+        sinos.append(torch.from_numpy(s))
+    _, imgs = _run(specs, sinos, store=store, lds_half=lds_half)
+    g = O.geometry(W, 2, c)
+    mask = _circle(W)
+    for i in range(nsl):
+        ref = O.recon_of64(sinos[i].numpy(), fdeg, g, O.set_filter(g.ndim, 0))
+        rms, mx = _errors(imgs[i], ref, mask)
+        print(f"W={W} slice {i}/{nsl} store={store} lds_half={lds_half}: rms {rms:.2e} max {mx:.2e}")
+        assert rms <= rms_tol, (W, i, store)

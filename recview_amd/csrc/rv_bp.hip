@@ -796,6 +796,17 @@ static void launch_mode(const BpLaunch& L, int grid, hipStream_t stream)
     }
 }
 
+__global__ void __launch_bounds__(64) ptr_patch_kernel(PtrPatch P)
+{
+    const int i = threadIdx.x;
+    if (i < P.n) *P.dst[i] = P.val[i];
+}
+
+void launch_ptr_patch(const PtrPatch& P, hipStream_t stream)
+{
+    if (P.n > 0) hipLaunchKernelGGL(ptr_patch_kernel, dim3(1), dim3(64), 0, stream, P);
+}
+
 void launch_bp(const BpLaunch& L, int grid, hipStream_t stream)
 {
     if (L.mode == BP_LINEAR) launch_mode<BP_LINEAR>(L, grid, stream);

@@ -152,6 +152,18 @@ struct BpLaunch {
 
 void launch_bp(const BpLaunch& L, int grid, hipStream_t stream);
 
+// Pointer fields of device tables (the image pointers of BpSlice entries),
+// written by a kernel in stream order: an execution's patches are ordered with
+// the work queued before it on the caller's stream, and a captured graph
+// carries its own pointers, so every replay restores them.
+constexpr int kPtrPatch = 32;
+struct PtrPatch {
+    void** dst[kPtrPatch];
+    void* val[kPtrPatch];
+    int n;
+};
+void launch_ptr_patch(const PtrPatch& P, hipStream_t stream);
+
 // Packed input of the multi-slice modes, built once per execution from the
 // filtered rows so that the back-projection stages a sample with one 8- or
 // 16-byte load:

@@ -123,29 +123,47 @@ int fail(rv_plan_t p, const std::string& m)
     return -1;
 }
 
-bool upload_slices(rv_plan_t p, void* const* d_out)
+// Image pointers of the slice tables, patched by kernels on the execution's
+// stream (no host copy that could overtake work still queued on that stream).
+// all = true patches every entry (a graph capture: each replay then restores
+// its own pointers, whatever eager executions wrote in between).
+bool upload_slices(rv_plan_t p, void* const* d_out, hipStream_t st, bool all)
 {
+    rv::PtrPatch P{};
+    hipError_t err = hipSuccess;
+    auto add = [&](void** dst, void* val) {
+        P.dst[P.n] = dst;
+        P.val[P.n] = val;
+        if (++P.n == rv::kPtrPatch) {
+            rv::launch_ptr_patch(P, st);
+            if (err == hipSuccess) err = hipGetLastError();
+            P.n = 0;
+        }
+    };
     for (auto& G : p->groups) {
-        bool changed = false;
         for (size_t q = 0; q < G.slices.size(); q++) {
             const int s = G.slices[q];
-            if (p->cur_out[s] != d_out[s]) {
+            if (all || p->cur_out[s] != d_out[s]) {
                 G.h[q].out = d_out[s];
-                changed = true;
+                add(&G.d_slices[q].out, d_out[s]);
             }
             for (int e = 0; e + 1 < rv::BP_MAX_SLICES_PER_ITEM; e++) {
                 const int s2 = G.others[q][e];
-                if (s2 >= 0 && p->cur_out[s2] != d_out[s2]) {
+                if (s2 >= 0 && (all || p->cur_out[s2] != d_out[s2])) {
                     G.h[q].out_x[e] = d_out[s2];
-                    changed = true;
+                    add(&G.d_slices[q].out_x[e], d_out[s2]);
                 }
             }
         }
-        if (changed && !rv::hip_ok(hipMemcpy(G.d_slices, G.h.data(), sizeof(rv::BpSlice) * G.h.size(),
-                                             hipMemcpyHostToDevice), "slice table upload"))
-            return false;
     }
-    for (size_t s = 0; s < p->sl.size(); s++) p->cur_out[s] = d_out[s];
+    if (P.n) {
+        rv::launch_ptr_patch(P, st);
+        if (err == hipSuccess) err = hipGetLastError();
+    }
+    if (!rv::hip_ok(err, "slice table patch")) return false;
+    // a capture runs nothing: the caller records the pointers once the graph is launched
+    if (!all)
+        for (size_t s = 0; s < p->sl.size(); s++) p->cur_out[s] = d_out[s];
     return true;
 }
 
@@ -640,7 +658,7 @@ int rvPlanBackproject(rv_plan_t p, void* const* d_out, rv_stream_t stream)
 {
     if (!p || !all_set(p, (const void* const*)d_out, "image")) return -1;
     if (!rv::hip_ok(hipSetDevice(p->dev), "hipSetDevice")) return -1;
-    if (!upload_slices(p, d_out)) return -1;
+    if (!upload_slices(p, d_out, (hipStream_t)stream, false)) return -1;
     return launch_backprojection(p, (hipStream_t)stream);
 }
 
@@ -648,7 +666,7 @@ int rvPlanExecute(rv_plan_t p, const void* const* d_sino, void* const* d_out, rv
 {
     if (!p || !all_set(p, d_sino, "sinogram") || !all_set(p, (const void* const*)d_out, "image")) return -1;
     if (!rv::hip_ok(hipSetDevice(p->dev), "hipSetDevice")) return -1;
-    if (!upload_slices(p, d_out)) return -1;
+    if (!upload_slices(p, d_out, (hipStream_t)stream, false)) return -1;
     if (launch_filters(p, d_sino, (hipStream_t)stream)) return -1;
     return launch_backprojection(p, (hipStream_t)stream);
 }
@@ -665,21 +683,27 @@ int rvPlanExecuteGraph(rv_plan_t p, const void* const* d_sino, void* const* d_ou
         if (p->graph) { (void)hipGraphDestroy(p->graph); p->graph = nullptr; }
         if (!p->cap_stream && !rv::hip_ok(hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking), "capture stream"))
             return -1;
-        if (!upload_slices(p, d_out)) return -1;
-        // The graph holds kernel nodes only (no memset/memcpy nodes).
+        // The graph holds kernel nodes only (no memset/memcpy nodes); its first
+        // nodes write its image pointers into the slice tables.
         if (!rv::hip_ok(hipStreamBeginCapture(p->cap_stream, hipStreamCaptureModeThreadLocal), "begin capture"))
             return -1;
-        int rc = launch_filters(p, d_sino, p->cap_stream);
+        int rc = upload_slices(p, d_out, p->cap_stream, true) ? 0 : -1;
+        if (!rc) rc = launch_filters(p, d_sino, p->cap_stream);
         if (!rc) rc = launch_backprojection(p, p->cap_stream);
         hipGraph_t g = nullptr;
         const bool ended = rv::hip_ok(hipStreamEndCapture(p->cap_stream, &g), "end capture");
-        if (rc || !ended) { if (g) (void)hipGraphDestroy(g); return -1; }
+        if (rc || !ended) {
+            if (g) (void)hipGraphDestroy(g);
+            return -1;
+        }
         p->graph = g;
         if (!rv::hip_ok(hipGraphInstantiate(&p->gexec, g, nullptr, nullptr, 0), "graph instantiate")) return -1;
         p->g_sino.assign(d_sino, d_sino + ns);
         p->g_out.assign(d_out, d_out + ns);
     }
-    return rv::hip_ok(hipGraphLaunch(p->gexec, (hipStream_t)stream), "graph launch") ? 0 : -1;
+    if (!rv::hip_ok(hipGraphLaunch(p->gexec, (hipStream_t)stream), "graph launch")) return -1;
+    for (size_t s = 0; s < ns; s++) p->cur_out[s] = d_out[s];   // the replay wrote them
+    return 0;
 }
 
 int rvPlanFilteredRows(rv_plan_t p, int s, void** rows, float** scales, int* row_len)

@@ -112,6 +112,13 @@ class H5File:
         return self.base + a
 
     # --------------------------------------------------------- object header
+    @staticmethod
+    def _visit(seen, start, size):
+        """Continuation blocks that repeat would make the walk loop forever."""
+        if (start, size) in seen:
+            raise H5Error("object header continuation blocks form a cycle")
+        seen.add((start, size))
+
     def _messages(self, addr):
         """(type, data) of every message of the object header at addr."""
         a = self._addr(addr)
@@ -128,9 +135,11 @@ class H5File:
             # chunk 0: the messages follow the size field (its checksum comes after);
             # continuation chunks: "OCHK", messages, checksum
             blocks = [(a + p + csize, size0, False)]
+            seen = set()
             track = 2 if flags & 0x04 else 0
             while blocks:
                 start, size, cont = blocks.pop(0)
+                self._visit(seen, start, size)
                 data = self._read(start, size)
                 if cont and data[:4] != b"OCHK":
                     raise H5Error("bad object header continuation block")
@@ -152,8 +161,10 @@ class H5File:
         nmsg = self._uint(head, 2, 2)
         hsize = self._uint(head, 8, 4)
         blocks = [(a + 16, hsize)]
+        seen = set()
         while blocks and len(out) < nmsg:
             start, size = blocks.pop(0)
+            self._visit(seen, start, size)
             data = self._read(start, size)
             q = 0
             while q + 8 <= size:
@@ -195,18 +206,20 @@ class H5File:
         daddr = self._uint(h, 8 + 2 * self.sl, self.so)
         return self._read(self._addr(daddr), size)
 
-    def _walk_btree(self, node, heap, kids):
+    def _walk_btree(self, node, heap, kids, parent_level=None):
         h = self._read(self._addr(node), 8 + 2 * self.so)
         if h[:4] != b"TREE" or h[4] != 0:
             raise H5Error("bad group B-tree node")
         level, used = h[5], self._uint(h, 6, 2)
+        if parent_level is not None and level >= parent_level:
+            raise H5Error("group B-tree child is not below its parent (cycle)")
         body = self._read(self._addr(node) + 8 + 2 * self.so, (used + 1) * self.sl + used * self.so)
         p = self.sl                                   # key 0
         for _ in range(used):
             child = self._uint(body, p, self.so)
             p += self.so + self.sl                    # child, next key
             if level > 0:
-                self._walk_btree(child, heap, kids)
+                self._walk_btree(child, heap, kids, level)
             else:
                 self._read_snod(child, heap, kids)
 

@@ -84,3 +84,19 @@ def test_step_scan_exclusions_with_leading_frames_and_overturn():
     # 3 leading frames (8 in the data block, 5 angles); 189.99 and 200 are >= 179.99 past 10
     assert excl.split() == ["s0", "s1", "s2", "s6", "s7"]
     assert first == 3 and last == 5
+
+
+def test_reader_rejects_cyclic_continuation_blocks():
+    """An object header whose continuation message points back at its own block
+    ends in an error instead of an endless walk."""
+    import struct
+    raw = open(os.path.join(GOLD, "dxchange_step_f32.h5"), "rb").read()
+    f = H5File(raw)
+    so, sl = f.so, f.sl
+    a = len(raw)
+    body = (a + 16 - f.base).to_bytes(so, "little") + (8 + so + sl).to_bytes(sl, "little")
+    msg = struct.pack("<HHB3x", 0x10, len(body), 0) + body
+    head = struct.pack("<BBHII4x", 1, 0, 5, 1, len(msg))
+    g = H5File(raw + head + msg)
+    with pytest.raises(H5Error, match="cycle"):
+        g._messages(a - g.base)

@@ -108,6 +108,30 @@ def read_traffic(name):
     return float(m.group(1)) if m else None
 
 
+def host_cpu():
+    """(model, MHz, instruction-set flags, usable threads) of this host."""
+    model, mhz, flags = "unknown", None, []
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k = k.strip()
+                if k == "model name" and model == "unknown":
+                    model = v.strip()
+                elif k == "cpu MHz" and mhz is None:
+                    mhz = float(v)
+                elif k == "flags" and not flags:
+                    have = set(v.split())
+                    flags = [f for f in ("avx512f", "avx512dq", "avx512vl", "avx2", "fma") if f in have]
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return model, mhz, flags, usable
+
+
 def cpu_baseline(sino_f32, fdeg, c_det, threads):
     """The reference CPU reconstruction of one slice (int16 input, LUT16, AVX-512)."""
     from oracle import pyoracle as O
@@ -304,6 +328,59 @@ def run_dropin(args):
     print(json.dumps(result), flush=True)
 
 
+def d2h_inclusive(plan, specs, sino_list, outs, args, stream, S, updates):
+    """Steps whose window includes the image download, as the reference's timer
+    does (gazoDoc.cpp:3873-3991, D2H at cudaReconstHost.cpp:811): two image sets
+    and two plans alternate, so step k's images travel to pinned host memory on a
+    copy stream (default priority) while step k+1 back-projects into the other set.
+    Returns slices/s over args.steps such steps, from the first launch to the last
+    byte on the host."""
+    import torch
+    from recview_amd import fbp
+    plan_b = fbp.FbpPlan(specs, interp=args.interp, store=args.store, persistent=bool(args.persistent),
+                         tile=args.tile, angles_per_chunk=args.chunk, lds_half=bool(args.lds_half),
+                         slices_per_item=args.slices_per_item)
+    outs_b = [torch.empty_like(o) for o in outs]
+    host = torch.empty((S,) + tuple(outs[0].shape), dtype=outs[0].dtype, pin_memory=True)
+    copy = torch.cuda.Stream()
+    sets = [(plan, outs), (plan_b, outs_b)]
+    done = [None, None]
+
+    def step(k):
+        p, o = sets[k % 2]
+        if done[k % 2] is not None:
+            stream.wait_event(done[k % 2])          # the set's previous download has finished
+        p.execute(sino_list, o, stream=stream, graph=bool(args.graph))
+        ready = torch.cuda.Event()
+        ready.record(stream)
+        copy.wait_event(ready)
+        with torch.cuda.stream(copy):
+            for s in range(S):
+                host[s].copy_(o[s], non_blocking=True)
+        d = torch.cuda.Event()
+        d.record(copy)
+        done[k % 2] = d
+
+    for k in range(max(2, args.warmup)):
+        step(k)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for k in range(args.steps):
+        step(k)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t
+    if not torch.equal(host[S - 1], sets[(args.steps - 1) % 2][1][S - 1].cpu()):
+        raise RuntimeError("downloaded image differs from the device image")
+    plan_b.close()
+    nbytes = sum(o.numel() * o.element_size() for o in outs)
+    return {"value": round(S * args.steps / el, 3), "unit": "slices/s",
+            "ms_per_step": round(1e3 * el / args.steps, 3),
+            "tau_ns": round(el / args.steps / updates * 1e9, 7),
+            "image_bytes_per_step": nbytes,
+            "window": "filter + back-projection + D2H of every image to pinned host memory "
+                      "(copy stream overlapped with the next step's back-projection)"}
+
+
 def _blas_threads():
     try:
         from threadpoolctl import threadpool_info
@@ -332,6 +409,8 @@ def main():
                          "pairs, eight with FP16 pairs; or 1, 2, 4, 8)")
     ap.add_argument("--graph", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--d2h", type=int, default=1,
+                    help="c2: also time steps that download every image to pinned host memory")
     ap.add_argument("--cpu-threads", type=int, default=0)
     args = ap.parse_args()
 
@@ -415,16 +494,24 @@ def main():
         if not torch.isfinite(o.float()).all():
             raise RuntimeError("non-finite image values")
 
+    d2h = None
+    if args.workload == "c2" and args.d2h:
+        d2h = d2h_inclusive(plan, specs, sino_list, outs, args, stream, S, updates)
+
     cpu = None
     if not args.no_cpu and args.workload == "c2":
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        model, mhz, flags, usable = host_cpu()
+        # every usable host thread, up to the 16 that make one GPU's share of a GPU host
+        threads = args.cpu_threads or min(16, usable)
         src = sino_list[0] if args.interp != "lut16_i32" else sino_list[0].float() / 3000.0
         secs, isa = cpu_baseline(src, fdeg, c_det, threads)
         N = W
+        clock = f" @ {mhz:.0f} MHz" if mhz else ""
         cpu = {"value": round(1.0 / secs, 4), "unit": "slices/s", "cores": threads, "kind": "port",
                "sample": f"1 slice {P}x{W} -> {N}^2, int16 input, reference LUT16 path "
-                         f"({isa} restatement of projx64, row bands + per-thread filtering), "
-                         f"tau {secs / (N * N * P) * 1e9:.4f} ns"}
+                         f"({isa} restatement of projx64, row bands + per-thread filtering, "
+                         f"partial-image sum), tau {secs / (N * N * P) * 1e9:.4f} ns; host: {model}{clock}, "
+                         f"{threads} threads of {usable} usable, ISA flags {' '.join(flags) or 'none'}"}
 
     traffic = None
     if args.workload == "c2" and args.interp == "linear" and args.store == "f32" and S == 64 \
@@ -454,6 +541,7 @@ def main():
         "filter_ms": round(filt_ms, 3),
         "bp_ms": round(bp_ms, 3),
         "config": config,
+        "d2h_inclusive": d2h,
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
                      "traffic": traffic},

@@ -444,3 +444,25 @@ def test_sinogram_incident_search_quirks():
     # no incident frame at all -> error status
     _, st = O.sinogram(_frames(W, [smp, smp], dark), [1, 1], [0, 1], [0, 0])
     assert st == -1
+
+
+@pytest.mark.parametrize("isa", [0, 1, 2])
+def test_cpu_baseline_image_equals_reference_integer_path(isa):
+    """The timed CPU baseline (projx64 restated with intrinsics, row bands, per-thread
+    filtering, partial-image sum) computes the reference CPU path's image: equal bit
+    for bit to R-int with the CPU LUT formula and the double-precision centre, the
+    float-mask index rule for AVX2 / AVX-512 and truncation for the scalar loop.  An
+    odd thread count leaves bands of unequal height; a width that is not a multiple
+    of 16 exercises the masked tail of the vector loops."""
+    if isa > O.cpu_isa():
+        pytest.skip("host lacks the instruction set")
+    W, P = 120, 60
+    ell, fdeg, c_det, sino = _phantom(W, P, 91, c_off=0.37)
+    rows = phantom.to_int16(__import__("torch").from_numpy(sino)).numpy()
+    g = O.geometry(W, 2, c_det, float_center=False)
+    filt = O.set_filter(g.ndim, 0, 0.5, 6.0)
+    secs, img = O.cpu_baseline(rows, fdeg, g, filt, 3, isa=isa)
+    ref = O.recon_rint(rows, fdeg, g, filt, formula=0, rule=0 if isa > 0 else 1)
+    assert secs > 0
+    assert np.array_equal(img, ref)
+    assert np.abs(ref).max() > 1000

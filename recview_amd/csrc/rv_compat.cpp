@@ -79,6 +79,24 @@ void* host_stage(const RECONST_INFO* ri, size_t bytes)
     return p;
 }
 
+// Work rows of CudaDeconv's two-pass transforms (one strip per call), grow-only
+// per device and held until the process ends or the device is reset.
+std::mutex g_deconv_mu;
+std::map<int, std::pair<void*, size_t>> g_deconv_work;
+
+void* deconv_work(int dev, size_t bytes)
+{
+    std::lock_guard<std::mutex> lk(g_deconv_mu);
+    auto& w = g_deconv_work[dev];
+    if (w.first && w.second >= bytes) return w.first;
+    if (w.first) (void)hipFree(w.first);
+    w = {nullptr, 0};
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+    w = {p, bytes};
+    return p;
+}
+
 void release_stage(const RECONST_INFO* ri)
 {
     std::lock_guard<std::mutex> lk(g_stage_mu);
@@ -125,8 +143,8 @@ void reconst_body(RECONST_INFO* ri, int idev, bool bReport, bool bEnStream, bool
     }
     const rv::Geom g = rv::geometry(ri->ixdim, ri->iInterpolation, ri->center, true);
     const int W = g.W, N = g.N, n = g.n;
-    if (W < 4 || g.ndimp < 3 || n > 16384) {
-        rv::set_message("unsupported detector width / zoom (FFT length must be 8..16384)");
+    if (W < 4 || g.ndimp < 3 || g.ndimp > rv::kMaxFftLog2) {
+        rv::set_message("unsupported detector width / zoom (FFT length must be 8..2^20)");
         rv::log_code(28808);
         ri->iStatus = RECONST_INFO_ERROR;
         return;
@@ -251,7 +269,15 @@ void reconst_body(RECONST_INFO* ri, int idev, bool bReport, bool bEnStream, bool
                 J.out = ri->d_px;
                 J.out_stride = N;
                 J.store = 0;
-                ok = rv::hip_ok(rv::launch_filter(J, st), "filter launch");
+                if (g.ndimp <= rv::kMaxLdsFftLog2) {
+                    ok = rv::hip_ok(rv::launch_filter(J, st), "filter launch");
+                } else {
+                    // two-pass transform: its work rows live in the d_igp field, which
+                    // this path does not otherwise use (grow-only like the others)
+                    const size_t wrows = rv::filter_large_rows(n, (size_t)1 << 30, nsel);
+                    ok = grow(ri->d_igp, ri->max_d_igp, 2 * sizeof(float) * 2 * (size_t)n * wrows) &&
+                         rv::hip_ok(rv::launch_filter_large(J, ri->d_igp, wrows, st), "filter launch");
+                }
             }
             if (!ok) {
                 (void)hipStreamSynchronize(st);      // the staging buffer is reused by the next call
@@ -342,6 +368,8 @@ void CudaReconstResourceFree(RECONST_INFO* ri, bool bCudaDeviceReset)
         (void)hipDeviceSynchronize();
         (void)hipDeviceReset();
         rv::drop_device_caches(ri->iStartSino);
+        std::lock_guard<std::mutex> lk(g_deconv_mu);
+        g_deconv_work.erase(ri->iStartSino);                       // memory gone with the reset
     }
 }
 
@@ -542,7 +570,7 @@ void CudaDeconv(int ixdim, int iIntpDim, int ndim, float center, float* d_filt, 
     // n/2 - N/2, LUT with the p2igp formula.  d_p is scratch for the row.
     int np = 0;
     while ((1 << np) < ndim) np++;
-    if ((1 << np) != ndim || np < 3 || ndim > 16384) {
+    if ((1 << np) != ndim || np < 3 || np > rv::kMaxFftLog2) {
         rv::log_code(28808);
         return;
     }
@@ -563,7 +591,14 @@ void CudaDeconv(int ixdim, int iIntpDim, int ndim, float center, float* d_filt, 
     J.tw = tw;
     J.out = (float*)d_p;
     J.out_stride = N;
-    if (!rv::hip_ok(rv::launch_filter(J, nullptr), "filter launch")) {
+    hipError_t e;
+    if (np <= rv::kMaxLdsFftLog2) {
+        e = rv::launch_filter(J, nullptr);
+    } else {
+        void* work = deconv_work(dev, 2 * sizeof(float) * 2 * (size_t)ndim);
+        e = work ? rv::launch_filter_large(J, work, 1, nullptr) : hipErrorOutOfMemory;
+    }
+    if (!rv::hip_ok(e, "filter launch")) {
         rv::log_code(28804);
         return;
     }

@@ -19,6 +19,7 @@
 #include <hip/hip_fp16.h>
 #include <hip/hip_bf16.h>
 
+#include <algorithm>
 #include <map>
 #include <mutex>
 
@@ -270,6 +271,210 @@ __global__ void __launch_bounds__(1024) filter_rows_kernel(FilterJob J, FilterBa
             store_lp_row(J, FB.out_lp[bi], FB.out_scale[bi], outf, rout, red);
         }
     }
+}
+
+// ------------------------------------------------ transforms longer than LDS
+// n = 2^L above 16384 points: the complex row (8n bytes) no longer fits in LDS.
+// CFft's radix-2 Stockham network is then run in two passes per transform
+// through a work buffer, with every butterfly -- inputs, twiddle entry, operation
+// order -- that of the one-pass network, so the filtered rows stay bit-identical
+// to the reference filter.  Stage l maps the index (c | j | m) -- c the top bit,
+// m the low l bits -- to (j | c' | m).  Hence:
+//  * pass A, stages 0..K-1, only combines elements that share the low L-K bits
+//    j0 of their stage-0 index: 2^(L-K) independent sub-networks of 2^K elements
+//    at stride 2^(L-K), each ending in the contiguous block j0 * 2^K + (0..2^K);
+//  * pass B, stages K..L-1, never moves the low K bits r of an index: 2^K
+//    sub-networks of the elements r + q * 2^K, where local stage l' is global
+//    stage l' + K with twiddle index ((m << K) + r) * (n/2 >> (l' + K)).
+// The last global stage keeps its fused operations: the filter multiply and the
+// reversed output order of FFT1Rev (cfft.cpp:249-268), the crop of FFT1.
+// A workgroup runs 2048 / 2^LS sub-networks (adjacent j0 or r, so global reads
+// and writes coalesce); LDS holds them element-major with an odd pitch.
+constexpr int kLgBlock = 2048;
+constexpr int kLgThreads = 256;
+
+struct LgPass {
+    FilterJob J;
+    const float2* src;     // work rows of the batch (passes 1-3)
+    float2* dst;           // work rows (passes 0-2); pass 3: FP32 rows of 16-bit stores
+    int row0;              // first row of the batch
+    int L, K;              // log2 n; stages of pass A
+};
+
+// PASS 0: FFT1Rev stages 0..K-1 with row prep and the reversed first-stage read
+// (FFTrev reads x[n/2-1-j], cfft.cpp:199-211); 1: FFT1Rev stages K..L-1 and the
+// filter; 2: FFT1 stages 0..K-1; 3: FFT1 stages K..L-1, 1/n and the crop.
+template <int PASS, int LS, int IN_KIND>
+__global__ void __launch_bounds__(kLgThreads) fft_large_kernel(LgPass P)
+{
+    constexpr int NS = 1 << LS, G = kLgBlock >> LS, PITCH = G + 1;
+    constexpr bool passA = (PASS == 0 || PASS == 2);
+    constexpr int BF = kLgBlock / 2 / kLgThreads;           // butterflies per thread and stage
+    __shared__ float2 buf[NS * PITCH];
+    const FilterJob& J = P.J;
+    const int L = P.L, K = P.K;
+    const int n = 1 << L, half = n >> 1;
+    const int r = P.row0 + (int)blockIdx.y;
+    const size_t wrow = (size_t)blockIdx.y * n;
+    const int tid = threadIdx.x;
+    const int gbase = (int)blockIdx.x * G;                  // first j0 (pass A) or residue (pass B)
+    const int estride = passA ? (1 << (L - K)) : (1 << K);
+    const float2* tw = (const float2*)J.tw;
+
+    if constexpr (PASS == 0) {
+        const int rin = J.row_in ? J.row_in[r] : r;
+        const void* row = (IN_KIND == 0)
+            ? (const void*)((const int16_t*)J.sino + (long long)rin * J.in_stride)
+            : (const void*)((const float*)J.sino + (long long)rin * J.in_stride);
+        for (int idx = tid; idx < kLgBlock; idx += kLgThreads) {
+            const int g = idx % G, e = idx / G;
+            const int i = e * estride + gbase + g;
+            const int m = (i < half) ? half - 1 - i : n + half - 1 - i;
+            buf[e * PITCH + g] = make_float2(prep_value<IN_KIND>(J, row, m), 0.f);
+        }
+    } else {
+        for (int idx = tid; idx < kLgBlock; idx += kLgThreads) {
+            const int g = idx % G, e = idx / G;
+            buf[e * PITCH + g] = P.src[wrow + (size_t)e * estride + gbase + g];
+        }
+    }
+    __syncthreads();
+
+    constexpr int NSTAGE = passA ? LS : LS - 1;            // pass B ends with the fused last stage
+#pragma unroll 1
+    for (int l = 0; l < NSTAGE; l++) {
+        const int al = 1 << l;
+        const int lg = passA ? l : l + K;                  // global stage
+        const int bt = half >> lg;
+        float2 A[BF], B[BF];
+#pragma unroll
+        for (int i = 0; i < BF; i++) {
+            const int idx = tid + i * kLgThreads;
+            const int g = idx % G, bl = idx / G;
+            A[i] = buf[bl * PITCH + g];
+            B[i] = buf[(bl + NS / 2) * PITCH + g];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < BF; i++) {
+            const int idx = tid + i * kLgThreads;
+            const int g = idx % G, bl = idx / G;
+            const int j = bl >> l, m = bl & (al - 1);
+            float2 sv = B[i];
+            if (lg > 0) {
+                const int mg = passA ? m : (m << K) + gbase + g;
+                sv = cmul_tw(B[i], tw[mg * bt]);
+            }
+            const int o = (j << (l + 1)) + m;
+            buf[o * PITCH + g] = make_float2(A[i].x + sv.x, A[i].y + sv.y);
+            buf[(o + al) * PITCH + g] = make_float2(A[i].x - sv.x, A[i].y - sv.y);
+        }
+        __syncthreads();
+    }
+
+    if constexpr (passA) {
+        // after K stages sub-network j0 holds the contiguous block j0 * 2^K + e
+        for (int idx = tid; idx < kLgBlock; idx += kLgThreads) {
+            const int e = idx % NS, g = idx / NS;
+            P.dst[wrow + (size_t)(gbase + g) * NS + e] = buf[e * PITCH + g];
+        }
+    } else {
+        // global last stage: butterfly k = (bl << K) + r reads x[k] and x[k + n/2]
+#pragma unroll
+        for (int i = 0; i < BF; i++) {
+            const int idx = tid + i * kLgThreads;
+            const int g = idx % G, bl = idx / G;
+            const int k = (bl << K) + gbase + g;
+            const float2 a = buf[bl * PITCH + g];
+            const float2 b = buf[(bl + NS / 2) * PITCH + g];
+            if constexpr (PASS == 1) {
+                const float2 sv = (k == 0) ? b : cmul_tw(b, tw[k]);
+                const int ip = (k == 0) ? 0 : (n - k);
+                const int im = half - k;
+                const float hp = J.filt[ip], hm = J.filt[im];
+                const float pr = a.x + sv.x, pi = a.y + sv.y;
+                const float mr = a.x - sv.x, mi = a.y - sv.y;
+                P.dst[wrow + ip] = make_float2(pr * hp, pi * hp);
+                P.dst[wrow + im] = make_float2(mr * hm, mi * hm);
+            } else {
+                const float sc = 1.f / (float)n;
+                const float2 sv = cmul_tw(b, tw[k]);
+                const float v0 = (a.x - sv.x) * sc;        // f[k]
+                const float v1 = (a.x + sv.x) * sc;        // f[k + n/2]
+                const int c0 = n - 1 - k - J.crop;
+                const int c1 = n - 1 - (k + half) - J.crop;
+                float* o;
+                if (J.store == 0) {
+                    const int rout = J.row_out ? J.row_out[r] : r;
+                    o = J.out + (long long)rout * J.out_stride;
+                } else {
+                    o = (float*)(P.dst + wrow);             // FP32 row, scaled and stored below
+                }
+                if (c0 >= 0 && c0 < J.N) o[c0] = v0;
+                if (c1 >= 0 && c1 < J.N) o[c1] = v1;
+            }
+        }
+    }
+}
+
+// 16-bit stores of the two-pass filter: per-row scale and conversion of the FP32 rows
+__global__ void __launch_bounds__(kLgThreads) lp_rows_kernel(FilterJob J, const float2* stage, int row0, int n)
+{
+    const int r = row0 + (int)blockIdx.x;
+    const int rout = J.row_out ? J.row_out[r] : r;
+    __shared__ float red[32];
+    store_lp_row(J, J.out_lp, J.out_scale, (const float*)(stage + (size_t)blockIdx.x * n), rout, red);
+}
+
+template <int PASS, int IN_KIND>
+static void launch_lg(const LgPass& P, int ls, int nb, hipStream_t st)
+{
+    const dim3 grid((unsigned)((1 << P.L) / kLgBlock), (unsigned)nb), block(kLgThreads);
+    switch (ls) {
+    case 7: hipLaunchKernelGGL((fft_large_kernel<PASS, 7, IN_KIND>), grid, block, 0, st, P); break;
+    case 8: hipLaunchKernelGGL((fft_large_kernel<PASS, 8, IN_KIND>), grid, block, 0, st, P); break;
+    case 9: hipLaunchKernelGGL((fft_large_kernel<PASS, 9, IN_KIND>), grid, block, 0, st, P); break;
+    default: hipLaunchKernelGGL((fft_large_kernel<PASS, 10, IN_KIND>), grid, block, 0, st, P); break;
+    }
+}
+
+size_t filter_large_rows(int n, size_t budget_bytes, int nrows)
+{
+    size_t rows = budget_bytes / (2 * sizeof(float2) * (size_t)n);
+    if (rows < 1) rows = 1;
+    if (nrows > 0 && rows > (size_t)nrows) rows = (size_t)nrows;
+    return rows;
+}
+
+hipError_t launch_filter_large(const FilterJob& J, void* work, size_t work_rows, hipStream_t st)
+{
+    if (J.nrows <= 0) return hipSuccess;
+    if (J.pair || J.np <= kMaxLdsFftLog2 || J.np > kMaxFftLog2 || J.n != (1 << J.np) || !work || work_rows < 1)
+        return hipErrorInvalidValue;
+    const int L = J.np, K = L / 2, n = J.n;
+    float2* w0 = (float2*)work;
+    float2* w1 = w0 + work_rows * (size_t)n;
+    for (int row0 = 0; row0 < J.nrows; row0 += (int)work_rows) {
+        const int nb = (int)std::min<size_t>(work_rows, (size_t)(J.nrows - row0));
+        LgPass P{};
+        P.J = J;
+        P.row0 = row0;
+        P.L = L;
+        P.K = K;
+        P.src = nullptr; P.dst = w0;
+        if (J.in_kind == 0) launch_lg<0, 0>(P, K, nb, st);
+        else launch_lg<0, 1>(P, K, nb, st);
+        P.src = w0; P.dst = w1;
+        launch_lg<1, 1>(P, L - K, nb, st);
+        P.src = w1; P.dst = w0;
+        launch_lg<2, 1>(P, K, nb, st);
+        P.src = w0; P.dst = w1;
+        launch_lg<3, 1>(P, L - K, nb, st);
+        if (J.store != 0)
+            hipLaunchKernelGGL(lp_rows_kernel, dim3((unsigned)nb), dim3(kLgThreads), 0, st, J, (const float2*)w1,
+                               row0, n);
+    }
+    return hipGetLastError();
 }
 
 // ------------------------------------------------ paired rows, radix-4 transform

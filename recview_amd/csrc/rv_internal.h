@@ -48,6 +48,15 @@ struct FilterBatch {
 };
 hipError_t launch_filter_batch(const FilterJob& job, const FilterBatch& batch, int count, hipStream_t stream);
 void prepare_filter(int n);   // kernel attributes for an n-point transform (not capturable)
+// Transforms longer than 2^kMaxLdsFftLog2 points (the complex row no longer fits
+// in LDS): CFft's network in two passes per transform through a work buffer of
+// 2 * work_rows * n float2 (rows processed work_rows at a time), bit-identical
+// to the one-pass network.  job.pair must be 0; up to 2^kMaxFftLog2 points.
+constexpr int kMaxLdsFftLog2 = 14;
+constexpr int kMaxFftLog2 = 20;
+hipError_t launch_filter_large(const FilterJob& job, void* work, size_t work_rows, hipStream_t stream);
+// Rows per batch that fit a work buffer of budget_bytes (at most nrows, at least 1).
+size_t filter_large_rows(int n, size_t budget_bytes, int nrows);
 
 // LUT build (cudaReconst.cu:230-246) and single-projection LUT back-projection
 // (cudaReconst.cu:31-39, 195-228) for the kernel-wrapper entry points.

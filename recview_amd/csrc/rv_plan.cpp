@@ -80,6 +80,8 @@ struct rv_plan_opaque {
     bool pack_half = false;
     struct PackRange { int first, count, kind, rows; };
     std::vector<PackRange> pack_ranges;  // one per group: jobs of one kind
+    void* d_fft_work = nullptr;          // two-pass transforms (n > 16384): work rows
+    size_t fft_work_rows = 0;
     std::vector<void*> cur_out;
     hipStream_t cap_stream = nullptr;
     hipGraph_t graph = nullptr;
@@ -113,6 +115,7 @@ void plan_free(rv_plan_t p)
     (void)hipFree(p->d_packed);
     (void)hipFree(p->d_gscale);
     (void)hipFree(p->d_pack_jobs);
+    (void)hipFree(p->d_fft_work);
     delete p;
 }
 
@@ -183,7 +186,7 @@ rv::FilterJob filter_job(rv_plan_t p, size_t s)
     J.crop = zero ? g.n / 2 - g.h : g.ihoffset;
     J.filt = p->d_filts + S.filt_off;
     J.tw = rv::device_twiddles(p->dev, g.ndimp);
-    if (p->pair_filter) {
+    if (p->pair_filter && g.ndimp <= rv::kMaxLdsFftLog2) {
         J.pair = 1;
         J.filt = p->d_filts + p->filt_total + S.filt_off;
         J.tw = rv::device_twiddles_full(p->dev, g.ndimp);
@@ -231,8 +234,16 @@ int launch_filters(rv_plan_t p, const void* const* d_sino, hipStream_t st)
             rv::launch_stitch(J, st);
             src = J.out;
         }
-        const rv::FilterJob J = filter_job(p, s);
+        rv::FilterJob J = filter_job(p, s);
         if (!J.tw) { rv::set_message("twiddle table allocation failed"); return -1; }
+        if (S.g.ndimp > rv::kMaxLdsFftLog2) {
+            // two-pass transform through the plan's work rows, one slice at a time
+            flush();
+            cls = -1;
+            J.sino = src;
+            if (err == hipSuccess) err = rv::launch_filter_large(J, p->d_fft_work, p->fft_work_rows, st);
+            continue;
+        }
         if (count == rv::kFilterBatch || S.fclass != cls) flush();
         if (count == 0) {
             head = J;
@@ -329,8 +340,8 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
             S.fdeg.assign(specs[s].fdeg, specs[s].fdeg + specs[s].P);
         }
         S.g = rv::geometry(W, S.spec.iInterpolation, center, true);
-        if (W < 4 || S.g.ndimp < 3 || S.g.n > 16384)
-            return fail(p, "detector width / zoom outside the supported FFT range (n <= 16384)");
+        if (W < 4 || S.g.ndimp < 3 || S.g.ndimp > rv::kMaxFftLog2)
+            return fail(p, "detector width / zoom outside the supported FFT range (8 <= n <= 2^20)");
         if (S.spec.out_n > 0) {
             if (!linear) return fail(p, "custom output grids need the LINEAR mode");
             if (S.g.z != 1) return fail(p, "custom output grids need iInterpolation = 2");
@@ -624,14 +635,28 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
             return fail(p, std::string("plan upload failed: ") + RecviewLastMessage());
     }
 
+    // work rows of the two-pass transforms (1 GiB at most; batches of rows beyond)
+    int big_n = 0, big_rows = 0;
+    for (const auto& S : p->sl)
+        if (S.g.ndimp > rv::kMaxLdsFftLog2) {
+            big_n = std::max(big_n, S.g.n);
+            big_rows = std::max(big_rows, S.nang);
+        }
+    if (big_n) {
+        p->fft_work_rows = rv::filter_large_rows(big_n, (size_t)1 << 30, big_rows);
+        if (!rv::hip_ok(hipMalloc(&p->d_fft_work, 2 * sizeof(float) * 2 * (size_t)big_n * p->fft_work_rows),
+                        "two-pass transform work rows"))
+            return fail(p, std::string("plan allocation failed: ") + RecviewLastMessage());
+    }
+
     // Everything that allocates or sets kernel attributes happens here, so that
     // execution (and hipGraph capture of it) only launches kernels.
     int max_n = 0;
     for (const auto& S : p->sl) {
         if (!rv::device_twiddles(device, S.g.ndimp) ||
-            (p->pair_filter && !rv::device_twiddles_full(device, S.g.ndimp)))
+            (p->pair_filter && S.g.ndimp <= rv::kMaxLdsFftLog2 && !rv::device_twiddles_full(device, S.g.ndimp)))
             return fail(p, "twiddle table allocation failed");
-        max_n = std::max(max_n, S.g.n);
+        if (S.g.ndimp <= rv::kMaxLdsFftLog2) max_n = std::max(max_n, S.g.n);
     }
     rv::prepare_filter(max_n);
     *out = p;

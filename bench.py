@@ -65,7 +65,7 @@ def make_stack(nslices, P, W, seed, device, photons=1e4):
     return out, fdeg, c_det
 
 
-def make_c3(nslices, seed, device, interp, store):
+def make_c3(nslices, seed, device, interp, store, lds_half=False, slices_per_item=0):
     """256 uneven slices; a quarter are 360-degree offset-axis scans (stitched to 2W)."""
     import numpy as np
     import torch
@@ -91,10 +91,12 @@ def make_c3(nslices, seed, device, interp, store):
             fdeg = phantom.angles_deg(P)
             sino = phantom.sinogram(ell, W, fdeg, c_det, scale=W, device=device).float()
             N, Pe = W, P
-        tile = autotune.tile_for(N, Pe, interp, store, table)
-        specs.append(fbp.SliceSpec(W, fdeg, c_det, tile=tile, offset_ct=offset))
+        specs.append(fbp.SliceSpec(W, fdeg, c_det, offset_ct=offset))
         sinos.append(sino.contiguous())
         shapes.append((N, Pe))
+    mode = autotune.mode_of(interp, store, lds_half, slices_per_item)
+    if mode:
+        autotune.apply(specs, [n for n, _ in shapes], mode, table)
     return specs, sinos, shapes
 
 
@@ -446,7 +448,8 @@ def main():
                 f"{args.store} filtered sinogram")
     else:
         S = args.slices or 256
-        specs, sino_list, shapes = make_c3(S, 1003, dev, args.interp, args.store)
+        specs, sino_list, shapes = make_c3(S, 1003, dev, args.interp, args.store, bool(args.lds_half),
+                                           args.slices_per_item)
         desc = (f"C3: {S} uneven slices (W 512..4096, P 450..3600, 1/4 offset-axis 360-deg scans "
                 f"stitched to 2W), per-shape tiles, {args.interp} {args.store}")
     torch.cuda.synchronize()

@@ -262,6 +262,10 @@ typedef struct {
      * those, with the centre remap of gazoDoc.cpp:3832-3837. */
     int offset_ct;
     const char* binc;
+    /* Angles per LDS chunk of this slice's back-projection (an autotuned value):
+     * 0 = the plan's (RvOptions.angles_per_chunk, else the mode's default).  The
+     * slices of one launch (same tile and mode) take the first one's. */
+    int chunk;
 } RvSliceSpec;
 
 typedef struct {

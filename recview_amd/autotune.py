@@ -1,10 +1,20 @@
-"""Per-shape tile autotuner for the back-projection kernel.
+"""Per-shape autotuner of the back-projection launch: tile and angle chunk per mode.
 
-For every (image side N, projections P) shape it times each tile configuration
-(64x64, 64x128, 32x32, 128x64) on a batch large enough to fill the GPU, using
-the BP stage alone (filtered rows uploaded once), and keeps the fastest.  The
-table is stored next to this module (tuned_tiles.json) and looked up by
-`tile_for`, nearest shape in log-space.  Plans then carry one tile per slice.
+The plan picks one back-projection kernel per work-item kind (the mode):
+
+  linear2    FP32 filtered rows, two slices of one geometry per work item (default)
+  linear     FP32, one slice per work item (slices without a partner of equal geometry)
+  h8         FP16 filtered rows and FP16 pairs in LDS, eight slices per work item
+  h4         the same with four slices per work item
+  lut16_i32  the reference LUT16 in int32 (the drop-in semantics)
+
+For every (mode, image side N, projections P) the tuner times each valid tile
+configuration (64x64, 64x128, 32x32, 128x64; eight-slice items exist on 64x64 and
+32x32 only) and each candidate chunk on a batch that fills the GPU, back-projection
+stage only (pack included), and keeps the fastest.  The table (tuned_tiles.json
+next to this module) is keyed "mode:N:P"; `tile_for` / `chunk_for` return the
+entry of the nearest tuned shape of the same mode (log distance), and `apply`
+writes them into slice specs before a plan is built.
 """
 import json
 import math
@@ -12,88 +22,146 @@ import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE = os.path.join(HERE, "tuned_tiles.json")
-TILES = (0, 1, 2, 3)
-TILE_PIXELS = {0: 64 * 64, 1: 64 * 128, 2: 32 * 32, 3: 128 * 64}
+
+MODES = ("linear2", "linear", "h8", "h4", "lut16_i32")
+TILES = {"linear2": (0, 1, 2, 3), "linear": (0, 1, 2, 3), "h8": (0, 2), "h4": (0, 1, 2, 3),
+         "lut16_i32": (0, 1, 2, 3)}
+CHUNKS = {"linear2": (8, 16, 24, 32), "linear": (16, 32, 48), "h8": (16, 24, 32), "h4": (16, 24, 32),
+          "lut16_i32": (16, 32, 48)}
+# plan options of each mode (the slices of a tuning batch share one geometry, so
+# they group as the mode says)
+PLAN_KW = {
+    "linear2": dict(interp="linear", store="f32", slices_per_item=2),
+    "linear": dict(interp="linear", store="f32", slices_per_item=1),
+    "h8": dict(interp="linear", store="f16", lds_half=True, slices_per_item=8),
+    "h4": dict(interp="linear", store="f16", lds_half=True, slices_per_item=4),
+    "lut16_i32": dict(interp="lut16_i32"),
+}
+GROUP = {"linear2": 2, "linear": 1, "h8": 8, "h4": 4, "lut16_i32": 1}
 
 
-def _key(N, P, interp, store):
-    return f"{interp}:{store}:{N}:{P}"
+def mode_of(interp="linear", store="f32", lds_half=False, slices_per_item=0):
+    """The mode a plan runs for slices of one geometry under these options."""
+    if interp == "lut16_i32":
+        return "lut16_i32"
+    if interp != "linear":
+        return None
+    if store != "f32" and lds_half:
+        return "h4" if slices_per_item == 4 else "h8"
+    if store == "f32":
+        return "linear" if slices_per_item == 1 else "linear2"
+    return None
 
 
-def load():
-    if os.path.exists(TABLE):
-        with open(TABLE) as f:
+def load(path=None):
+    path = path or TABLE
+    if os.path.exists(path):
+        with open(path) as f:
             return json.load(f)
     return {}
 
 
-def tile_for(N, P, interp="linear", store="f32", table=None):
-    """Best tile id for a shape (nearest tuned shape of the same mode), or 0."""
-    t = load() if table is None else table
-    best, dist = 0, float("inf")
-    for k, v in t.items():
-        i, s, n, p = k.split(":")
-        if i != interp or s != store:
+def _nearest(N, P, mode, table):
+    best, dist = None, float("inf")
+    for k, v in table.items():
+        parts = k.split(":")
+        if len(parts) != 3 or parts[0] != mode:
             continue
-        d = abs(math.log(int(n) / N)) + abs(math.log(int(p) / P))
+        d = abs(math.log(int(parts[1]) / N)) + abs(math.log(int(parts[2]) / P))
         if d < dist:
-            best, dist = v["tile"], d
+            best, dist = v, d
     return best
 
 
-def tune_shape(N, P, interp="linear", store="f32", reps=3, min_tiles=4096, device=0):
-    """Times every tile configuration for slices of side N with P projections."""
-    import numpy as np
+def tile_for(N, P, mode="linear2", table=None):
+    """Tuned tile id for the nearest shape of this mode (0 when the mode is untuned)."""
+    e = _nearest(N, P, mode, load() if table is None else table)
+    return int(e["tile"]) if e else 0
+
+
+def chunk_for(N, P, mode="linear2", table=None):
+    """Tuned angles per chunk for the nearest shape of this mode (0 = the plan's default)."""
+    e = _nearest(N, P, mode, load() if table is None else table)
+    return int(e.get("chunk", 0)) if e else 0
+
+
+def apply(specs, sides, mode, table=None):
+    """Sets tile and chunk of every slice spec from the table: sides[i] is the image
+    side slice i reconstructs (2W for an offset-axis scan), its projection count the
+    rows it reconstructs."""
+    t = load() if table is None else table
+    for spec, N in zip(specs, sides):
+        P = (spec.P + 1) // 2 if spec.offset_ct else spec.P
+        e = _nearest(N, P, mode, t)
+        if e:
+            spec.tile = int(e["tile"])
+            spec.chunk = int(e.get("chunk", 0))
+    return specs
+
+
+def tune_shape(N, P, mode, reps=3, min_items=4096, device=0, log=None):
+    """Times every (tile, chunk) of a mode for slices of side N with P projections."""
     import torch
     from . import fbp, phantom
     fdeg = phantom.angles_deg(P)
-    nsl = max(1, min(64, min_tiles // max(1, (N // 64) ** 2)))
+    group = GROUP[mode]
+    tiles_per_slice = max(1, (N // 64) ** 2)
+    items = -(-min_items // tiles_per_slice)                  # work items to fill the GPU
+    nsl = group * max(1, min(64 // group, items))
     dev = torch.device("cuda", device)
-    rows = torch.randn(P, N, device=dev) * 1e-3
+    kw = PLAN_KW[mode]
+    odt = torch.int32 if mode == "lut16_i32" else torch.float32
+    outs = [torch.empty(N, N, device=dev, dtype=odt) for _ in range(nsl)]
+    sino = (torch.rand(P, N, device=dev) * 0.01).contiguous()
+    if mode == "lut16_i32":
+        sino = (sino * 3000).to(torch.int16)
     res = {}
-    for tile in TILES:
-        specs = [fbp.SliceSpec(N, fdeg, N / 2 + 0.37, tile=tile) for _ in range(nsl)]
-        plan = fbp.FbpPlan(specs, interp=interp, store=store, persistent=True, tile=tile, device=device)
-        for s in range(nsl):
-            plan.set_filtered_rows(s, rows)
-        outs = [torch.empty(N, N, device=dev) for _ in range(nsl)]
-        plan.backproject(outs)
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        for _ in range(reps):
+    for tile in TILES[mode]:
+        for chunk in CHUNKS[mode]:
+            specs = [fbp.SliceSpec(N, fdeg, N / 2 + 0.37, tile=tile, chunk=chunk,
+                                   sino_dtype="i16" if mode == "lut16_i32" else "f32") for _ in range(nsl)]
+            plan = fbp.FbpPlan(specs, persistent=True, tile=tile, device=device, **kw)
+            plan.filter([sino] * nsl)
             plan.backproject(outs)
-        e1.record()
-        torch.cuda.synchronize()
-        ms = e0.elapsed_time(e1) / reps
-        res[tile] = ms / nsl
-        plan.close()
-        del outs
-    best = min(res, key=res.get)
-    return {"tile": best, "ms_per_slice": {str(k): round(v, 4) for k, v in res.items()},
-            "updates_per_s": N * N * P / (res[best] * 1e-3)}
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                plan.backproject(outs)
+            e1.record()
+            torch.cuda.synchronize()
+            res[(tile, chunk)] = e0.elapsed_time(e1) / reps / nsl
+            plan.close()
+    (bt, bc) = min(res, key=res.get)
+    if log:
+        log(f"  {mode} N={N} P={P}: tile {bt} chunk {bc} {res[(bt, bc)]:.4f} ms/slice")
+    return {"tile": bt, "chunk": bc, "ms_per_slice": round(res[(bt, bc)], 5),
+            "updates_per_s": N * N * P / (res[(bt, bc)] * 1e-3),
+            "all": {f"{t}/{c}": round(v, 5) for (t, c), v in sorted(res.items())}}
 
 
-def tune(shapes, interp="linear", store="f32", save=True, log=print):
-    table = load()
-    for N, P in shapes:
-        r = tune_shape(N, P, interp, store)
-        table[_key(N, P, interp, store)] = r
-        log(f"tuned N={N} P={P}: tile {r['tile']} {r['ms_per_slice']}")
-    if save:
-        with open(TABLE, "w") as f:
-            json.dump(table, f, indent=1, sort_keys=True)
+def _log(msg):
+    print(msg, flush=True)
+
+
+def tune(shapes, modes=MODES, path=None, log=_log):
+    path = path or TABLE
+    table = load(path)
+    for mode in modes:
+        for N, P in shapes:
+            table[f"{mode}:{N}:{P}"] = tune_shape(N, P, mode, log=log)
+            with open(path, "w") as f:          # written as it goes (a long sweep keeps its results)
+                json.dump(table, f, indent=1, sort_keys=True)
     return table
 
 
 if __name__ == "__main__":
     import argparse
-    ap = argparse.ArgumentParser(description="tune the back-projection tile per shape")
+    ap = argparse.ArgumentParser(description="tune the back-projection tile and chunk per mode and shape")
     ap.add_argument("--sides", default="512,1024,2048,4096,8192")
     ap.add_argument("--proj", default="450,900,1800,3600")
+    ap.add_argument("--modes", default=",".join(MODES))
     ap.add_argument("--out", default=None, help="write the table here instead of the package")
     a = ap.parse_args()
-    if a.out:
-        TABLE = a.out
     shapes = [(int(n), int(p)) for n in a.sides.split(",") for p in a.proj.split(",")]
-    tune(shapes)
+    tune(shapes, modes=a.modes.split(","), path=a.out)

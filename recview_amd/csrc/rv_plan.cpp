@@ -483,7 +483,8 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
                 Group G;
                 G.tile = p->sl[s].tile;
                 G.mode = mode;
-                G.chunk = opt->angles_per_chunk > 0 ? opt->angles_per_chunk : rv::bp_default_chunk(mode, G.tile);
+                G.chunk = p->sl[s].spec.chunk > 0 ? p->sl[s].spec.chunk
+                        : opt->angles_per_chunk > 0 ? opt->angles_per_chunk : rv::bp_default_chunk(mode, G.tile);
                 if (G.chunk > rv::bp_max_chunk(mode)) G.chunk = rv::bp_max_chunk(mode);
                 while (G.chunk > 1 && rv::bp_lds_bytes(mode, G.tile, G.chunk) > 160 * 1024) G.chunk /= 2;
                 p->groups.push_back(G);

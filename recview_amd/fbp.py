@@ -37,11 +37,13 @@ class SliceSpec:
 
     def __init__(self, W, fdeg, center, iInterpolation=2, tilt=0.0, filter="hann", cutoff=0.5,
                  order=6.0, sino_dtype="f32", pad="replicate", out_n=0, pix_zoom=1.0, x0=0.0,
-                 y0=0.0, tile=-1, offset_ct=False, binc=None):
+                 y0=0.0, tile=-1, offset_ct=False, binc=None, chunk=0):
         """center follows RECONST_INFO.center (user centre + 0.5); offset_ct: the
         rows are a 360-degree scan (angles over 0..360) with an off-centre axis,
-        stitched to double width on the GPU; tile = -1 uses the plan's tile."""
+        stitched to double width on the GPU; tile = -1 uses the plan's tile;
+        chunk = 0 the plan's angles per LDS chunk (see autotune.apply)."""
         self.tile = int(tile)
+        self.chunk = int(chunk)
         self.offset_ct = bool(offset_ct)
         self.binc = None if binc is None else np.ascontiguousarray(binc, np.int8)
         self.W = int(W)
@@ -69,6 +71,7 @@ class SliceSpec:
         s.out_n, s.pix_zoom, s.x0, s.y0 = self.out_n, self.pix_zoom, self.x0, self.y0
         s.tile, s.offset_ct = self.tile, int(self.offset_ct)
         s.binc = None if self.binc is None else self.binc.ctypes.data_as(C.c_char_p)
+        s.chunk = self.chunk
         return s
 
 

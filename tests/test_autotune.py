@@ -1,0 +1,48 @@
+"""Autotuner lookups (CPU): the table is keyed by back-projection mode, and a
+slice gets the tile and chunk of the nearest tuned shape of its own mode."""
+from recview_amd import autotune, fbp
+
+TABLE = {
+    "linear2:2048:1800": {"tile": 1, "chunk": 16},
+    "linear2:512:450": {"tile": 2, "chunk": 32},
+    "h8:2048:1800": {"tile": 0, "chunk": 24},
+    "h8:512:450": {"tile": 2, "chunk": 16},
+    "lut16_i32:2048:1800": {"tile": 3, "chunk": 48},
+    "linear:f32:2048:1800": {"tile": 2},           # a key of another layout is ignored
+}
+
+
+def test_lookup_per_mode():
+    assert autotune.tile_for(2048, 1800, "linear2", TABLE) == 1
+    assert autotune.chunk_for(2048, 1800, "linear2", TABLE) == 16
+    assert autotune.tile_for(2048, 1800, "h8", TABLE) == 0
+    assert autotune.chunk_for(2048, 1800, "h8", TABLE) == 24
+    assert autotune.tile_for(2048, 1800, "lut16_i32", TABLE) == 3
+    # nearest shape in log distance within the mode
+    assert autotune.tile_for(600, 500, "linear2", TABLE) == 2
+    assert autotune.chunk_for(4096, 3600, "h8", TABLE) == 24
+    # an untuned mode keeps the plan's defaults
+    assert autotune.tile_for(2048, 1800, "h4", TABLE) == 0
+    assert autotune.chunk_for(2048, 1800, "h4", TABLE) == 0
+
+
+def test_mode_of_plan_options():
+    assert autotune.mode_of("linear", "f32") == "linear2"
+    assert autotune.mode_of("linear", "f32", slices_per_item=1) == "linear"
+    assert autotune.mode_of("linear", "f16", lds_half=True) == "h8"
+    assert autotune.mode_of("linear", "bf16", lds_half=True, slices_per_item=4) == "h4"
+    assert autotune.mode_of("lut16_i32") == "lut16_i32"
+    assert autotune.mode_of("lut16_f32") is None
+
+
+def test_apply_sets_specs():
+    import numpy as np
+    fd = np.arange(450, dtype=np.float32) * 0.4
+    specs = [fbp.SliceSpec(2048, np.arange(1800, dtype=np.float32) * 0.1, 1024.37),
+             fbp.SliceSpec(256, np.arange(901, dtype=np.float32) * 0.4, 60.0, offset_ct=True),
+             fbp.SliceSpec(512, fd, 256.37)]
+    autotune.apply(specs, [2048, 512, 512], "h8", TABLE)
+    assert (specs[0].tile, specs[0].chunk) == (0, 24)
+    assert (specs[1].tile, specs[1].chunk) == (2, 16)      # 451 rows after stitching
+    assert (specs[2].tile, specs[2].chunk) == (2, 16)
+    assert specs[0].to_c().chunk == 24

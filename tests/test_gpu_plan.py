@@ -504,7 +504,10 @@ def test_plan_filtered_rows_non_pow2(W, interp):
     rmax = np.abs(ref64).max(axis=1, keepdims=True)
     spec = fbp.SliceSpec(W, fdeg, c, iInterpolation=interp)
     for exact in (False, True):
-        for store, tol in (("f32", 5e-6), ("f16", 1e-3), ("bf16", 6e-3)):
+        # FP32: rounding of the shared transform relative to the rows' magnitudes (a row
+        # with a small maximum next to a large partner sees its partner's rounding);
+        # a column left unwritten errs by the order of the row maximum itself
+        for store, tol in (("f32", 1e-4), ("f16", 1e-3), ("bf16", 6e-3)):
             if exact and store == "f32":
                 continue
             plan = fbp.FbpPlan([spec], store=store, exact_filter=exact)
@@ -516,6 +519,10 @@ def test_plan_filtered_rows_non_pow2(W, interp):
             print(f"W={W} z={g.z} n={g.ndim} N={g.N} store={store} exact={exact}: "
                   f"max {err.max():.2e} of the row maximum (column {int(err.max(axis=0).argmax())})")
             assert err.max() <= tol, (W, store, exact)
+            if store == "f32":
+                # as accurate as the reference's own FP32 filter (CFft order) against FP64
+                e_ref = np.abs(O.filter_rows(s, g, O.set_filter(g.ndim, 0), kind=1) - ref64).max()
+                assert np.abs(got - ref64).max() <= 4 * e_ref, (np.abs(got - ref64).max(), e_ref)
 
 
 @pytest.mark.parametrize("W,nsl,store,lds_half,rms_tol", [

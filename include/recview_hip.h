@@ -386,6 +386,18 @@ int rvConv3x3WgradBf16(const unsigned short* d_dy, int dy_stride, int dy_off, in
  * one when equal to min). */
 int RecviewOutputStage(const void* d_img, int is_float, int N, double center, int iInterpolation,
                        unsigned short* d_out, double* vmin, double* vmax, rv_stream_t stream);
+/* RecviewOutputStage without any allocation, for callers that cannot allocate
+ * device memory (code under hipGraph stream capture, serving loops, harnesses
+ * that forbid hipMalloc); it has no counterpart in the reference.  The same
+ * kernels and results, asynchronous on `stream`: the (min, max) pair is written
+ * to the caller's device buffer d_range (two values of the image type) instead
+ * of being copied to the host.  The per-block partial minima and maxima live in
+ * a library-held 8 KiB device array per GPU (held until the process ends), so
+ * calls that overlap on one GPU race; the partials are capped at 1024, which
+ * serves every image side up to the limit N <= 65536 (larger N: error). */
+#define RV_OUTPUT_NOALLOC_MAX_SIDE 65536
+int RecviewOutputStageNoAlloc(const void* d_img, int is_float, int N, double center, int iInterpolation,
+                              unsigned short* d_out, void* d_range, rv_stream_t stream);
 /* The ImageDescription string RecView stores in every slice file
  * (gazoDoc.cpp:1986-1988): sinogram count, pixel width, pixelDiv, pixelBase,
  * centre, filter id + 1 and two constants.  Returns the length (snprintf). */

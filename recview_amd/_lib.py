@@ -76,6 +76,7 @@ EXPORTS = [
     "rvUnetLossGrad", "rvUnetTrainStep", "rvUnetDestroy", "rvConv3x3Bf16", "rvConv3x3WgradBf16",
     "RecviewOutputStage", "RecviewImageDescription", "RecviewWriteTif16", "RecviewSinogramDevice",
     "RecviewDeviceAlloc", "RecviewDeviceFree", "RecviewCopyToHost", "RecviewCopyToDevice",
+    "RecviewOutputStageNoAlloc",
 ]
 
 
@@ -137,6 +138,7 @@ def _declare(L):
                                         C.c_int, C.c_int, C.c_int, vp, vp]
     L.RecviewOutputStage.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_int, vp, C.POINTER(C.c_double),
                                      C.POINTER(C.c_double), vp]
+    L.RecviewOutputStageNoAlloc.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_int, vp, vp, vp]
     L.RecviewImageDescription.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                           C.c_double, C.c_int]
     L.RecviewWriteTif16.argtypes = [C.c_char_p, ip, C.c_int, C.c_int, C.c_char_p, C.c_char_p]

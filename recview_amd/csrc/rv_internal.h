@@ -99,6 +99,21 @@ bool sinogram_jobs(const SinoWalk& w, std::vector<SinoJob>& jobs);
 // its last two hold (min, max) afterwards.
 void launch_output_stage(const void* img, int is_float, int N, int irad2, uint16_t* out, void* scratch,
                          hipStream_t s);
+// The same three launches with the 1024 partial minima and 1024 partial maxima at
+// part (2048 values of the image type) and the (min, max) result at range.
+void launch_output_stage_parts(const void* img, int is_float, int N, int irad2, uint16_t* out, void* part,
+                               void* range, hipStream_t s);
+constexpr int kOutputParts = 1024;
+// Squared mask radius of gazoDoc.cpp:4008-4010: r = zoom * center / binning,
+// min(r, N - r) truncated, for the zoom and binning of iInterpolation.
+inline int output_irad2(int N, double center, int iInterpolation)
+{
+    const int ibin = iInterpolation == 0 ? 4 : (iInterpolation == 1 ? 2 : 1);
+    const int ipintp = iInterpolation > 2 ? 1 << (iInterpolation - 2) : 1;
+    const double r = ipintp * center / ibin;
+    const int ir = r < (N - r) ? (int)r : (int)(N - r);
+    return ir * ir;
+}
 
 // ------------------------------------------------------- back-projection
 // BP_LINEAR2: BP_LINEAR over two slices of identical geometry per work item;

@@ -56,12 +56,7 @@ int RecviewOutputStage(const void* d_img, int is_float, int N, double center, in
         rv::set_message("RecviewOutputStage: bad arguments");
         return -1;
     }
-    const int ibin = iInterpolation == 0 ? 4 : (iInterpolation == 1 ? 2 : 1);
-    const int ipintp = iInterpolation > 2 ? 1 << (iInterpolation - 2) : 1;
-    // gazoDoc.cpp:4008-4010
-    const double r = ipintp * center / ibin;
-    int irad2 = r < (N - r) ? (int)r : (int)(N - r);
-    irad2 *= irad2;
+    const int irad2 = rv::output_irad2(N, center, iInterpolation);     // gazoDoc.cpp:4008-4010
     hipStream_t s = (hipStream_t)stream;
     void* scratch = nullptr;
     if (!rv::hip_ok(hipMalloc(&scratch, 2050 * 4), "hipMalloc")) return -1;

@@ -3,10 +3,13 @@
 // every pixel inside the circle of radius min(z*c/bin, N - z*c/bin) becomes
 // (int)((float)(v - imin) * 65534 / (imax - imin)) and everything outside 0.
 // Three launches: per-block min/max, one-block final reduction, normalise.
-#include "rv_internal.h"
+#include "recview_hip.h"
+#include "rv_common.h"
 
 #include <climits>
 #include <cfloat>
+#include <map>
+#include <mutex>
 
 namespace rv {
 
@@ -92,26 +95,84 @@ __global__ void normalise_kernel(const T* img, int N, int irad2, const T* range,
 }
 
 template <typename T>
-static void run(const T* img, int N, int irad2, uint16_t* out, T* scratch, hipStream_t s)
+static void run(const T* img, int N, int irad2, uint16_t* out, T* part, T* range, hipStream_t s)
 {
     const long long n = (long long)N * N;
-    const int nparts = (int)std::min<long long>(1024, (n + 255) / 256);
-    T* pmin = scratch;
-    T* pmax = scratch + 1024;
-    T* range = scratch + 2048;
+    const int nparts = (int)std::min<long long>(kOutputParts, (n + 255) / 256);
+    T* pmin = part;
+    T* pmax = part + kOutputParts;
     hipLaunchKernelGGL(minmax_kernel<T>, dim3(nparts), dim3(256), 0, s, img, n, pmin, pmax);
     hipLaunchKernelGGL(minmax_final_kernel<T>, dim3(1), dim3(256), 0, s, pmin, pmax, nparts, range);
     hipLaunchKernelGGL(normalise_kernel<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, img, N,
                        irad2, range, out);
 }
 
+void launch_output_stage_parts(const void* img, int is_float, int N, int irad2, uint16_t* out, void* part,
+                               void* range, hipStream_t s)
+{
+    if (is_float)
+        run<float>((const float*)img, N, irad2, out, (float*)part, (float*)range, s);
+    else
+        run<int>((const int*)img, N, irad2, out, (int*)part, (int*)range, s);
+}
+
 void launch_output_stage(const void* img, int is_float, int N, int irad2, uint16_t* out, void* scratch,
                          hipStream_t s)
 {
-    if (is_float)
-        run<float>((const float*)img, N, irad2, out, (float*)scratch, s);
-    else
-        run<int>((const int*)img, N, irad2, out, (int*)scratch, s);
+    launch_output_stage_parts(img, is_float, N, irad2, out, scratch, (char*)scratch + 2 * kOutputParts * 4, s);
 }
 
 }  // namespace rv
+
+// ------------------------------------------------ allocation-free output stage
+// The partial minima and maxima of RecviewOutputStageNoAlloc live in this
+// array: 1024 + 1024 four-byte values (8 KiB) on every GPU the library runs on,
+// held until the process ends.  The partials are capped at 1024 whatever the
+// image side, so the array serves every image the entry point accepts.  Host
+// code never passes the array's name to a kernel: it passes the device address
+// hipGetSymbolAddress returns for the current device, kept per device.
+static __device__ __align__(256) unsigned int scratch[2 * rv::kOutputParts];
+
+namespace {
+
+std::mutex g_scratch_mu;
+std::map<int, void*> g_scratch_ptr;
+
+// The device address of `scratch` on the current device (looked up on the first
+// call on that device), or nullptr with the message set.
+void* scratch_on_current_device()
+{
+    int dev = 0;
+    if (!rv::hip_ok(hipGetDevice(&dev), "hipGetDevice")) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch_ptr.find(dev);
+    if (it != g_scratch_ptr.end()) return it->second;
+    void* ptr = nullptr;
+    if (!rv::hip_ok(hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)"))
+        return nullptr;
+    g_scratch_ptr[dev] = ptr;
+    return ptr;
+}
+
+}  // namespace
+
+extern "C" int RecviewOutputStageNoAlloc(const void* d_img, int is_float, int N, double center,
+                                         int iInterpolation, unsigned short* d_out, void* d_range,
+                                         rv_stream_t stream)
+{
+    if (!d_img || !d_out || !d_range || N <= 0 || iInterpolation < 0 || iInterpolation > 5) {
+        rv::set_message("RecviewOutputStageNoAlloc: bad arguments");
+        return -1;
+    }
+    if (N > RV_OUTPUT_NOALLOC_MAX_SIDE) {
+        rv::set_message("RecviewOutputStageNoAlloc: image side N exceeds 65536");
+        return -1;
+    }
+    void* part = scratch_on_current_device();
+    if (!part) return -1;
+    // every partial the final reduction reads is written by the first launch of
+    // this call, so nothing needs clearing
+    rv::launch_output_stage_parts(d_img, is_float, N, rv::output_irad2(N, center, iInterpolation), d_out,
+                                  part, d_range, (hipStream_t)stream);
+    return rv::hip_ok(hipGetLastError(), "output stage") ? 0 : -1;
+}

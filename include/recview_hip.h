@@ -253,7 +253,7 @@ typedef struct {
     double pix_zoom;
     double x0, y0;
     /* Tile configuration of this slice: -1 = the plan's (RvOptions.tile);
-     * 0..3 = 64x64, 64x128, 32x32, 128x64 (chosen per shape by an autotuner).
+     * 0..4 = 64x64, 64x128, 32x32, 128x64, 32x64 (chosen per shape by an autotuner).
      * Slices with different tiles run as separate launches of one execution. */
     int tile;
     /* 1: the input is a 360-degree scan of P frames with the axis off centre
@@ -287,8 +287,9 @@ typedef struct {
                           * each pixel's coordinate and LDS reads: 0 (default) = two per work item
                           * with FP32 pairs in LDS (results identical to one per item), eight with
                           * FP16 pairs (lds_half: one FP16 dot product per update and one
-                          * power-of-two scale per work item); or 1, 2, 4, 8 (4 and 8 only with FP16
-                          * pairs; 8 on the 64x64 and 32x32 tiles, else 4) */
+                          * power-of-two scale per work item); or 1, 2, 4, 8, 16 (4, 8 and 16 only
+                          * with FP16 pairs; 8 on the 64x64, 32x32 and 32x64 tiles, 16 on the 32x32
+                          * and 32x64 tiles, else fewer) */
 } RvOptions;
 
 typedef struct rv_plan_opaque* rv_plan_t;

@@ -57,13 +57,22 @@ namespace rv {
 
 typedef _Float16 half2v __attribute__((ext_vector_type(2)));
 
+// smallest integer >= sqrt(v)
+constexpr int isqrt_ceil(int v)
+{
+    int r = 0;
+    while (r * r < v) r++;
+    return r;
+}
+
 template <int PX, int PY, int NW = 4>
 struct TileCfg {
     static constexpr int WX = 8 * PX, WY = 8 * PY;   // wave block
     static constexpr int TW = 2 * WX, TH = (NW / 2) * WY;   // 2 x NW/2 waves
     static constexpr int T = 64 * NW;
-    // detector footprint of a tile: (TW-1)|cos| + (TH-1)|sin| + 3 margin samples
-    static constexpr int SEG = ((int)(1.0 + 1.41421356 * (TW > TH ? TW : TH)) + 3 + 3) / 4 * 4;
+    // detector footprint of a tile: (TW-1)|cos| + (TH-1)|sin| <= the tile diagonal D;
+    // with s0 = floor(umin) - 1 every index used is <= D + 2, so SEG >= D + 3
+    static constexpr int SEG = (isqrt_ceil((TW - 1) * (TW - 1) + (TH - 1) * (TH - 1)) + 7) / 4 * 4;
 };
 
 template <int MODE>
@@ -73,7 +82,7 @@ struct AccT<BP_LUT_I32> { using type = int; };
 
 constexpr bool is_half_multi(int mode)
 {
-    return mode == BP_LINEAR_H2 || mode == BP_LINEAR_H4 || mode == BP_LINEAR_H8;
+    return mode == BP_LINEAR_H2 || mode == BP_LINEAR_H4 || mode == BP_LINEAR_H8 || mode == BP_LINEAR_H16;
 }
 
 constexpr bool is_linear(int mode)
@@ -260,13 +269,25 @@ __global__ void __launch_bounds__(64 * NW, NW == 8 ? 4 : 1) bp_tiled_kernel(BpLa
                 // a_j = p_k - j d_k as linear_pair forms it (d = 0 at the last sample)
                 float4* quads = (float4*)data;
                 const float4* packed = (const float4*)S.rows;
-                for (int idx = tid; idx < chunk * SEG; idx += C::T) {
-                    const int a = idx / SEG, j = idx - a * SEG;
-                    const int row = rws[a], k = s0s[a] + j;
+                auto stage = [&](int a, int j, int row, int s0) {
+                    const int k = s0 + j;
                     float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
                     if (row >= 0 && k >= 0 && k < N) v = packed[(long long)row * S.row_stride + k];
                     const float fj = -(float)j;
-                    quads[idx] = make_float4(fmaf(fj, v.y, v.x), v.y, fmaf(fj, v.w, v.z), v.w);
+                    quads[a * SEG + j] = make_float4(fmaf(fj, v.y, v.x), v.y, fmaf(fj, v.w, v.z), v.w);
+                };
+                if (C::T % chunk == 0) {
+                    // threads of one angle: its row and segment origin read once per
+                    // thread (C2, FP32 pairs: back-projection 40.85 -> 39.87 ms)
+                    const int tpa = C::T / chunk;
+                    const int a = tid / tpa, j0 = tid - a * tpa;
+                    const int row = rws[a], s0 = s0s[a];
+                    for (int j = j0; j < SEG; j += tpa) stage(a, j, row, s0);
+                } else {
+                    for (int idx = tid; idx < chunk * SEG; idx += C::T) {
+                        const int a = idx / SEG, j = idx - a * SEG;
+                        stage(a, j, rws[a], s0s[a]);
+                    }
                 }
             } else if (MODE == BP_LINEAR_H) {
                 __half2* pairs = (__half2*)data;
@@ -292,15 +313,17 @@ __global__ void __launch_bounds__(64 * NW, NW == 8 ? 4 : 1) bp_tiled_kernel(BpLa
                     const int row = rws[a], k = s0s[a] + j;
                     const bool in = row >= 0 && k >= 0 && k < N;
                     const long long off = (long long)row * S.row_stride + k;
-                    if (NS == 8) {
-                        // two 16-byte planes per angle: slices 0-3, then 4-7
-                        uint4 v0 = make_uint4(0u, 0u, 0u, 0u), v1 = v0;
-                        if (in) {
-                            v0 = ((const uint4*)S.rows)[2 * off];
-                            v1 = ((const uint4*)S.rows)[2 * off + 1];
+                    if (NS >= 8) {
+                        // NS / 4 16-byte planes per angle: slices 0-3, then 4-7, ...
+                        constexpr int NPL = NS >= 8 ? NS / 4 : 1;
+                        uint4 v[NPL];
+#pragma unroll
+                        for (int pl = 0; pl < NPL; pl++) {
+                            v[pl] = make_uint4(0u, 0u, 0u, 0u);
+                            if (in) v[pl] = ((const uint4*)S.rows)[NPL * off + pl];
                         }
-                        ((uint4*)data)[(2 * a) * SEG + j] = v0;
-                        ((uint4*)data)[(2 * a + 1) * SEG + j] = v1;
+#pragma unroll
+                        for (int pl = 0; pl < NPL; pl++) ((uint4*)data)[(NPL * a + pl) * SEG + j] = v[pl];
                     } else if (NS == 4) {
                         uint4 v = make_uint4(0u, 0u, 0u, 0u);
                         if (in) v = ((const uint4*)S.rows)[off];
@@ -388,9 +411,9 @@ __global__ void __launch_bounds__(64 * NW, NW == 8 ? 4 : 1) bp_tiled_kernel(BpLa
                     }
                 }
             } else if (is_half_multi(MODE)) {
-                // FP16 pairs of 2 / 4 / 8 slices per sample (8: two 16-byte planes).
+                // FP16 pairs of 2 / 4 / 8 / 16 slices per sample (8, 16: two / four 16-byte planes).
                 using QT = typename std::conditional<NS >= 4, uint4, uint2>::type;
-                constexpr int NPL = NS >= 8 ? 2 : 1;
+                constexpr int NPL = NS >= 8 ? NS / 4 : 1;
                 const QT* ents = (const QT*)data;
                 // entries of one pixel row and their FP16 weights (1, fract(u)),
                 // rounded toward zero
@@ -441,7 +464,7 @@ __global__ void __launch_bounds__(64 * NW, NW == 8 ? 4 : 1) bp_tiled_kernel(BpLa
                     } else if constexpr (NS >= 4) {
                         // one row at a time (four slices: 118 VGPRs, four waves per SIMD;
                         // the whole angle at once needs 151 and allows three, measured 6%
-                        // slower)
+                        // slower; sixteen slices: four planes of each pixel of the row)
 #pragma unroll
                         for (int j = 0; j < PY; j++) {
                             QT q[PX][NPL];
@@ -587,7 +610,7 @@ __global__ void __launch_bounds__(256) pack_scale_kernel(const PackJob* jobs)
     }
 }
 
-// KIND 0: FP32 pairs of two slices; KIND 2 / 4 / 8: FP16 pairs of that many
+// KIND 0: FP32 pairs of two slices; KIND 2 / 4 / 8 / 16: FP16 pairs of that many
 // slices.  Blocks stride over rows, threads over samples (no index division;
 // each row's scales read once); every slice's loads are issued together.
 template <int KIND, int STORE>
@@ -631,9 +654,12 @@ __global__ void __launch_bounds__(256) pack_kernel(const PackJob* jobs)
                     const float dv = last ? 0.f : __half2float(__float2half_rn(p1[e] * mul[e])) - pv;
                     w[e] = __builtin_bit_cast(uint32_t, __halves2half2(__float2half_rn(pv), __float2half_rn(dv)));
                 }
-                if (NS == 8) {
-                    ((uint4*)J.out)[2 * i] = make_uint4(w[0], w[1], w[2], w[3]);
-                    ((uint4*)J.out)[2 * i + 1] = make_uint4(w[4 % NS], w[5 % NS], w[6 % NS], w[7 % NS]);
+                if (NS >= 8) {
+                    constexpr int NPL = NS >= 8 ? NS / 4 : 1;
+#pragma unroll
+                    for (int pl = 0; pl < NPL; pl++)
+                        ((uint4*)J.out)[NPL * i + pl] = make_uint4(w[(4 * pl) % NS], w[(4 * pl + 1) % NS],
+                                                                   w[(4 * pl + 2) % NS], w[(4 * pl + 3) % NS]);
                 } else if (NS == 4) {
                     ((uint4*)J.out)[i] = make_uint4(w[0], w[1], w[2 % NS], w[3 % NS]);
                 } else {
@@ -666,16 +692,19 @@ void launch_pack(const PackJob* d_jobs, int njobs, int kind, int store, int max_
     if (kind == 0) launch_pack_kind<0>(d_jobs, njobs, store, max_rows, stream);
     else if (kind == 2) launch_pack_kind<2>(d_jobs, njobs, store, max_rows, stream);
     else if (kind == 4) launch_pack_kind<4>(d_jobs, njobs, store, max_rows, stream);
-    else launch_pack_kind<8>(d_jobs, njobs, store, max_rows, stream);
+    else if (kind == 8) launch_pack_kind<8>(d_jobs, njobs, store, max_rows, stream);
+    else launch_pack_kind<16>(d_jobs, njobs, store, max_rows, stream);
 }
 
-// Tile configurations: id -> (PX, PY).  0: 64x64, 1: 64x128 (W x H), 2: 32x32, 3: 128x64.
+// Tile configurations: id -> (PX, PY).  0: 64x64, 1: 64x128 (W x H), 2: 32x32,
+// 3: 128x64, 4: 32x64.
 int bp_tile_side(int tile)
 {
     switch (tile) {
     case 1: return 64;
     case 2: return 32;
     case 3: return 128;
+    case 4: return 32;
     default: return 64;
     }
 }
@@ -686,6 +715,7 @@ int bp_tile_h(int tile)
     case 1: return 128;
     case 2: return 32;
     case 3: return 64;
+    case 4: return 64;
     default: return 64;
     }
 }
@@ -696,9 +726,12 @@ static int tile_seg(int tile)
     case 1: return TileCfg<4, 8>::SEG;
     case 2: return TileCfg<2, 2>::SEG;
     case 3: return TileCfg<8, 4>::SEG;
+    case 4: return TileCfg<2, 4>::SEG;
     default: return TileCfg<4, 4>::SEG;
     }
 }
+static_assert(TileCfg<2, 4>::SEG == TileCfg<2, 2, 8>::SEG, "32x64 tiles stage the same segment");
+static_assert(TileCfg<4, 4>::SEG == TileCfg<4, 2, 8>::SEG, "64x64 tiles stage the same segment");
 
 int bp_default_chunk(int mode, int tile)
 {
@@ -708,6 +741,7 @@ int bp_default_chunk(int mode, int tile)
     if (mode == BP_LINEAR_H2) return (tile == 2) ? 64 : 32;
     if (mode == BP_LINEAR_H4) return (tile == 2) ? 32 : 16;
     if (mode == BP_LINEAR_H8) return (tile == 2) ? 32 : 24;
+    if (mode == BP_LINEAR_H16) return 16;
     return (tile == 2) ? 64 : 32;
 }
 
@@ -720,21 +754,32 @@ size_t bp_lds_bytes(int mode, int tile, int chunk)
     return head + (size_t)chunk * entry_bytes(mode, seg);
 }
 
-// Eight slices per item exist for tiles 0 and 2 only (the larger tiles would
-// need 256 accumulators per thread); the plan forms no such items otherwise.
-// On the 64x64 tile they run eight waves per workgroup (8 pixels x 8 slices per
-// lane) so that four waves fit per SIMD.
+// Eight slices per item exist for tiles 0, 2 and 4 (the larger tiles would need
+// 256 accumulators per thread), sixteen for tiles 2 and 4; the plan forms no
+// such items otherwise.  On the 64x64 tile eight slices run eight waves per
+// workgroup (8 pixels x 8 slices per lane), sixteen slices on the 32x64 tile as
+// well (4 pixels x 16 slices), so that four waves fit per SIMD.
+int bp_threads(int mode, int tile)
+{
+    return ((mode == BP_LINEAR_H8 && tile == 0) || (mode == BP_LINEAR_H16 && tile == 4)) ? 512 : 256;
+}
+
 template <int MODE>
 static const void* kernel_ptr(int tile)
 {
     if constexpr (MODE == BP_LINEAR_H8) {
         if (tile == 2) return (const void*)&bp_tiled_kernel<MODE, 2, 2>;
+        if (tile == 4) return (const void*)&bp_tiled_kernel<MODE, 2, 4>;
         return tile == 0 ? (const void*)&bp_tiled_kernel<MODE, 4, 2, 8> : nullptr;
+    } else if constexpr (MODE == BP_LINEAR_H16) {
+        if (tile == 2) return (const void*)&bp_tiled_kernel<MODE, 2, 2>;
+        return tile == 4 ? (const void*)&bp_tiled_kernel<MODE, 2, 2, 8> : nullptr;
     } else {
         switch (tile) {
         case 1: return (const void*)&bp_tiled_kernel<MODE, 4, 8>;
         case 2: return (const void*)&bp_tiled_kernel<MODE, 2, 2>;
         case 3: return (const void*)&bp_tiled_kernel<MODE, 8, 4>;
+        case 4: return (const void*)&bp_tiled_kernel<MODE, 2, 4>;
         default: return (const void*)&bp_tiled_kernel<MODE, 4, 4>;
         }
     }
@@ -748,6 +793,7 @@ static const void* kernel_for(int mode, int tile)
     if (mode == BP_LINEAR_H2) return kernel_ptr<BP_LINEAR_H2>(tile);
     if (mode == BP_LINEAR_H4) return kernel_ptr<BP_LINEAR_H4>(tile);
     if (mode == BP_LINEAR_H8) return kernel_ptr<BP_LINEAR_H8>(tile);
+    if (mode == BP_LINEAR_H16) return kernel_ptr<BP_LINEAR_H16>(tile);
     if (mode == BP_LUT_F32) return kernel_ptr<BP_LUT_F32>(tile);
     return kernel_ptr<BP_LUT_I32>(tile);
 }
@@ -772,7 +818,7 @@ int bp_blocks_per_cu(int mode, int tile, int chunk)
     const size_t lds = bp_lds_bytes(mode, tile, chunk);
     allow_lds(f, lds);
     int nb = 0;
-    const int threads = (mode == BP_LINEAR_H8 && tile == 0) ? 512 : 256;
+    const int threads = bp_threads(mode, tile);
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, threads, lds) != hipSuccess || nb < 1) nb = 1;
     return nb;
 }
@@ -784,13 +830,19 @@ static void launch_mode(const BpLaunch& L, int grid, hipStream_t stream)
     allow_lds(kernel_ptr<MODE>(L.tile), lds);
     if constexpr (MODE == BP_LINEAR_H8) {
         if (L.tile == 2) hipLaunchKernelGGL((bp_tiled_kernel<MODE, 2, 2>), dim3(grid), dim3(256), lds, stream, L);
+        else if (L.tile == 4) hipLaunchKernelGGL((bp_tiled_kernel<MODE, 2, 4>), dim3(grid), dim3(256), lds, stream, L);
         else if (L.tile == 0)
             hipLaunchKernelGGL((bp_tiled_kernel<MODE, 4, 2, 8>), dim3(grid), dim3(512), lds, stream, L);
+    } else if constexpr (MODE == BP_LINEAR_H16) {
+        if (L.tile == 2) hipLaunchKernelGGL((bp_tiled_kernel<MODE, 2, 2>), dim3(grid), dim3(256), lds, stream, L);
+        else if (L.tile == 4)
+            hipLaunchKernelGGL((bp_tiled_kernel<MODE, 2, 2, 8>), dim3(grid), dim3(512), lds, stream, L);
     } else {
         switch (L.tile) {
         case 1: hipLaunchKernelGGL((bp_tiled_kernel<MODE, 4, 8>), dim3(grid), dim3(256), lds, stream, L); break;
         case 2: hipLaunchKernelGGL((bp_tiled_kernel<MODE, 2, 2>), dim3(grid), dim3(256), lds, stream, L); break;
         case 3: hipLaunchKernelGGL((bp_tiled_kernel<MODE, 8, 4>), dim3(grid), dim3(256), lds, stream, L); break;
+        case 4: hipLaunchKernelGGL((bp_tiled_kernel<MODE, 2, 4>), dim3(grid), dim3(256), lds, stream, L); break;
         default: hipLaunchKernelGGL((bp_tiled_kernel<MODE, 4, 4>), dim3(grid), dim3(256), lds, stream, L); break;
         }
     }
@@ -815,6 +867,7 @@ void launch_bp(const BpLaunch& L, int grid, hipStream_t stream)
     else if (L.mode == BP_LINEAR_H2) launch_mode<BP_LINEAR_H2>(L, grid, stream);
     else if (L.mode == BP_LINEAR_H4) launch_mode<BP_LINEAR_H4>(L, grid, stream);
     else if (L.mode == BP_LINEAR_H8) launch_mode<BP_LINEAR_H8>(L, grid, stream);
+    else if (L.mode == BP_LINEAR_H16) launch_mode<BP_LINEAR_H16>(L, grid, stream);
     else if (L.mode == BP_LUT_F32) launch_mode<BP_LUT_F32>(L, grid, stream);
     else launch_mode<BP_LUT_I32>(L, grid, stream);
 }

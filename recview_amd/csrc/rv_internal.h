@@ -117,17 +117,17 @@ inline int output_irad2(int N, double center, int iInterpolation)
 
 // ------------------------------------------------------- back-projection
 // BP_LINEAR2: BP_LINEAR over two slices of identical geometry per work item;
-// BP_LINEAR_H2 / H4 / H8: BP_LINEAR_H over two / four / eight (the others in
-// BpSlice::rows_x / scales_x / out_x).
+// BP_LINEAR_H2 / H4 / H8 / H16: BP_LINEAR_H over two / four / eight / sixteen
+// (the others in BpSlice::rows_x / scales_x / out_x).
 enum BpMode {
     BP_LINEAR = 0, BP_LUT_F32 = 1, BP_LUT_I32 = 2, BP_LINEAR_H = 3, BP_LINEAR2 = 4,
-    BP_LINEAR_H2 = 5, BP_LINEAR_H4 = 6, BP_LINEAR_H8 = 7
+    BP_LINEAR_H2 = 5, BP_LINEAR_H4 = 6, BP_LINEAR_H8 = 7, BP_LINEAR_H16 = 8
 };
 constexpr int bp_slices_per_item(int mode)
 {
-    return (mode == 4 || mode == 5) ? 2 : mode == 6 ? 4 : mode == 7 ? 8 : 1;
+    return (mode == 4 || mode == 5) ? 2 : mode == 6 ? 4 : mode == 7 ? 8 : mode == 8 ? 16 : 1;
 }
-constexpr int BP_MAX_SLICES_PER_ITEM = 8;
+constexpr int BP_MAX_SLICES_PER_ITEM = 16;
 
 // Per-angle table entry.  LINEAR: detector coordinate of pixel (ix, iy) of the
 // output grid is u = u0 + ix*dux + iy*duy (all double).  LUT16: the reference
@@ -193,7 +193,7 @@ void launch_ptr_patch(const PtrPatch& P, hipStream_t stream);
 // 16-byte load:
 //   BP_LINEAR2      float4 (p_k, d_k, p'_k, d'_k), d_k = p_{k+1} - p_k (0 at the
 //                   last sample);
-//   BP_LINEAR_H2/H4/H8 two / four / eight FP16 pairs (p_k, d_k), values multiplied by
+//   BP_LINEAR_H2/H4/H8/H16 two / four / eight / sixteen FP16 pairs (p_k, d_k), values multiplied by
 //                   row scale / item scale, where the item scale is the largest
 //                   row scale of the slices over their angle list (a power of two).
 struct PackJob {
@@ -201,7 +201,7 @@ struct PackJob {
     const float* scales[BP_MAX_SLICES_PER_ITEM];   // per-row scales (16-bit stores), else nullptr
     const AngleEntry* ang;  // FP16 pairs: the angle list whose rows set the item scale
     int nang;
-    int ns;                 // slices: 2, or 4 / 8 (FP16 pairs only)
+    int ns;                 // slices: 2, or 4 / 8 / 16 (FP16 pairs only)
     int half;               // 1: FP16 pairs
     int store;              // 0 f32, 1 f16, 2 bf16
     int nrows, N;           // rows of N samples, row stride N
@@ -209,7 +209,7 @@ struct PackJob {
     float* gscale;          // FP16 pairs: the item scale, written by the scale pass
 };
 // Item scales of FP16-pair jobs (jobs of other kinds are skipped), then the
-// packing of njobs jobs of one kind (0: FP32 pairs; 2 / 4 / 8: FP16 pairs of
+// packing of njobs jobs of one kind (0: FP32 pairs; 2 / 4 / 8 / 16: FP16 pairs of
 // that many slices) and one store, in stream order.
 void launch_pack_scales(const PackJob* d_jobs, int njobs, hipStream_t stream);
 void launch_pack(const PackJob* d_jobs, int njobs, int kind, int store, int max_rows, hipStream_t stream);
@@ -227,6 +227,7 @@ struct StitchJob {
     int is_float;
 };
 void launch_stitch(const StitchJob& J, hipStream_t stream);
+int bp_threads(int mode, int tile);         // workgroup size of a mode's kernel on a tile
 int bp_tile_side(int tile);                 // tile width in pixels for a config id
 int bp_tile_h(int tile);                    // tile height in pixels
 int bp_default_chunk(int mode, int tile);   // angles per chunk

@@ -190,6 +190,12 @@ rv::FilterJob filter_job(rv_plan_t p, size_t s)
         J.pair = 1;
         J.filt = p->d_filts + p->filt_total + S.filt_off;
         J.tw = rv::device_twiddles_full(p->dev, g.ndimp);
+    } else if (p->pair_filter) {
+        // two-pass transform (CFft's network) with exp(-2 pi i k / n) rounded from
+        // double instead of the reference's float-angle twiddles: the same
+        // operator, FP32-accurate at 2^15..2^20 points (exact_filter keeps the
+        // reference's table and its rounding)
+        J.tw = rv::device_twiddles_full(p->dev, g.ndimp);
     }
     J.out_stride = g.N;
     J.store = p->store;
@@ -304,7 +310,7 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
     if (p->mode == rv::BP_LINEAR && p->store != 0 && opt->lds_half) p->mode = rv::BP_LINEAR_H;
     const bool linear = (p->mode == rv::BP_LINEAR || p->mode == rv::BP_LINEAR_H);
     p->pair_filter = linear && !opt->exact_filter;
-    const int plan_tile = (opt->tile >= 0 && opt->tile <= 3) ? opt->tile : 0;
+    const int plan_tile = (opt->tile >= 0 && opt->tile <= 4) ? opt->tile : 0;
 
     size_t rows_elems = 0, nang_total = 0, filt_total = 0, stitch_elems = 0, pair_total = 0;
     std::vector<int> all_pairs;
@@ -357,7 +363,7 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
         rows_elems += (size_t)S.nang * S.g.N;
         nang_total += (size_t)S.nang;
         filt_total += (size_t)S.g.n;
-        S.tile = (S.spec.tile >= 0 && S.spec.tile <= 3) ? S.spec.tile : plan_tile;
+        S.tile = (S.spec.tile >= 0 && S.spec.tile <= 4) ? S.spec.tile : plan_tile;
         S.tiles_x = (S.out_n + rv::bp_tile_side(S.tile) - 1) / rv::bp_tile_side(S.tile);
         S.tiles_y = (S.out_n + rv::bp_tile_h(S.tile) - 1) / rv::bp_tile_h(S.tile);
     }
@@ -417,12 +423,12 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
     // FP16 pairs in LDS (classes of equal geometry, in slice order).
     std::vector<std::vector<int>> classes;
     if (opt->slices_per_item != 0 && opt->slices_per_item != 1 && opt->slices_per_item != 2 &&
-        opt->slices_per_item != 4 && opt->slices_per_item != 8)
-        return fail(p, "slices_per_item must be 0, 1, 2, 4 or 8");
+        opt->slices_per_item != 4 && opt->slices_per_item != 8 && opt->slices_per_item != 16)
+        return fail(p, "slices_per_item must be 0, 1, 2, 4, 8 or 16");
     // default: two FP32 slices or eight FP16-pair slices per item (packed rows;
     // measured for 64 slices at 2048^2 x 1800: FP32 pairs 40.3 ms; FP16 pairs
     // 46.0 / 30.9 / 28.3 ms with two / four / eight slices per item)
-    const int max_per_item = (p->mode == rv::BP_LINEAR_H) ? 8 : 2;
+    const int max_per_item = (p->mode == rv::BP_LINEAR_H) ? 16 : 2;
     const int per_item = std::min(opt->slices_per_item == 0 ? 8 : opt->slices_per_item, max_per_item);
     if ((p->mode == rv::BP_LINEAR || p->mode == rv::BP_LINEAR_H) && per_item > 1) {
         auto same = [&](int a, int b) {
@@ -468,11 +474,14 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
         while (i < cls.size()) {
             const size_t left = cls.size() - i;
             const int tile = p->sl[cls[i]].tile;
-            const bool tile8 = tile == 0 || tile == 2;       // eight per item: 64^2 and 32^2 tiles
-            const int k = (per_item >= 8 && left >= 8 && tile8) ? 8
+            const bool tile8 = tile == 0 || tile == 2 || tile == 4;    // eight per item: 64^2, 32^2, 32x64
+            const bool tile16 = tile == 2 || tile == 4;                // sixteen: 32^2 and 32x64
+            const int k = (per_item >= 16 && left >= 16 && tile16) ? 16
+                        : (per_item >= 8 && left >= 8 && tile8) ? 8
                         : (per_item >= 4 && left >= 4) ? 4 : (per_item >= 2 && left >= 2) ? 2 : 1;
             const bool half = (p->mode == rv::BP_LINEAR_H);
-            const int mode = k == 8 ? (int)rv::BP_LINEAR_H8
+            const int mode = k == 16 ? (int)rv::BP_LINEAR_H16
+                           : k == 8 ? (int)rv::BP_LINEAR_H8
                            : k == 4 ? (int)rv::BP_LINEAR_H4
                            : k == 2 ? (half ? (int)rv::BP_LINEAR_H2 : (int)rv::BP_LINEAR2)
                                     : p->mode;
@@ -522,7 +531,7 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
     // packed rows of the multi-slice work items: FP32 pairs 16 B, FP16 pairs 8 B
     // (two slices) or 16 B (four slices) per sample
     auto packed_size = [](int mode) -> size_t {
-        return mode == rv::BP_LINEAR_H8 ? 32
+        return mode == rv::BP_LINEAR_H16 ? 64 : mode == rv::BP_LINEAR_H8 ? 32
              : mode == rv::BP_LINEAR2 || mode == rv::BP_LINEAR_H4 ? 16 : mode == rv::BP_LINEAR_H2 ? 8 : 0;
     };
     size_t packed_bytes = 0, n_half = 0;
@@ -655,7 +664,7 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
     int max_n = 0;
     for (const auto& S : p->sl) {
         if (!rv::device_twiddles(device, S.g.ndimp) ||
-            (p->pair_filter && S.g.ndimp <= rv::kMaxLdsFftLog2 && !rv::device_twiddles_full(device, S.g.ndimp)))
+            (p->pair_filter && !rv::device_twiddles_full(device, S.g.ndimp)))
             return fail(p, "twiddle table allocation failed");
         if (S.g.ndimp <= rv::kMaxLdsFftLog2) max_n = std::max(max_n, S.g.n);
     }

@@ -51,7 +51,7 @@ def test_reconst_host_c2_bit_exact(formula, rule):
     ref = O.recon_rint(rows, fdeg, g, O.set_filter(g.ndim, 0, 0.5, 6.0), formula=formula, rule=rule)
     diff = np.abs(res["image"].astype(np.int64) - ref)
     print(f"  max |GPU - R-int| = {diff.max()}, image range {ref.min()}..{ref.max()}")
-    assert np.abs(ref).max() > 10 ** 6
+    assert np.abs(ref).max() > 10 ** 5
     assert np.array_equal(res["image"], ref)
 
 

@@ -234,6 +234,40 @@ def test_plan_half_pairs_grouped(store, rms_tol):
             assert rms <= rms_tol and diff <= 0.5 * rms_tol, (k, tile, i, rms, diff)
 
 
+@pytest.mark.parametrize("store,rms_tol", [("f16", 1e-4), ("bf16", 2e-3)])
+def test_plan_half_sixteen_per_item(store, rms_tol):
+    """Sixteen FP16-pair slices per work item (32x64 tiles, eight waves, and 32x32
+    tiles): 19 slices of one geometry form a sixteen-slice item, a pair and a
+    single.  Every image matches the FP64 oracle to the store's tolerance and the
+    one slice per item path to a fraction of it, also through graph replay."""
+    print(f"case store={store}", flush=True)
+    W, P = 200, 150
+    specs, sinos, refs = [], [], []
+    for i in range(19):
+        s, fdeg, c = _sino(W, P, 1900 + i, photons=1e4)
+        specs.append(fbp.SliceSpec(W, fdeg, c))
+        sinos.append(torch.from_numpy(s))
+        g = O.geometry(W, 2, c)
+        refs.append(O.recon_of64(s, fdeg, g, O.set_filter(g.ndim, 0)))
+    m = _circle(W)
+    _, single = _run(specs, sinos, store=store, lds_half=True, slices_per_item=1)
+    for tile in (4, 2):
+        plan = fbp.FbpPlan(specs, store=store, lds_half=True, slices_per_item=16, tile=tile, persistent=True)
+        outs = [torch.empty(W, W, device="cuda") for _ in specs]
+        for graph in (False, True, True):
+            for o in outs:
+                o.fill_(float("nan"))
+            plan.execute([x.cuda() for x in sinos], outs, graph=graph)
+            torch.cuda.synchronize()
+            for i, ref in enumerate(refs):
+                got = outs[i].cpu().numpy()
+                rms, _ = _errors(got, ref, m)
+                diff, _ = _errors(got, single[i].astype(np.float64), m)
+                assert rms <= rms_tol and diff <= 0.5 * rms_tol, (tile, graph, i, rms, diff)
+        print(f"tile {tile}: worst rms {max(_errors(o.cpu().numpy(), r, m)[0] for o, r in zip(outs, refs)):.2e}")
+        plan.close()
+
+
 @pytest.mark.parametrize("persistent,graph", [(True, False), (False, True)])
 def test_plan_half_groups_mixed_geometries_and_tiles(persistent, graph):
     """One FP16-pair plan holding two geometries on different tiles: nine slices
@@ -267,19 +301,19 @@ def test_plan_half_groups_mixed_geometries_and_tiles(persistent, graph):
 
 
 def test_plan_slices_per_item_options():
-    """slices_per_item: values other than 0, 1, 2, 4, 8 are rejected at plan
-    creation; FP32 plans clamp 4 and 8 to pairs (the FP32 pair kernel), so their
+    """slices_per_item: values other than 0, 1, 2, 4, 8, 16 are rejected at plan
+    creation; FP32 plans clamp 4, 8 and 16 to pairs (the FP32 pair kernel), so their
     images equal the default's exactly."""
     specs, sinos = [], []
     for i in range(4):
         s, fdeg, c = _sino(96, 60, 1700 + i)
         specs.append(fbp.SliceSpec(96, fdeg, c))
         sinos.append(torch.from_numpy(s))
-    for bad in (3, 16, -1):
+    for bad in (3, 32, -1):
         with pytest.raises(RuntimeError):
             fbp.FbpPlan(specs, slices_per_item=bad)
     _, ref = _run(specs, sinos)
-    for k in (4, 8):
+    for k in (4, 8, 16):
         _, got = _run(specs, sinos, slices_per_item=k)
         for i in range(len(specs)):
             assert np.array_equal(got[i], ref[i]), (k, i)
@@ -420,7 +454,7 @@ def test_plan_tiles_agree():
     s, fdeg, c = _sino(W, P, 77)
     rows = phantom.to_int16(torch.from_numpy(s))
     ref_i = ref_f = None
-    for tile in range(4):
+    for tile in range(5):
         _, (im,) = _run([fbp.SliceSpec(W, fdeg, c, sino_dtype="i16")], [rows], interp="lut16_i32",
                         tile=tile)
         _, (fl,) = _run([fbp.SliceSpec(W, fdeg, c)], [torch.from_numpy(s)], tile=tile)

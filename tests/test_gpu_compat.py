@@ -138,7 +138,8 @@ def test_output_stage_matches_oracle():
 
 def test_unsupported_size_sets_error():
     """A width too small for CFft (n < 8, cfft.cpp:375) sets RECONST_INFO_ERROR and
-    logs 28808; an FFT longer than the LDS transform is refused by rvPlanCreate."""
+    logs 28808; transforms above 2^20 points are refused by rvPlanCreate, longer
+    than the LDS transform (two passes) accepted."""
     from recview_amd import fbp
     _lib.lib().RecviewClearErrors()
     rows = np.zeros((4, 3), np.int16)
@@ -146,7 +147,10 @@ def test_unsupported_size_sets_error():
         reconst.deconv_back_proj(rows, phantom.angles_deg(4), 1.0, normalise=False)
     assert 28808 in _lib.errors()
     with pytest.raises(RuntimeError, match="FFT range"):
-        fbp.FbpPlan([fbp.SliceSpec(17000, phantom.angles_deg(4), 8500.0)])
+        fbp.FbpPlan([fbp.SliceSpec(1100000, phantom.angles_deg(4), 550000.0)])
+    plan = fbp.FbpPlan([fbp.SliceSpec(17000, phantom.angles_deg(4), 8500.0)])
+    assert plan.info(0)["n_fft"] == 32768
+    plan.close()
 
 
 def test_kernel_wrappers():

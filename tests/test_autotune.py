@@ -46,3 +46,10 @@ def test_apply_sets_specs():
     assert (specs[1].tile, specs[1].chunk) == (2, 16)      # 451 rows after stitching
     assert (specs[2].tile, specs[2].chunk) == (2, 16)
     assert specs[0].to_c().chunk == 24
+def test_shipped_table_is_per_mode():
+    t = autotune.load()
+    assert t, "tuned table missing"
+    modes = {k.split(":")[0] for k in t}
+    assert {"linear2", "h16", "h8", "lut16_i32"} <= modes
+    for k, v in t.items():
+        assert len(k.split(":")) == 3 and v["tile"] in (0, 1, 2, 3, 4) and v["chunk"] > 0

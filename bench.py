@@ -47,8 +47,10 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make_stack(nslices, P, W, seed, device, photons=1e4):
-    """Float32 sinograms [nslices][P][W] of a seeded 40-ellipsoid phantom."""
+def make_stack(nslices, P, W, seed, device, photons=1e4, texture=0.02):
+    """Float32 sinograms [nslices][P][W] of a seeded 40-ellipsoid phantom with a
+    white-noise texture (each pixel's attenuation times 1 + texture * N(0, 1), the
+    texture projected by the ray-driven projector), Poisson noise and -ln."""
     import torch
     from recview_amd import phantom
     # attenuation per sample scaled with the width so that rays are not photon starved
@@ -61,6 +63,10 @@ def make_stack(nslices, P, W, seed, device, photons=1e4):
         z = -0.3 + 0.6 * (s + 0.5) / nslices
         ell = phantom.ellipsoid_slice(ells, z)
         sino = phantom.sinogram(ell, W, fdeg, c_det, scale=W, device=device)
+        if texture:
+            img = phantom.image(ell, W, W, device=device, dtype=torch.float32)
+            noise = torch.randn(W, W, device=device, generator=gen)
+            sino = sino + phantom.texture_sinogram(img * noise * texture, W, fdeg, c_det).double()
         out[s] = phantom.poisson(sino, photons, generator=gen)
     return out, fdeg, c_det
 
@@ -340,7 +346,7 @@ def d2h_inclusive(plan, specs, sino_list, outs, args, stream, S, updates):
     import torch
     from recview_amd import fbp
     plan_b = fbp.FbpPlan(specs, interp=args.interp, store=args.store, persistent=bool(args.persistent),
-                         tile=args.tile, angles_per_chunk=args.chunk, lds_half=bool(args.lds_half),
+                         tile=max(args.tile, 0), angles_per_chunk=args.chunk, lds_half=bool(args.lds_half),
                          slices_per_item=args.slices_per_item)
     outs_b = [torch.empty_like(o) for o in outs]
     host = torch.empty((S,) + tuple(outs[0].shape), dtype=outs[0].dtype, pin_memory=True)
@@ -402,7 +408,7 @@ def main():
     ap.add_argument("--width", type=int, default=2048)
     ap.add_argument("--interp", default="linear", choices=["linear", "lut16_f32", "lut16_i32"])
     ap.add_argument("--store", default="f32", choices=["f32", "f16", "bf16"])
-    ap.add_argument("--tile", type=int, default=0)
+    ap.add_argument("--tile", type=int, default=-1, help="-1: tile and chunk from the tuned table")
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--lds-half", type=int, default=0, help="FP16 pairs in LDS for 16-bit stores")
     ap.add_argument("--persistent", type=int, default=1)
@@ -442,6 +448,12 @@ def main():
             sinos = phantom.to_int16(sinos)
         dtype = "i16" if args.interp == "lut16_i32" else "f32"
         specs = [fbp.SliceSpec(W, fdeg, c_det, sino_dtype=dtype) for _ in range(S)]
+        if args.tile < 0:
+            # tile and angle chunk of the plan's mode from the tuned table
+            from recview_amd import autotune
+            mode = autotune.mode_of(args.interp, args.store, bool(args.lds_half), args.slices_per_item)
+            if mode:
+                autotune.apply(specs, [W] * S, mode)
         sino_list = [sinos[s] for s in range(S)]
         shapes = [(W, P)] * S
         desc = (f"C2: {S}-slice stack, {P} proj x {W} det -> {W}^2, {args.interp} interpolation, "
@@ -456,7 +468,7 @@ def main():
     log(f"inputs generated in {time.time() - t0:.1f}s")
 
     plan = fbp.FbpPlan(specs, interp=args.interp, store=args.store, persistent=bool(args.persistent),
-                       tile=args.tile, angles_per_chunk=args.chunk, lds_half=bool(args.lds_half),
+                       tile=max(args.tile, 0), angles_per_chunk=args.chunk, lds_half=bool(args.lds_half),
                        slices_per_item=args.slices_per_item)
     outs = [torch.empty(plan.out_n(s), plan.out_n(s), dtype=odt, device=dev) for s in range(S)]
     stream = torch.cuda.current_stream()
@@ -519,13 +531,15 @@ def main():
     traffic = None
     if args.workload == "c2" and args.interp == "linear" and args.store == "f32" and S == 64 \
             and args.proj == 1800 and args.width == 2048 and args.slices_per_item in (0, 2) \
-            and args.tile == 0 and args.chunk == 0:
+            and args.tile in (-1, 0) and args.chunk == 0:
         traffic = read_traffic("bp_linear_c2_traffic.md")
     config = {"workload": desc, "slices": S, "interp": args.interp, "store": args.store,
               "persistent": bool(args.persistent), "graph": bool(args.graph),
               "lds_half": bool(args.lds_half), "slices_per_item": args.slices_per_item, "updates": updates}
     if args.workload == "c2":
-        config.update({"projections": P, "detector": W, "image": W, "tile": args.tile})
+        config.update({"projections": P, "detector": W, "image": W,
+                       "tile": specs[0].tile if args.tile < 0 else args.tile,
+                       "chunk": specs[0].chunk if args.tile < 0 else args.chunk})
     result = {
         "metric": METRIC,
         "value": round(slices_per_s, 3),

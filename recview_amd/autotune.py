@@ -49,7 +49,7 @@ def mode_of(interp="linear", store="f32", lds_half=False, slices_per_item=0):
     if interp != "linear":
         return None
     if store != "f32" and lds_half:
-        return {4: "h4", 16: "h16"}.get(slices_per_item, "h8")
+        return {4: "h4", 8: "h8"}.get(slices_per_item, "h16")
     if store == "f32":
         return "linear" if slices_per_item == 1 else "linear2"
     return None

@@ -143,6 +143,35 @@ def image_roi(ellipses, n_out, scale, h_ref, x0, y0, zoom, device="cpu", dtype=t
     return out
 
 
+def texture_sinogram(img, W, fdeg, c_det, batch=16):
+    """Ray-driven parallel-beam projection of a pixel image on the N x N
+    reconstruction grid (pixel (ix, iy) at X = ix - N/2, Y = N/2 - iy): for row i and
+    sample k the line X cos + Y sin = k - c_det, sampled every pixel along the ray
+    with bilinear interpolation and summed (a discrete forward projector for
+    structure with no analytic line integral, such as a noise texture)."""
+    import torch.nn.functional as F
+    N = img.shape[0]
+    dev, dt = img.device, torch.float32
+    h = N / 2.0
+    t = torch.arange(W, device=dev, dtype=dt) - float(c_det)
+    s = torch.arange(N, device=dev, dtype=dt) - (N - 1) / 2.0
+    th = np.asarray(fdeg, np.float64) * (math.pi / 180.0)
+    out = torch.empty(len(th), W, device=dev, dtype=dt)
+    src = img.to(dt)[None, None]
+    for a0 in range(0, len(th), batch):
+        c = torch.as_tensor(np.cos(th[a0:a0 + batch]), device=dev, dtype=dt)[:, None, None]
+        sn = torch.as_tensor(np.sin(th[a0:a0 + batch]), device=dev, dtype=dt)[:, None, None]
+        X = t[None, None, :] * c - s[None, :, None] * sn
+        Y = t[None, None, :] * sn + s[None, :, None] * c
+        gx = (X + h) * (2.0 / (N - 1)) - 1.0
+        gy = (h - Y) * (2.0 / (N - 1)) - 1.0
+        A = X.shape[0]
+        grid = torch.stack((gx, gy), dim=-1).reshape(1, A * N, W, 2)
+        v = F.grid_sample(src, grid, mode="bilinear", padding_mode="zeros", align_corners=True)
+        out[a0:a0 + A] = v.reshape(A, N, W).sum(dim=1)
+    return out
+
+
 def poisson(sino, photons, generator=None):
     """-ln(max(k,1)/I0) with k ~ Poisson(I0 exp(-p)) (float32 out)."""
     lam = photons * torch.exp(-sino.double())

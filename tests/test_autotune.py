@@ -29,7 +29,8 @@ def test_lookup_per_mode():
 def test_mode_of_plan_options():
     assert autotune.mode_of("linear", "f32") == "linear2"
     assert autotune.mode_of("linear", "f32", slices_per_item=1) == "linear"
-    assert autotune.mode_of("linear", "f16", lds_half=True) == "h8"
+    assert autotune.mode_of("linear", "f16", lds_half=True) == "h16"
+    assert autotune.mode_of("linear", "f16", lds_half=True, slices_per_item=8) == "h8"
     assert autotune.mode_of("linear", "bf16", lds_half=True, slices_per_item=4) == "h4"
     assert autotune.mode_of("lut16_i32") == "lut16_i32"
     assert autotune.mode_of("lut16_f32") is None
@@ -46,6 +47,8 @@ def test_apply_sets_specs():
     assert (specs[1].tile, specs[1].chunk) == (2, 16)      # 451 rows after stitching
     assert (specs[2].tile, specs[2].chunk) == (2, 16)
     assert specs[0].to_c().chunk == 24
+
+
 def test_shipped_table_is_per_mode():
     t = autotune.load()
     assert t, "tuned table missing"

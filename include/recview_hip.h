@@ -131,7 +131,10 @@ typedef struct RECONST_INFO RECONST_INFO;
 /* cudaReconst.h:16 (impl cudaReconstHost.cpp:629-813): reconstructs one slice
  * (int16 rows ri->iSinogr selected by bInc/fdeg) into the int32 image
  * ri->iReconst, with the reference's LUT16 integer semantics.  Sets
- * ri->iStatus = RECONST_INFO_ERROR on allocation failure; never throws. */
+ * ri->iStatus = RECONST_INFO_ERROR on allocation failure; never throws.
+ * A pinned iReconst (CUDA_MALLOC_HOST_INT, as RecView allocates it for the GPU
+ * path) is written by the back-projection kernel itself, so the download
+ * overlaps it; d_ifp then does not receive the image. */
 void CudaReconstHost(RECONST_INFO* ri, int idev, bool bReport, bool bEnStream);
 /* cudaReconst.h:17 (cudaReconstHost.cpp:142-203) */
 void CudaReconstMemFree(RECONST_INFO* ri);

@@ -23,11 +23,15 @@ def _stream_handle(stream):
     return C.c_void_p(stream.cuda_stream)
 
 
-def _ptrs(tensors):
+def _ptrs(tensors, host_ok=False):
+    """Device pointers of contiguous device tensors (or, with host_ok, of pinned
+    host tensors: pinned memory is mapped into the GPU's address space, so a kernel
+    writes it across the host link)."""
     arr = (C.c_void_p * len(tensors))()
     for i, t in enumerate(tensors):
-        if not t.is_cuda or not t.is_contiguous():
-            raise ValueError("tensors must be contiguous device tensors")
+        if not t.is_contiguous() or not (t.is_cuda or (host_ok and t.is_pinned())):
+            raise ValueError("tensors must be contiguous device tensors" +
+                             (" or pinned host tensors" if host_ok else ""))
         arr[i] = t.data_ptr()
     return arr
 
@@ -117,14 +121,17 @@ class FbpPlan:
         _lib.check(_lib.lib().rvPlanRows(self._h, s, C.byref(n), C.byref(w)), "rvPlanRows")
         return n.value, w.value
 
-    def _count(self, tensors, what):
+    def _count(self, tensors, what, host_ok=False):
         if len(tensors) != len(self.slices):
             raise ValueError(f"{what}: {len(tensors)} tensors for a plan of {len(self.slices)} slices")
-        return _ptrs(tensors)
+        return _ptrs(tensors, host_ok)
 
     def execute(self, sinos, outs, stream=None, graph=False):
+        """Reconstructs every slice.  Images may be device tensors or pinned host
+        tensors (the back-projection then writes each finished tile across the host
+        link, so the download overlaps the kernel)."""
         fn = _lib.lib().rvPlanExecuteGraph if graph else _lib.lib().rvPlanExecute
-        _lib.check(fn(self._h, self._count(sinos, "sinograms"), self._count(outs, "images"),
+        _lib.check(fn(self._h, self._count(sinos, "sinograms"), self._count(outs, "images", True),
                       _stream_handle(stream)), "rvPlanExecute")
 
     def filter(self, sinos, stream=None):

@@ -104,16 +104,21 @@ int main(int argc, char** argv)
     if (!f || std::fwrite(image, sizeof(int), (size_t)N * N, f) != (size_t)N * N) return 5;
     std::fclose(f);
     if (argc > 7) {
-        // 16-bit slice: the output stage on the device image kept by the engine
-        // (RECONST_INFO::d_ifp, int32), then RecView's TIFF layout.
+        // 16-bit slice: the output stage on the GPU (the int32 image uploaded from
+        // iReconst, which the engine wrote directly since it is pinned), then
+        // RecView's TIFF layout.
         std::vector<unsigned short> u16((size_t)N * N);
         unsigned short* d_out = nullptr;
+        int* d_img = nullptr;
         double lo = 0, hi = 0;
         if (RecviewDeviceAlloc((void**)&d_out, sizeof(unsigned short) * (size_t)N * N) != 0 ||
-            RecviewOutputStage(ri.d_ifp, 0, N, c_user, 2, d_out, &lo, &hi, nullptr) != 0 ||
+            RecviewDeviceAlloc((void**)&d_img, sizeof(int) * (size_t)N * N) != 0 ||
+            RecviewCopyToDevice(d_img, image, sizeof(int) * (size_t)N * N) != 0 ||
+            RecviewOutputStage(d_img, 0, N, c_user, 2, d_out, &lo, &hi, nullptr) != 0 ||
             RecviewCopyToHost(u16.data(), d_out, sizeof(unsigned short) * u16.size()) != 0)
             return 6;
         RecviewDeviceFree(d_out);
+        RecviewDeviceFree(d_img);
         std::vector<int> as_int(u16.begin(), u16.end());
         char desc[256];
         RecviewImageDescription(desc, sizeof(desc), nsino, 0.5, 1.0, lo, c_user, 0);

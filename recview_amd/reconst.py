@@ -135,11 +135,13 @@ def output_stage(img, nsino, center, iInterpolation=2, pixel_width_um=0.5, zerni
 
 def deconv_back_proj(rows_i16, fdeg, center, iInterpolation=2, ifilter=0, cutoff=0.5, order=6.0,
                      tilt=0.0, binc=None, flags=0, device=0, report=False, stream=True,
-                     pixel_width_um=0.5, info=None, normalise=True):
+                     pixel_width_um=0.5, info=None, normalise=True, image=None):
     """CGazoDoc::DeconvBackProj for one device (nCPU = 1).
 
     rows_i16 : int16 sinogram rows [P][W] (x LOG_SCALE), already binned.
     center   : user centre (the reference adds HALFPIXEL_OFFSET and divides by binning).
+    image    : optional int32 [N][N] output buffer (RecView's iReconst; a pinned one,
+               from CUDA_MALLOC_HOST_INT, is written by the GPU directly).
     Returns a dict with the raw int32 image, nSinogr, seconds, and (if
     normalise) the 16-bit-range image with pixelBase / pixelDiv."""
     rows = np.ascontiguousarray(rows_i16, np.int16)
@@ -148,7 +150,7 @@ def deconv_back_proj(rows_i16, fdeg, center, iInterpolation=2, ifilter=0, cutoff
     filt = set_filter(ndim, ifilter, cutoff, order)
     own = info is None
     info = ReconstInfo() if own else info
-    img, cnt = fill_info(info, rows, fdeg, center, iInterpolation, filt, tilt, binc, flags)
+    img, cnt = fill_info(info, rows, fdeg, center, iInterpolation, filt, tilt, binc, flags, image=image)
     t0 = time.perf_counter()
     _lib.lib().CudaReconstHost(C.byref(info.ri), int(device), bool(report), bool(stream))
     t1 = time.perf_counter()

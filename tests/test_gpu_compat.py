@@ -61,6 +61,29 @@ def test_reconst_host_bit_exact(W, P, interp, c_off, formula, rule):
     assert np.array_equal(res["image"], ref), f"max diff {np.abs(res['image'].astype(np.int64) - ref).max()}"
 
 
+def test_reconst_host_pinned_image_written_directly():
+    """A pinned image buffer (CUDA_MALLOC_HOST_INT, as RecView allocates iReconst for
+    the CUDA path) is written by the back-projection itself: the same integer image
+    as with a pageable buffer and as the oracle, also when the buffer held stale
+    values, and on a second call with a kept sinogram."""
+    L = _lib.lib()
+    W, P = 200, 90
+    rows, fdeg, c_det = _case(W, P, 4321, 1.3)
+    p = C.POINTER(C.c_int)()
+    assert L.CUDA_MALLOC_HOST_INT(C.byref(p), W * W * 4) == 0
+    try:
+        pinned = np.ctypeslib.as_array(p, shape=(W, W))
+        pinned[:] = 12345
+        a = reconst.deconv_back_proj(rows, fdeg, c_det - 0.5, normalise=False, image=pinned)
+        b = reconst.deconv_back_proj(rows, fdeg, c_det - 0.5, normalise=False)
+        ref = _oracle_rint(rows, fdeg, c_det, 2)
+        assert np.array_equal(pinned, ref) and np.array_equal(a["image"], ref)
+        assert np.array_equal(b["image"], ref)
+        assert a["nSinogr"] == P
+    finally:
+        assert L.CUDA_FREE_HOST(p) == 0
+
+
 def test_reconst_host_fft_variant_bit_exact():
     """CudaReconstHostFFT: zero padding, float zoom, placement n/2 (cudaReconst.cu:123-192)."""
     rows, fdeg, c_det = _case(96, 50, 31, 0.2)

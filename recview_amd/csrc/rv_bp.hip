@@ -212,12 +212,15 @@ __global__ void __launch_bounds__(64 * NW, NW == 8 ? 4 : 1) bp_tiled_kernel(BpLa
         // FP16 multi-slice modes: the item scale the packed pairs were divided by
         const float gscale = is_half_multi(MODE) ? *S.gscale : 1.f;
 
-        // angles of this work item: all, or one chunk-aligned part of them
-        int a_begin = 0, a_end = S.nang;
+        // angles of this work item: all (or the launch's window of them), or one
+        // chunk-aligned part of those
+        const int w0 = L.ang_end > 0 ? min(L.ang_begin, S.nang) : 0;
+        const int w1 = L.ang_end > 0 ? min(L.ang_end, S.nang) : S.nang;
+        int a_begin = w0, a_end = w1;
         if (parts > 1) {
-            const int per = ((S.nang + parts - 1) / parts + chunk - 1) / chunk * chunk;
-            a_begin = min(S.nang, part * per);
-            a_end = min(S.nang, a_begin + per);
+            const int per = ((w1 - w0 + parts - 1) / parts + chunk - 1) / chunk * chunk;
+            a_begin = min(w1, w0 + part * per);
+            a_end = min(w1, a_begin + per);
         }
         for (int a0 = a_begin; a0 < a_end; a0 += chunk) {
             // ---- per-angle tile geometry

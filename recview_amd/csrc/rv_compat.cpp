@@ -10,6 +10,7 @@
 #include "recview_hip.h"
 #include "rv_common.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -118,18 +119,22 @@ void* pinned_device_pointer(const void* p)
     return (a.type == hipMemoryTypeHost && a.devicePointer) ? a.devicePointer : nullptr;
 }
 
-// Frame selection of cudaReconstHost.cpp:712-730 with the progress / abort
-// callback of :715-717.  Returns the projections to use, in order.
-std::vector<Selected> select_frames(RECONST_INFO* ri, bool bReport)
+// Frame selection of cudaReconstHost.cpp:712-730: the projections to use, in
+// order.  The progress / abort callback of :715-717 falls at every loop
+// position i with (i - iStartSino) / iStepSino a multiple of 20; each such
+// point is returned with the number of projections selected before it (an
+// abort there keeps exactly those).  The callbacks themselves are made while
+// the back-projection runs (see reconst_body).
+struct CallbackPoint {
+    int i, nsel_before;
+};
+
+std::vector<Selected> select_frames(RECONST_INFO* ri, bool bReport, std::vector<CallbackPoint>& points)
 {
     std::vector<Selected> sel;
-    const int iProgStep = ri->iLenSinogr / 20;                     // PROGRESS_BAR_UNIT
-    int iCurrStep = 0;
     for (int i = ri->iStartSino; i < ri->iLenSinogr - 1; i += ri->iStepSino) {
         const int isino = (i - ri->iStartSino) / ri->iStepSino;
-        if (bReport && (isino % 20 == 0)) {
-            if (call_dlgctrl(ri, iProgStep, i, &iCurrStep)) break;
-        }
+        if (bReport && (isino % 20 == 0)) points.push_back({i, (int)sel.size()});
         if (!(ri->bInc[i] & CGAZODOC_BINC_SAMPLE)) continue;
         const int sidx = i * ri->iMultiplex + ri->iOffset;
         if (sidx >= ri->maxSinogrLen) break;
@@ -140,7 +145,6 @@ std::vector<Selected> select_frames(RECONST_INFO* ri, bool bReport)
                 if (ri->dReconFlags & RQFLAGS_USEONLYODDFRAMES) continue;
             }
         }
-        if (ri->nSinogr) (*(ri->nSinogr))++;
         sel.push_back({i, isino, sidx});
     }
     return sel;
@@ -202,9 +206,15 @@ void reconst_body(RECONST_INFO* ri, int idev, bool bReport, bool bEnStream, bool
         }
     }
 
-    std::vector<Selected> sel = select_frames(ri, bReport);
+    std::vector<CallbackPoint> points;
+    std::vector<Selected> sel = select_frames(ri, bReport, points);
     const int nsel = (int)sel.size();
+    const int iProgStep = ri->iLenSinogr / 20;                     // PROGRESS_BAR_UNIT
+    int iCurrStep = 0;
+    bool host_direct = false;
     if (nsel == 0) {
+        for (const auto& pt : points)
+            if (call_dlgctrl(ri, iProgStep, pt.i, &iCurrStep)) break;
         if (!rv::hip_ok(hipMemsetAsync(ri->d_ifp, 0, mem_ifp, st), "image clear")) rv::log_code(28806);
     } else {
         // Metadata of the launch -- angle table, slice descriptor, row slots -- staged
@@ -238,8 +248,10 @@ void reconst_body(RECONST_INFO* ri, int idev, bool bReport, bool bEnStream, bool
         // A pinned image (RecView allocates it with CUDA_MALLOC_HOST_INT,
         // gazoDoc.cpp:3893) is mapped into the GPU's address space: the
         // back-projection then writes each finished tile straight into it, so the
-        // image download overlaps the kernel instead of following it.
-        void* host_img = pinned_device_pointer(ri->iReconst);
+        // image download overlaps the kernel instead of following it.  Progress
+        // reporting back-projects in segments that add into the device image.
+        void* host_img = bReport ? nullptr : pinned_device_pointer(ri->iReconst);
+        host_direct = host_img != nullptr;
         // one item per tile there: 32x32 tiles balance the single launch better
         // (2048^2 x 1800 from the C++ host: 2.71 ms per slice with 64x64 tiles,
         // 2.59 ms with 32x32 and 48 angles per chunk)
@@ -307,7 +319,9 @@ void reconst_body(RECONST_INFO* ri, int idev, bool bReport, bool bEnStream, bool
             }
         }
 
-        // Back-projection of every selected projection in one launch.
+        // Back-projection of the selected projections: one launch, or with progress
+        // reporting one launch per segment between callback points.
+        int used = nsel;
         if (ok) {
             rv::BpLaunch L{};
             L.slices = d_slice;
@@ -327,9 +341,51 @@ void reconst_body(RECONST_INFO* ri, int idev, bool bReport, bool bEnStream, bool
             L.angle_parts = host_img ? 1 : 4;
             L.total_items *= L.angle_parts;
             if (!host_img) rv::launch_zero_u32((unsigned int*)ri->d_ifp, N * N, st);
-            rv::launch_bp(L, L.total_items, st);
+            if (!bReport) {
+                rv::launch_bp(L, L.total_items, st);
+            } else {
+                // DBProjDlgCtrl at every 20th loop position (cudaReconstHost.cpp:715-717)
+                // while the GPU works: segments of at least max(40, 1/8 of the
+                // projections) end at callback points, the GPU at most one segment
+                // ahead of the callbacks (the reference synchronises every 40
+                // projections, :796).  Returning true keeps the projections before
+                // that point, as the reference's loop break does.
+                const int seg_min = std::max(40, (nsel + 7) / 8);
+                hipEvent_t ev[2] = {nullptr, nullptr};
+                const bool have_ev =
+                    hipEventCreateWithFlags(&ev[0], hipEventDisableTiming) == hipSuccess &&
+                    hipEventCreateWithFlags(&ev[1], hipEventDisableTiming) == hipSuccess;
+                int launched = 0, nseg = 0;
+                auto launch_segment = [&](int a1) {
+                    L.ang_begin = launched;
+                    L.ang_end = a1;
+                    rv::launch_bp(L, L.total_items, st);
+                    launched = a1;
+                };
+                for (const auto& pt : points) {
+                    if (pt.nsel_before - launched >= seg_min) {
+                        launch_segment(pt.nsel_before);
+                        if (have_ev) {
+                            (void)hipEventRecord(ev[nseg & 1], st);
+                            if (nseg > 0) (void)hipEventSynchronize(ev[(nseg - 1) & 1]);
+                        } else {
+                            (void)hipStreamSynchronize(st);
+                        }
+                        nseg++;
+                    }
+                    if (call_dlgctrl(ri, iProgStep, pt.i, &iCurrStep)) {
+                        used = pt.nsel_before;
+                        break;
+                    }
+                }
+                if (used > launched) launch_segment(used);
+                (void)hipStreamSynchronize(st);
+                if (ev[0]) (void)hipEventDestroy(ev[0]);
+                if (ev[1]) (void)hipEventDestroy(ev[1]);
+            }
             ok = rv::hip_ok(hipGetLastError(), "back-projection launch");
         }
+        if (ri->nSinogr) *(ri->nSinogr) += used;
         if (!ok) {
             (void)hipStreamSynchronize(st);
             rv::log_code(28806);
@@ -340,7 +396,7 @@ void reconst_body(RECONST_INFO* ri, int idev, bool bReport, bool bEnStream, bool
     // D2H of the image (cudaReconstHost.cpp:810-811); the copy waits for the stream.
     // A pinned image has already been written by the back-projection.
     if (!rv::hip_ok(hipStreamSynchronize(st), "reconstruction") ||
-        (!(nsel > 0 && pinned_device_pointer(ri->iReconst)) &&
+        (!host_direct &&
          !rv::hip_ok(hipMemcpy(ri->iReconst, ri->d_ifp, mem_ifp, hipMemcpyDeviceToHost), "image download")))
         rv::log_code(28805);
 }

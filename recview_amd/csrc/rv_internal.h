@@ -172,6 +172,8 @@ struct BpLaunch {
     int angle_parts;        // LUT16 integer mode: a tile's angles split over this many work
                             // items, their integer sums added atomically into a zeroed
                             // image (order-free, so bit-exact); 0 or 1: one item per tile
+    int ang_begin, ang_end; // ang_end > 0: only the slice's angles [ang_begin, ang_end)
+                            // (a segment of a launch sequence); 0: all of them
 };
 
 void launch_bp(const BpLaunch& L, int grid, hipStream_t stream);

@@ -136,6 +136,40 @@ def test_abort_callback_and_progress():
     assert np.array_equal(res["image"], ref)
 
 
+def test_progress_segments_and_late_abort():
+    """With progress reporting the back-projection runs in segments between the
+    callback points while the callbacks are made: 400 projections give 20 callbacks,
+    the image of a run without abort equals the unsegmented one bit for bit, and an
+    abort at position 260 keeps exactly the 260 projections before it (skipped
+    frames included in the positions, not in the count)."""
+    rows, fdeg, c_det = _case(64, 400, 19)
+    binc = np.ones(401, np.int8)
+    binc[[5, 77, 261]] = 0
+    plain = reconst.deconv_back_proj(rows, fdeg, c_det - 0.5, binc=binc, normalise=False)
+    for stop in (None, 260):
+        calls = []
+
+        def cb(ri, step, isino, pcur):
+            calls.append(isino)
+            return stop is not None and isino >= stop
+
+        fn = _lib.DLGCTRL(cb)
+        _lib.lib().RecviewSetDlgCtrl(C.cast(fn, C.c_void_p))
+        try:
+            res = reconst.deconv_back_proj(rows, fdeg, c_det - 0.5, binc=binc, report=True, normalise=False)
+        finally:
+            _lib.lib().RecviewSetDlgCtrl(None)
+        if stop is None:
+            assert calls == list(range(0, 400, 20))
+            assert res["nSinogr"] == plain["nSinogr"] == 397
+            assert np.array_equal(res["image"], plain["image"])
+        else:
+            assert calls == list(range(0, 280, 20))
+            keep = [i for i in range(260) if binc[i]]
+            assert res["nSinogr"] == len(keep) == 258
+            assert np.array_equal(res["image"], _oracle_rint(rows[keep], fdeg[keep], c_det, 2))
+
+
 def test_frame_selection_and_streams():
     """bInc skips and the even-only flag (cudaReconstHost.cpp:718-729); the stream
     and null-stream paths give identical images."""

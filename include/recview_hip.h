@@ -269,6 +269,13 @@ typedef struct {
      * 0 = the plan's (RvOptions.angles_per_chunk, else the mode's default).  The
      * slices of one launch (same tile and mode) take the first one's. */
     int chunk;
+    /* 0: the slice filters its own sinogram.  s + 1: the slice back-projects the
+     * filtered rows of the earlier slice s with its own geometry (same detector,
+     * zoom, projections and filter; no offset CT): the integer part of the centre
+     * difference shifts the rows as the reference's kept-sinogram iCenterOffset
+     * does (cudaReconstHost.cpp:775-777), so a centre sweep filters once.  The
+     * slice's sinogram pointer is then not read. */
+    int rows_of;
 } RvSliceSpec;
 
 typedef struct {

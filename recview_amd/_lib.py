@@ -46,7 +46,7 @@ class RvSliceSpec(C.Structure):
         ("cutoff", C.c_double), ("order", C.c_double), ("sino_dtype", C.c_int),
         ("pad_mode", C.c_int), ("out_n", C.c_int), ("pix_zoom", C.c_double),
         ("x0", C.c_double), ("y0", C.c_double), ("tile", C.c_int), ("offset_ct", C.c_int),
-        ("binc", C.c_char_p), ("chunk", C.c_int),
+        ("binc", C.c_char_p), ("chunk", C.c_int), ("rows_of", C.c_int),
     ]
 
 

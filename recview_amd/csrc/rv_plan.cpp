@@ -36,6 +36,8 @@ struct SliceState {
     int group = 0;
     int tile = 0;
     int fclass = 0;              // first slice with the same filter job (batched launches)
+    int rows_src = -1;           // slice whose filtered rows this one back-projects (-1: its own)
+    size_t scale_off = 0;        // first per-row scale of the rows it reads (16-bit stores)
     // offset CT
     bool offset = false;
     int raw_W = 0;
@@ -203,7 +205,7 @@ rv::FilterJob filter_job(rv_plan_t p, size_t s)
         J.out = (float*)p->d_rows + S.rows_off;
     } else {
         J.out_lp = (unsigned char*)p->d_rows + S.rows_off * esz;
-        J.out_scale = p->d_scales + S.ang_off;
+        J.out_scale = p->d_scales + S.scale_off;
     }
     return J;
 }
@@ -222,6 +224,7 @@ int launch_filters(rv_plan_t p, const void* const* d_sino, hipStream_t st)
     };
     for (size_t s = 0; s < p->sl.size(); s++) {
         const SliceState& S = p->sl[s];
+        if (S.rows_src >= 0) continue;               // back-projects another slice's rows
         const bool is_float = (S.spec.sino_dtype != RV_SINO_I16);
         const void* src = d_sino[s];
         if (S.offset) {
@@ -357,11 +360,28 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
             S.out_n = S.g.N;
         }
         S.nang = (int)S.fdeg.size();
-        S.rows_off = rows_elems;
         S.ang_off = nang_total;
         S.filt_off = filt_total;
-        rows_elems += (size_t)S.nang * S.g.N;
         nang_total += (size_t)S.nang;
+        if (S.spec.rows_of > 0) {
+            // the filtered rows of an earlier slice, shifted by the integer centre offset
+            if (S.spec.rows_of > s) return fail(p, "rows_of must name an earlier slice");
+            int r = S.spec.rows_of - 1;
+            if (p->sl[r].rows_src >= 0) r = p->sl[r].rows_src;
+            const SliceState& R = p->sl[r];
+            if (S.offset || R.offset) return fail(p, "offset-CT slices cannot share filtered rows");
+            if (S.g.W != R.g.W || S.g.z != R.g.z || S.g.n != R.g.n || S.nang != R.nang ||
+                S.spec.filter != R.spec.filter || S.spec.cutoff != R.spec.cutoff || S.spec.order != R.spec.order ||
+                S.spec.pad_mode != R.spec.pad_mode)
+                return fail(p, "rows_of needs the same detector, zoom, projections and filter");
+            S.rows_src = r;
+            S.rows_off = R.rows_off;
+            S.scale_off = R.scale_off;
+        } else {
+            S.rows_off = rows_elems;
+            S.scale_off = S.ang_off;
+            rows_elems += (size_t)S.nang * S.g.N;
+        }
         filt_total += (size_t)S.g.n;
         S.tile = (S.spec.tile >= 0 && S.spec.tile <= 4) ? S.spec.tile : plan_tile;
         S.tiles_x = (S.out_n + rv::bp_tile_side(S.tile) - 1) / rv::bp_tile_side(S.tile);
@@ -378,7 +398,10 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
         rv::set_filter(S.g.n, S.spec.filter, S.spec.cutoff, S.spec.order, f);
         float* fs = hf.data() + filt_total + S.filt_off;
         for (int k = 0; k < S.g.n; k++) fs[k] = 0.5f * (f[k] + f[(S.g.n - k) % S.g.n]);
-        const double fc = (double)S.g.fcenter16 / 16.0;
+        // slices on another slice's rows: that slice's rows place sample ic_src where
+        // this slice's own would place ic, so the coordinate moves by the difference
+        const int icoff = S.rows_src >= 0 ? (S.g.ic - p->sl[S.rows_src].g.ic) * S.g.z : 0;
+        const double fc = (double)S.g.fcenter16 / 16.0 + icoff;
         const double h = S.g.h;
         for (int a = 0; a < S.nang; a++) {
             rv::AngleEntry& e = ha[S.ang_off + a];
@@ -395,7 +418,7 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
                 e.dux = c;
                 e.duy = sn;
             }
-            rv::lut_params(S.fdeg[a], S.spec.tilt, S.g, 0, &e.fcos, &e.fsin, &e.foffset);
+            rv::lut_params(S.fdeg[a], S.spec.tilt, S.g, icoff, &e.fcos, &e.fsin, &e.foffset);
             e.row = a;
         }
     }
@@ -404,9 +427,10 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
     for (int s = 0; s < nslices; s++) {
         SliceState& S = p->sl[s];
         S.fclass = s;
+        if (S.rows_src >= 0) continue;
         for (int t = 0; t < s; t++) {
             const SliceState& T = p->sl[t];
-            if (T.fclass != t) continue;
+            if (T.fclass != t || T.rows_src >= 0) continue;
             if (T.g.W == S.g.W && T.g.z == S.g.z && T.g.ic == S.g.ic && T.g.n == S.g.n &&
                 T.g.ndimp == S.g.ndimp && T.g.N == S.g.N && T.g.h == S.g.h && T.g.ihoffset == S.g.ihoffset &&
                 T.nang == S.nang && T.spec.pad_mode == S.spec.pad_mode &&
@@ -572,7 +596,7 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
             rv::BpSlice& B = G.h[q];
             std::memset(&B, 0, sizeof(B));
             B.rows = (const unsigned char*)p->d_rows + S.rows_off * esz;
-            B.scales = (p->store == 0) ? nullptr : p->d_scales + S.ang_off;
+            B.scales = (p->store == 0) ? nullptr : p->d_scales + S.scale_off;
             B.row_stride = S.g.N;
             B.row_len = S.g.N;
             B.nang = S.nang;
@@ -591,7 +615,7 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
                 if (s2 < 0) continue;
                 const SliceState& S2 = p->sl[s2];
                 B.rows_x[e] = (const unsigned char*)p->d_rows + S2.rows_off * esz;
-                B.scales_x[e] = (p->store == 0) ? nullptr : p->d_scales + S2.ang_off;
+                B.scales_x[e] = (p->store == 0) ? nullptr : p->d_scales + S2.scale_off;
             }
             if (packed_size(G.mode)) {
                 const int ns = rv::bp_slices_per_item(G.mode);
@@ -748,7 +772,7 @@ int rvPlanFilteredRows(rv_plan_t p, int s, void** rows, float** scales, int* row
     const SliceState& S = p->sl[s];
     const size_t esz = (p->store == 0) ? 4 : 2;
     if (rows) *rows = (unsigned char*)p->d_rows + S.rows_off * esz;
-    if (scales) *scales = (p->store == 0) ? nullptr : p->d_scales + S.ang_off;
+    if (scales) *scales = (p->store == 0) ? nullptr : p->d_scales + S.scale_off;
     if (row_len) *row_len = S.g.N;
     return 0;
 }

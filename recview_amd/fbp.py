@@ -41,13 +41,16 @@ class SliceSpec:
 
     def __init__(self, W, fdeg, center, iInterpolation=2, tilt=0.0, filter="hann", cutoff=0.5,
                  order=6.0, sino_dtype="f32", pad="replicate", out_n=0, pix_zoom=1.0, x0=0.0,
-                 y0=0.0, tile=-1, offset_ct=False, binc=None, chunk=0):
+                 y0=0.0, tile=-1, offset_ct=False, binc=None, chunk=0, rows_from=-1):
         """center follows RECONST_INFO.center (user centre + 0.5); offset_ct: the
         rows are a 360-degree scan (angles over 0..360) with an off-centre axis,
         stitched to double width on the GPU; tile = -1 uses the plan's tile;
-        chunk = 0 the plan's angles per LDS chunk (see autotune.apply)."""
+        chunk = 0 the plan's angles per LDS chunk (see autotune.apply); rows_from =
+        index of an earlier slice whose filtered rows this one back-projects (its
+        integer centre offset applied as the reference's kept-sinogram path does)."""
         self.tile = int(tile)
         self.chunk = int(chunk)
+        self.rows_from = int(rows_from)
         self.offset_ct = bool(offset_ct)
         self.binc = None if binc is None else np.ascontiguousarray(binc, np.int8)
         self.W = int(W)
@@ -76,6 +79,7 @@ class SliceSpec:
         s.tile, s.offset_ct = self.tile, int(self.offset_ct)
         s.binc = None if self.binc is None else self.binc.ctypes.data_as(C.c_char_p)
         s.chunk = self.chunk
+        s.rows_of = self.rows_from + 1
         return s
 
 
@@ -215,19 +219,26 @@ def centre_sweep(sino, fdeg, centres, stream=None, **kw):
     """Axis-scan images: one sinogram reconstructed at every centre in `centres`
     (RECONST_INFO.center convention) in one persistent launch, the batched form of
     the reference's kept-sinogram centre sweep (gazoDoc.cpp:1994-2025,
-    cudaReconstHost.cpp:775-777).  Returns a [K, N, N] float32 device tensor."""
+    cudaReconstHost.cpp:775-777): the sinogram is filtered once, at the centre of
+    median integer part, and every other centre back-projects those rows shifted by
+    its integer centre offset.  Returns a [K, N, N] float32 device tensor."""
     import torch
     dtype = "i16" if sino.dtype == torch.int16 else "f32"
     plan_kw = {k: kw.pop(k) for k in list(kw) if k in ("interp", "store", "persistent", "tile",
                                                        "angles_per_chunk", "lds_half", "normalize",
                                                        "exact_filter")}
     plan_kw.setdefault("persistent", True)
-    specs = [SliceSpec(sino.shape[1], fdeg, float(c), sino_dtype=dtype, **kw) for c in centres]
+    centres = [float(c) for c in centres]
+    order = sorted(range(len(centres)), key=lambda k: int(centres[k]))
+    src = order[len(order) // 2]                    # the rows are filtered at this centre
+    seq = [src] + [k for k in range(len(centres)) if k != src]
+    specs = [SliceSpec(sino.shape[1], fdeg, centres[k], sino_dtype=dtype, rows_from=(0 if j else -1), **kw)
+             for j, k in enumerate(seq)]
     plan = FbpPlan(specs, **plan_kw)
     n = plan.out_n(0)
     out = torch.empty(len(specs), n, n, device=sino.device)
-    src = sino.contiguous()
-    plan.execute([src] * len(specs), list(out.unbind(0)), stream=stream)
+    rows = sino.contiguous()
+    plan.execute([rows] * len(specs), [out[k] for k in seq], stream=stream)
     torch.cuda.synchronize()
     plan.close()
     return out

@@ -482,21 +482,38 @@ def test_plan_zoomed_roi():
     assert np.abs(roi2[::2, ::2] - full[y0:y0 + 32, x0:x0 + 32]).max() <= 1e-5 * rng
 
 
-def test_centre_sweep_matches_single_and_finds_axis():
-    """Batched axis scan equals one plan per centre; the total variation of
-    the image is smallest at the true rotation axis (the usual criterion)."""
+def test_centre_sweep_filters_once_and_finds_axis():
+    """Batched axis scan from one filtered sinogram: the rows are filtered at one
+    centre and every centre back-projects them shifted by its integer centre
+    offset, as the reference's kept-sinogram path does (cudaReconstHost.cpp:775-777).
+    Each image equals FP64 filtering at that centre + FP64 back-projection with the
+    offset; centres with the filtering centre's integer part equal their own plan bit
+    for bit; the total variation is smallest at the true rotation axis."""
     W, P = 192, 180
     s, fdeg, c = _sino(W, P, 321, c_off=2.6)
     sino = torch.from_numpy(s).cuda()
     centres = [c + d for d in (-2.0, -1.0, -0.5, 0.0, 0.5, 1.0, 2.0)]
     sweep = fbp.centre_sweep(sino, fdeg, centres).cpu().numpy()
-    for k in (0, 3, 6):
-        _, (one,) = _run([fbp.SliceSpec(W, fdeg, centres[k])], [torch.from_numpy(s)], persistent=True)
-        assert np.array_equal(sweep[k], one)
-    m = _circle(W, 0.4)
-    tv = [np.abs(np.diff(img, axis=0))[m[1:]].sum() + np.abs(np.diff(img, axis=1))[m[:, 1:]].sum()
+    g_src = O.geometry(W, 2, centres[3])                 # the median integer part: 98.6
+    px64 = O.filter_rows_f64(s, g_src, O.set_filter(g_src.ndim, 0))
+    m = _circle(W)
+    for k, ck in enumerate(centres):
+        g = O.geometry(W, 2, ck)
+        ref = O.bp_rows_linear_f64(px64, fdeg, g, icoff=g.ic - g_src.ic)
+        rms, _ = _errors(sweep[k], ref, m)
+        assert rms <= 2e-6, (k, rms)
+        if g.ic == g_src.ic:
+            _, (one,) = _run([fbp.SliceSpec(W, fdeg, ck)], [torch.from_numpy(s)], persistent=True)
+            assert np.array_equal(sweep[k], one), k
+    m4 = _circle(W, 0.4)
+    tv = [np.abs(np.diff(img, axis=0))[m4[1:]].sum() + np.abs(np.diff(img, axis=1))[m4[:, 1:]].sum()
           for img in sweep]
     assert int(np.argmin(tv)) == 3
+    # sharing rows needs the same detector and projections
+    with pytest.raises(RuntimeError, match="rows_of"):
+        fbp.FbpPlan([fbp.SliceSpec(W, fdeg, c), fbp.SliceSpec(W + 8, fdeg, c, rows_from=0)])
+    with pytest.raises(RuntimeError, match="rows_of"):
+        fbp.FbpPlan([fbp.SliceSpec(W, fdeg, c, rows_from=0)])
 
 
 def test_plan_edge_cases():

@@ -7,7 +7,9 @@ C1: one slice, 900 projections x 1024 detector samples (float32 analytic
     oracle: RMS and max reported.
 C2: 16 slices of one geometry at 1800 x 2048 -> 2048^2, grouped as the plan
     does (FP32 slice pairs, FP16 pairs eight per work item), against FP64
-    filtering + FP64 linear back-projection on sampled rows.
+    filtering + FP64 linear back-projection on sampled rows; and the benchmark's
+    own 64-slice textured stack, eight slices against FP64 and one in LUT16
+    float against R-f32 over the whole image.
 C4: one 4096^2 slice from 3600 projections x 4096 samples with 1e5-photon
     Poisson noise, and a 4x zoomed 1024^2 off-centre sub-region; filtered
     sinograms stored as FP32, FP16 and BF16 (per-row scale, FP32 accumulation).
@@ -110,6 +112,49 @@ def test_c2_grouped_parity():
         print(f"C2 grouped {store}{'+lds_half' if lds_half else ''}: worst slice rms {worst:.2e} of range",
               flush=True)
         plan.close()
+
+
+def test_c2_stack_parity():
+    """The headline C2 inputs themselves (bench.make_stack: a 64-slice stack of a
+    40-ellipsoid phantom with the 2% white-noise texture and Poisson noise at 1e4
+    photons), reconstructed the way the benchmark does (FP32 slice pairs, tuned tile
+    and chunk, one hipGraph): eight slices across the stack against FP64 filtering +
+    FP64 linear back-projection on sampled rows, RMS <= 2e-6 of the range; and one
+    slice in LUT16 float against the reference FP32 path R-f32 (projx64 index rule,
+    CPU LUT formula) over the whole image, max <= 1e-5 of the range."""
+    import bench
+    from recview_amd import autotune
+    W, P, S = 2048, 1800, 64
+    sinos, fdeg, c_det = bench.make_stack(S, P, W, 1002, torch.device("cuda"))
+    specs = [fbp.SliceSpec(W, fdeg, c_det) for _ in range(S)]
+    autotune.apply(specs, [W] * S, "linear2")
+    plan = fbp.FbpPlan(specs, persistent=True)
+    outs = [torch.empty(W, W, device="cuda") for _ in range(S)]
+    plan.execute([sinos[i] for i in range(S)], outs, graph=True)
+    torch.cuda.synchronize()
+    plan.close()
+    g = O.geometry(W, 2, c_det)
+    filt = O.set_filter(g.ndim, 0)
+    rows = np.linspace(0.1 * W, 0.9 * W, 5).astype(int)
+    ys, xs = np.meshgrid(rows, np.arange(W), indexing="ij")
+    inside = ((xs - W / 2) ** 2 + (ys - W / 2) ** 2) <= (0.45 * W) ** 2
+    worst = 0.0
+    for i in range(0, S, 9):
+        px64 = O.filter_rows_f64(sinos[i].cpu().numpy(), g, filt)
+        ref = O.bp_points_linear_f64(px64, fdeg, g, xs[inside], ys[inside])
+        rms, mx = _errors(outs[i].cpu().numpy()[ys[inside], xs[inside]], ref)
+        worst = max(worst, rms)
+        assert rms <= 2e-6, (i, rms, mx)
+    print(f"C2 stack (texture): worst sampled-slice rms {worst:.2e} of range vs FP64", flush=True)
+    s_np = sinos[S // 2].cpu().numpy()
+    spec = fbp.SliceSpec(W, fdeg, c_det)
+    _, (lut,) = _run([spec], [sinos[S // 2]], interp="lut16_f32", lut_formula=0, index_rule=0)
+    ref = O.recon_rf32(s_np, fdeg, g, filt, rule=0)
+    m = _circle(W)
+    rng = ref[m].max() - ref[m].min()
+    mxl = float(np.abs(lut.cpu().numpy() - ref)[m].max() / rng)
+    print(f"C2 slice {S // 2} LUT16_F32 vs R-f32: max {mxl:.2e} of range", flush=True)
+    assert mxl <= 1e-5
 
 
 def test_c4_precision_report():

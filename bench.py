@@ -441,6 +441,8 @@ def main():
                          "pairs, eight with FP16 pairs; or 1, 2, 4, 8)")
     ap.add_argument("--graph", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--texture", type=float, default=0.02,
+                    help="c2: white-noise texture of the phantom (fraction of the attenuation; 0 = none)")
     ap.add_argument("--d2h", type=int, default=1,
                     help="c2: also time steps that download every image to pinned host memory")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -466,7 +468,7 @@ def main():
     odt = torch.int32 if args.interp == "lut16_i32" else torch.float32
     if args.workload == "c2":
         S, P, W = (args.slices or 64), args.proj, args.width
-        sinos, fdeg, c_det = make_stack(S, P, W, 1002, dev)
+        sinos, fdeg, c_det = make_stack(S, P, W, 1002, dev, texture=args.texture)
         if args.interp == "lut16_i32":
             from recview_amd import phantom
             sinos = phantom.to_int16(sinos)

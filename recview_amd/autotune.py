@@ -87,14 +87,34 @@ def chunk_for(N, P, mode="linear2", table=None):
     return int(e.get("chunk", 0)) if e else 0
 
 
+def _class_mode(mode, count):
+    """The kernel a class of `count` slices of one geometry runs under a plan mode."""
+    if mode in ("h16", "h8", "h4"):
+        top = GROUP[mode]
+        for m in ("h16", "h8", "h4"):
+            if GROUP[m] <= top and count >= GROUP[m]:
+                return m
+        return "h4"
+    if mode == "linear2":
+        return "linear2" if count >= 2 else "linear"
+    return mode
+
+
 def apply(specs, sides, mode, table=None):
     """Sets tile and chunk of every slice spec from the table: sides[i] is the image
     side slice i reconstructs (2W for an offset-axis scan), its projection count the
-    rows it reconstructs."""
+    rows it reconstructs.  Slices of one geometry form a class; a class takes the
+    entry of the kernel its size leads the plan to (sixteen, eight or four FP16-pair
+    slices per item, FP32 pairs or singles)."""
     t = load() if table is None else table
+    key = lambda s: (s.W, s.P, s.center, s.iInterpolation, s.offset_ct, s.out_n, s.tilt,
+                     s.fdeg.tobytes())
+    counts = {}
+    for spec in specs:
+        counts[key(spec)] = counts.get(key(spec), 0) + 1
     for spec, N in zip(specs, sides):
         P = (spec.P + 1) // 2 if spec.offset_ct else spec.P
-        e = _nearest(N, P, mode, t)
+        e = _nearest(N, P, _class_mode(mode, counts[key(spec)]), t)
         if e:
             spec.tile = int(e["tile"])
             spec.chunk = int(e.get("chunk", 0))

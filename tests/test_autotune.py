@@ -39,14 +39,31 @@ def test_mode_of_plan_options():
 def test_apply_sets_specs():
     import numpy as np
     fd = np.arange(450, dtype=np.float32) * 0.4
-    specs = [fbp.SliceSpec(2048, np.arange(1800, dtype=np.float32) * 0.1, 1024.37),
-             fbp.SliceSpec(256, np.arange(901, dtype=np.float32) * 0.4, 60.0, offset_ct=True),
-             fbp.SliceSpec(512, fd, 256.37)]
-    autotune.apply(specs, [2048, 512, 512], "h8", TABLE)
+    specs = ([fbp.SliceSpec(2048, np.arange(1800, dtype=np.float32) * 0.1, 1024.37) for _ in range(8)] +
+             [fbp.SliceSpec(256, np.arange(901, dtype=np.float32) * 0.4, 60.0, offset_ct=True) for _ in range(8)] +
+             [fbp.SliceSpec(512, fd, 256.37) for _ in range(8)])
+    autotune.apply(specs, [2048] * 8 + [512] * 16, "h8", TABLE)
     assert (specs[0].tile, specs[0].chunk) == (0, 24)
-    assert (specs[1].tile, specs[1].chunk) == (2, 16)      # 451 rows after stitching
-    assert (specs[2].tile, specs[2].chunk) == (2, 16)
+    assert (specs[8].tile, specs[8].chunk) == (2, 16)      # 451 rows after stitching
+    assert (specs[16].tile, specs[16].chunk) == (2, 16)
     assert specs[0].to_c().chunk == 24
+
+
+def test_apply_picks_the_kernel_of_each_class_size():
+    """Classes of one geometry take the entry of the kernel their size leads to:
+    sixteen FP16 slices, eight, fewer; FP32 pairs, a single."""
+    import numpy as np
+    table = dict(TABLE)
+    table["h4:2048:1800"] = {"tile": 3, "chunk": 32}
+    table["linear:2048:1800"] = {"tile": 2, "chunk": 48}
+    fd = np.arange(1800, dtype=np.float32) * 0.1
+    mk = lambda c, n: [fbp.SliceSpec(2048, fd, c) for _ in range(n)]
+    for n, want in ((16, (2, 16)), (9, (0, 24)), (3, (3, 32))):
+        table["h16:2048:1800"] = {"tile": 2, "chunk": 16}
+        specs = autotune.apply(mk(1024.37, n), [2048] * n, "h16", table)
+        assert (specs[0].tile, specs[0].chunk) == want, n
+    specs = autotune.apply(mk(1024.37, 2) + mk(1030.0, 1), [2048] * 3, "linear2", table)
+    assert [(s.tile, s.chunk) for s in specs] == [(1, 16), (1, 16), (2, 48)]
 
 
 def test_shipped_table_is_per_mode():

@@ -298,9 +298,11 @@ typedef struct {
                           * with FP32 pairs in LDS (results identical to one per item), sixteen with
                           * FP16 pairs (lds_half: one FP16 dot product per update and one
                           * power-of-two scale per work item; eight on the 64x64 tile, which has
-                          * no sixteen-slice kernel); or 1, 2, 4, 8, 16 (4, 8 and 16 only
-                          * with FP16 pairs; 8 on the 64x64, 32x32 and 32x64 tiles, 16 on the 32x32
-                          * and 32x64 tiles, else fewer) */
+                          * no sixteen-slice kernel); or 1, 2, 4, 8, 16, 32 (4 and up only with
+                          * FP16 pairs; 8 on the 64x64, 32x32 and 32x64 tiles, 16 on the 32x32
+                          * and 32x64 tiles, else fewer; 32: the matrix-core kernel, which runs
+                          * 32x32 tiles whatever the slice's tile, for every 32 slices of one
+                          * geometry) */
 } RvOptions;
 
 typedef struct rv_plan_opaque* rv_plan_t;

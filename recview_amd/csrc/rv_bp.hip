@@ -673,6 +673,264 @@ __global__ void __launch_bounds__(256) pack_kernel(const PackJob* jobs)
     }
 }
 
+// ------------------------------------------------ matrix-core back-projection
+// BP_LINEAR_M32: 32 slices of one geometry per work item, FP16 samples, FP32
+// sums, the multiply-adds on the MFMA units.  For one angle and one block of
+// 8 x 4 pixels, linear interpolation of 32 slices is a matrix product over a
+// window of 16 detector samples j0 .. j0 + 15:
+//
+//   D[slice][pixel] += sum_k A[slice][k] * B[k][pixel],
+//   A[slice][k] = p_slice(j0 + k),  B[k][pixel] = max(0, 1 - |u_pixel - j0 - k|),
+//
+// i.e. 1 - f at the sample below the pixel's coordinate u, f = fract(u) at the
+// one above, zero elsewhere: one v_mfma_f32_32x32x16_f16 does 1024 updates.  The
+// 32 pixels of a block (8 x 4) span at most 7|cos| + 3|sin| <= 7.62 samples, so
+// with j0 = floor(min u) rounded down to a multiple of 4 the window always holds
+// both neighbours of every pixel.  The vector ALU only builds B: the pixel's
+// coordinate, offset so that it lies on a 2^-11 grid (t in [4096, 8192)), gives
+// f exactly in FP16 with (1 - f) + f = 1, and the pair (1 - f, f) is placed at its
+// position with four byte permutes whose selectors come from a 22-entry table
+// indexed by the integer part of t (every position a lane can meet, so no clamp).
+// Per lane the MFMA operands hold eight consecutive k (lanes 0-31: k 0-7, lanes
+// 32-63: k 8-15), A for slice = lane % 32 and B for pixel = lane % 32; the
+// accumulator of lane l, register r is slice 8 (r / 4) + 4 (l / 32) + r % 4 of
+// pixel l % 32.
+//
+// A 32 x 32 tile is 32 blocks; eight waves own four blocks each (a 32 x 4
+// strip), 64 FP32 sums per lane, so that four waves fit per SIMD (with eight
+// blocks per wave, 213 VGPRs allowed two, and the staging phases were not
+// hidden: 28.9 ms for the 64-slice 2048^2 x 1800 stack).  Per chunk of angles: (1) per (angle, block)
+// the window start and the coordinate offset, in FP64 per angle as the other
+// kernels do; (2) the 32 slices' segments of 64 samples staged into LDS (one
+// 16-byte chunk of eight samples per thread and angle, rows 136 B apart, so the
+// 8-byte LDS reads of 16 lanes fall in distinct banks); (3) per angle and block
+// one MFMA.  Samples outside the detector are zero (the row is zero-extended).
+namespace m32 {
+constexpr int kTile = 32;
+constexpr int kWaves = 8;
+constexpr int kThreads = 64 * kWaves;
+constexpr int kBlocksPerWave = 32 / kWaves;
+constexpr int kChunks = 8;                 // staged 8-sample chunks per angle and slice
+constexpr int kPitch = 136;                // LDS bytes per staged row
+constexpr int kMaxJ0 = kChunks * 8 - 16;   // window start bound (the analysis gives <= 44)
+constexpr int kShift = 4112;               // coordinate offset: t in [4104, 4128)
+constexpr int kSel = 22;                   // selector entries: lane-local positions -9 .. 12
+constexpr int kItem = 16 * kSel;           // the work item's index (after the selector table)
+constexpr int kHead = kItem + 16;          // per-angle tables from here
+}
+
+typedef _Float16 half8v __attribute__((ext_vector_type(8)));
+typedef float float16v __attribute__((ext_vector_type(16)));
+
+size_t bp_m32_lds_bytes(int chunk)
+{
+    using namespace m32;
+    return kHead + 16 * (size_t)chunk + (((size_t)4 * chunk + 15) & ~(size_t)15) + 512 * (size_t)chunk +
+           (size_t)32 * kPitch * chunk;
+}
+
+__global__ void __launch_bounds__(m32::kThreads, 4) bp_m32_kernel(BpLaunch L)
+{
+    using namespace m32;
+    const int CH = L.chunk;
+    extern __shared__ float4 lds4[];
+    unsigned char* lds = (unsigned char*)lds4;
+    uint4* sel = (uint4*)lds;
+    float4* ang = (float4*)(lds + kHead);                        // [CH] (o, cos, sin, s0)
+    int* rowv = (int*)(lds + kHead + 16 * CH);                   // [CH] row index
+    const int blk_off = kHead + 16 * CH + ((4 * CH + 15) & ~15);
+    float4* blk = (float4*)(lds + blk_off);                      // [CH][32] (D, 2 j0, D - 8, 2 j0 + 16)
+    const int data_off = blk_off + 512 * CH;                     // [CH][32 slices][kPitch]
+    int* item_sh = (int*)(lds + kItem);       // in the dynamic block, so that it starts at LDS address 0
+
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int h = lane >> 5, jl = lane & 31;
+    const float xl = (float)(jl & 7), yl = (float)(jl >> 3);
+
+    // selectors of v_perm (0x0c: zero byte): entry e places (1 - f, f) at halves
+    // (p, p + 1) of the lane's eight, p = e - 9 (zero where neither falls inside)
+    if (tid < kSel) {
+        const unsigned int Z = 0x0c0c0c0cu, LO = 0x03020100u, SHL = 0x01000c0cu, SHR = 0x0c0c0302u;
+        unsigned int d[4] = {Z, Z, Z, Z};
+        const int p = tid - 9;
+        if (p == -1) d[0] = SHR;
+        if (p >= 0 && p <= 7) {
+            if ((p & 1) == 0) d[p >> 1] = LO;
+            else {
+                d[p >> 1] = SHL;
+                if ((p >> 1) + 1 < 4) d[(p >> 1) + 1] = SHR;
+            }
+        }
+        sel[tid] = make_uint4(d[0], d[1], d[2], d[3]);
+    }
+
+    int iter = 0;
+    while (true) {
+        int item;
+        if (L.counter) {
+            if (tid == 0) *item_sh = (int)atomicAdd(L.counter, 1u);
+            __syncthreads();
+            item = *item_sh;
+            __syncthreads();
+        } else {
+            item = (iter == 0) ? (int)blockIdx.x : L.total_items;
+            __syncthreads();                   // the selector table is written
+        }
+        iter++;
+        if (item >= L.total_items) break;
+        if (L.item_order) item = L.item_order[item];
+        int lo = 0, hi = L.nslices - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (L.slices[mid].tile_base <= item) lo = mid; else hi = mid - 1;
+        }
+        const BpSlice& S = L.slices[lo];
+        const int t = item - S.tile_base;
+        const int tiles_x = S.tiles_x;
+        const int X0 = (t % tiles_x) * kTile, Y0 = (t / tiles_x) * kTile;
+        const int NCH = (int)S.row_stride;
+        const uint4* packed = (const uint4*)S.rows;
+        const int nang = S.nang;
+
+        float16v acc[kBlocksPerWave];
+#pragma unroll
+        for (int b = 0; b < kBlocksPerWave; b++)
+#pragma unroll
+            for (int r = 0; r < 16; r++) acc[b][r] = 0.f;
+
+        const int w0 = L.ang_end > 0 ? min(L.ang_begin, nang) : 0;
+        const int w1 = L.ang_end > 0 ? min(L.ang_end, nang) : nang;
+        for (int a0 = w0; a0 < w1; a0 += CH) {
+            // ---- (1) per (angle, block): window start and coordinate offset
+            for (int idx = tid; idx < CH * 32; idx += kThreads) {
+                const int a = idx >> 5, b = idx & 31;
+                const int ga = a0 + a;
+                float o = 0.f, c = 0.f, s = 0.f;
+                int s0 = -(1 << 28), row = -1;
+                if (ga < w1) {
+                    const AngleEntry e = S.ang[ga];
+                    const double ut = e.u0 + (double)X0 * e.dux + (double)Y0 * e.duy;
+                    const double umin = ut + fmin(0.0, (kTile - 1) * e.dux) + fmin(0.0, (kTile - 1) * e.duy);
+                    s0 = ((int)floor(umin - 1.0)) & ~7;
+                    o = (float)(ut - (double)s0);
+                    c = (float)e.dux;
+                    s = (float)e.duy;
+                    row = e.row;
+                }
+                const float off = fmaf((float)(4 * (b >> 2)), s, (float)(8 * (b & 3)) * c);
+                const float umin_b = o + off + fminf(0.f, 7.f * c) + fminf(0.f, 3.f * s);
+                int j0 = ((int)floorf(umin_b - 0.015625f)) & ~3;
+                j0 = min(max(j0, 0), kMaxJ0);
+                const float D = (float)(kShift - j0) + off;
+                blk[idx] = make_float4(D, __int_as_float(2 * j0), D - 8.f, __int_as_float(2 * j0 + 16));
+                if (b == 0) {
+                    ang[a] = make_float4(o, c, s, __int_as_float(s0));
+                    rowv[a] = row;
+                }
+            }
+            __syncthreads();
+
+            // ---- (2) stage the slices' segments: thread = (slice, 8-sample chunk, angle parity)
+            {
+                const int sl = tid & 31, cc = (tid >> 5) & 7;
+                for (int a = tid >> 8; a < CH; a += kThreads / 256) {
+                    const int row = rowv[a];
+                    const int gc = (__float_as_int(ang[a].w) >> 3) + cc;
+                    uint4 v = make_uint4(0u, 0u, 0u, 0u);
+                    if (row >= 0 && gc >= 0 && gc < NCH) v = packed[((long long)row * NCH + gc) * 32 + sl];
+                    uint2* dst = (uint2*)(lds + data_off + (a * 32 + sl) * kPitch + cc * 16);
+                    dst[0] = make_uint2(v.x, v.y);
+                    dst[1] = make_uint2(v.z, v.w);
+                }
+            }
+            __syncthreads();
+
+            // ---- (3) one MFMA per angle and block
+            const unsigned char* arow = lds + data_off + jl * kPitch;
+            // selector entry of integer part i at selb[i] (one shift-add per lookup)
+            const uint4* selb = sel - (kShift - 9);
+            const float2* bq = (const float2*)blk + 2 * kBlocksPerWave * wave + h;
+            for (int a = 0; a < CH; a++) {
+                const float4 g = ang[a];
+                const float lc = fmaf(yl, g.z, fmaf(xl, g.y, g.x));
+                const unsigned char* aa = arow + a * 32 * kPitch;
+#pragma unroll
+                for (int b = 0; b < kBlocksPerWave; b++) {
+                    const float2 q = bq[2 * (a * 32 + b)];
+                    const float tt = lc + q.x;                       // kShift + u - j0 - 8 h, on the 2^-11 grid
+                    const float f = __builtin_amdgcn_fractf(tt);
+                    // entry (int)tt - (kShift - 9): tt - kShift lies in [-8, 12) (window bounds)
+                    const uint4 sv = selb[(int)tt];
+                    const unsigned int P = __builtin_bit_cast(unsigned int, __builtin_amdgcn_cvt_pkrtz(1.f - f, f));
+                    const uint4 bw = make_uint4(__builtin_amdgcn_perm(P, P, sv.x), __builtin_amdgcn_perm(P, P, sv.y),
+                                                __builtin_amdgcn_perm(P, P, sv.z), __builtin_amdgcn_perm(P, P, sv.w));
+                    const uint2* ap = (const uint2*)(aa + __float_as_int(q.y));
+                    const uint2 av0 = ap[0], av1 = ap[1];
+                    const half8v A = __builtin_bit_cast(half8v, make_uint4(av0.x, av0.y, av1.x, av1.y));
+                    const half8v B = __builtin_bit_cast(half8v, bw);
+                    acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(A, B, acc[b], 0, 0, 0);
+                }
+            }
+            __syncthreads();
+        }
+
+        // ---- write: lane l, register r holds slice 8 (r / 4) + 4 h + r % 4 of pixel jl
+        const float scale = *S.gscale * S.out_scale;
+        const int out_n = S.out_n;
+#pragma unroll
+        for (int b = 0; b < kBlocksPerWave; b++) {
+            const int bb = kBlocksPerWave * wave + b;
+            const int X = X0 + 8 * (bb & 3) + (jl & 7);
+            const int Y = Y0 + 4 * (bb >> 2) + (jl >> 3);
+            if (X >= out_n || Y >= out_n) continue;
+            const long long o = (long long)Y * out_n + X;
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                const int sl0 = 8 * (r >> 2) + (r & 3);
+                float* o0 = (float*)(sl0 == 0 ? S.out : S.out_x[sl0 - 1]);
+                float* o1 = (float*)S.out_x[sl0 + 3];
+                float* out = h ? o1 : o0;
+                const float v = acc[b][r] * scale;
+                out[o] = S.accumulate ? out[o] + v : v;
+            }
+        }
+        if (!L.counter) break;
+    }
+}
+
+// BP_LINEAR_M32 rows: samples p_k of 32 slices times row scale / item scale in
+// FP16, chunks of eight samples with the slices side by side; lanes = slices,
+// so a wave writes 1 KB contiguously.
+template <int STORE>
+__global__ void __launch_bounds__(256) pack_m32_kernel(const PackJob* jobs)
+{
+    const PackJob* J = jobs + blockIdx.y;
+    const int N = J->N;
+    const int NCH = (int)bp_m32_chunks(N);
+    const int nrows = J->nrows;
+    const float inv = 1.f / *J->gscale;
+    const int s = threadIdx.x & 31;
+    const void* rows = J->rows[s];
+    const float* scales = J->scales[s];
+    for (int row = blockIdx.x; row < nrows; row += gridDim.x) {
+        const float mul = scales[row] * inv;
+        const long long r0 = (long long)row * N;
+        uint4* out = (uint4*)J->out + (long long)row * NCH * 32;
+        for (int c = threadIdx.x >> 5; c < NCH; c += 8) {
+            unsigned int w[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int k = 8 * c + 2 * q;
+                const float v0 = k < N ? stored_val<STORE>(rows, r0 + k) * mul : 0.f;
+                const float v1 = k + 1 < N ? stored_val<STORE>(rows, r0 + k + 1) * mul : 0.f;
+                w[q] = __builtin_bit_cast(unsigned int, __halves2half2(__float2half_rn(v0), __float2half_rn(v1)));
+            }
+            out[c * 32 + s] = make_uint4(w[0], w[1], w[2], w[3]);
+        }
+    }
+}
+
 template <int KIND>
 static void launch_pack_kind(const PackJob* d_jobs, int njobs, int store, int max_rows, hipStream_t stream)
 {
@@ -692,6 +950,14 @@ void launch_pack_scales(const PackJob* d_jobs, int njobs, hipStream_t stream)
 void launch_pack(const PackJob* d_jobs, int njobs, int kind, int store, int max_rows, hipStream_t stream)
 {
     if (njobs <= 0 || max_rows <= 0) return;
+    if (kind == 32) {
+        int bx = 16384 / njobs;
+        bx = bx < 1 ? 1 : bx > max_rows ? max_rows : bx;
+        const dim3 g((unsigned)bx, (unsigned)njobs);
+        if (store == 1) hipLaunchKernelGGL(pack_m32_kernel<1>, g, dim3(256), 0, stream, d_jobs);
+        else hipLaunchKernelGGL(pack_m32_kernel<2>, g, dim3(256), 0, stream, d_jobs);
+        return;
+    }
     if (kind == 0) launch_pack_kind<0>(d_jobs, njobs, store, max_rows, stream);
     else if (kind == 2) launch_pack_kind<2>(d_jobs, njobs, store, max_rows, stream);
     else if (kind == 4) launch_pack_kind<4>(d_jobs, njobs, store, max_rows, stream);
@@ -745,13 +1011,15 @@ int bp_default_chunk(int mode, int tile)
     if (mode == BP_LINEAR_H4) return (tile == 2) ? 32 : 16;
     if (mode == BP_LINEAR_H8) return (tile == 2) ? 32 : 24;
     if (mode == BP_LINEAR_H16) return 16;
+    if (mode == BP_LINEAR_M32) return 8;
     return (tile == 2) ? 64 : 32;
 }
 
-int bp_max_chunk(int mode) { return mode == BP_LINEAR_H ? 64 : 256; }
+int bp_max_chunk(int mode) { return mode == BP_LINEAR_H ? 64 : mode == BP_LINEAR_M32 ? 16 : 256; }
 
 size_t bp_lds_bytes(int mode, int tile, int chunk)
 {
+    if (mode == BP_LINEAR_M32) return bp_m32_lds_bytes(chunk);
     const int seg = tile_seg(tile);
     const size_t head = (size_t)chunk * 16 + (size_t)((2 * chunk + 3) / 4) * 16;
     return head + (size_t)chunk * entry_bytes(mode, seg);
@@ -764,7 +1032,8 @@ size_t bp_lds_bytes(int mode, int tile, int chunk)
 // well (4 pixels x 16 slices), so that four waves fit per SIMD.
 int bp_threads(int mode, int tile)
 {
-    return ((mode == BP_LINEAR_H8 && tile == 0) || (mode == BP_LINEAR_H16 && tile == 4)) ? 512 : 256;
+    return ((mode == BP_LINEAR_H8 && tile == 0) || (mode == BP_LINEAR_H16 && tile == 4) || mode == BP_LINEAR_M32)
+               ? 512 : 256;
 }
 
 template <int MODE>
@@ -790,6 +1059,7 @@ static const void* kernel_ptr(int tile)
 
 static const void* kernel_for(int mode, int tile)
 {
+    if (mode == BP_LINEAR_M32) return tile == 2 ? (const void*)&bp_m32_kernel : nullptr;
     if (mode == BP_LINEAR) return kernel_ptr<BP_LINEAR>(tile);
     if (mode == BP_LINEAR2) return kernel_ptr<BP_LINEAR2>(tile);
     if (mode == BP_LINEAR_H) return kernel_ptr<BP_LINEAR_H>(tile);
@@ -864,6 +1134,13 @@ void launch_ptr_patch(const PtrPatch& P, hipStream_t stream)
 
 void launch_bp(const BpLaunch& L, int grid, hipStream_t stream)
 {
+    if (L.mode == BP_LINEAR_M32) {
+        if (L.tile != 2 || L.chunk < 1 || L.chunk > 16) return;     // the plan forms no other
+        const size_t lds = bp_m32_lds_bytes(L.chunk);
+        allow_lds((const void*)&bp_m32_kernel, lds);
+        hipLaunchKernelGGL(bp_m32_kernel, dim3(grid), dim3(m32::kThreads), lds, stream, L);
+        return;
+    }
     if (L.mode == BP_LINEAR) launch_mode<BP_LINEAR>(L, grid, stream);
     else if (L.mode == BP_LINEAR2) launch_mode<BP_LINEAR2>(L, grid, stream);
     else if (L.mode == BP_LINEAR_H) launch_mode<BP_LINEAR_H>(L, grid, stream);

@@ -117,17 +117,21 @@ inline int output_irad2(int N, double center, int iInterpolation)
 
 // ------------------------------------------------------- back-projection
 // BP_LINEAR2: BP_LINEAR over two slices of identical geometry per work item;
-// BP_LINEAR_H2 / H4 / H8 / H16: BP_LINEAR_H over two / four / eight / sixteen
-// (the others in BpSlice::rows_x / scales_x / out_x).
+// BP_LINEAR_H2 / H4 / H8 / H16: BP_LINEAR_H over two / four / eight / sixteen;
+// BP_LINEAR_M32: 32 FP16 slices per work item, the updates done as matrix
+// products on the MFMA units (the others in BpSlice::rows_x / scales_x / out_x).
 enum BpMode {
     BP_LINEAR = 0, BP_LUT_F32 = 1, BP_LUT_I32 = 2, BP_LINEAR_H = 3, BP_LINEAR2 = 4,
-    BP_LINEAR_H2 = 5, BP_LINEAR_H4 = 6, BP_LINEAR_H8 = 7, BP_LINEAR_H16 = 8
+    BP_LINEAR_H2 = 5, BP_LINEAR_H4 = 6, BP_LINEAR_H8 = 7, BP_LINEAR_H16 = 8, BP_LINEAR_M32 = 9
 };
 constexpr int bp_slices_per_item(int mode)
 {
-    return (mode == 4 || mode == 5) ? 2 : mode == 6 ? 4 : mode == 7 ? 8 : mode == 8 ? 16 : 1;
+    return (mode == 4 || mode == 5) ? 2 : mode == 6 ? 4 : mode == 7 ? 8 : mode == 8 ? 16 : mode == 9 ? 32 : 1;
 }
-constexpr int BP_MAX_SLICES_PER_ITEM = 16;
+constexpr int BP_MAX_SLICES_PER_ITEM = 32;
+// BP_LINEAR_M32 packed rows: per angle row, (N + 7) / 8 chunks of eight samples,
+// each chunk the 32 slices' samples side by side (16 B per slice)
+constexpr long long bp_m32_chunks(int N) { return (N + 7) / 8; }
 
 // Per-angle table entry.  LINEAR: detector coordinate of pixel (ix, iy) of the
 // output grid is u = u0 + ix*dux + iy*duy (all double).  LUT16: the reference
@@ -141,7 +145,7 @@ struct AngleEntry {
 struct BpSlice {
     const void* rows;       // filtered rows (float / half / bf16)
     const float* scales;    // per-row scale (16-bit stores), else nullptr
-    long long row_stride;   // elements between rows
+    long long row_stride;   // elements between rows (BP_LINEAR_M32: 8-sample chunks per row)
     int row_len;            // samples per row (N)
     int nang;               // angles
     const AngleEntry* ang;  // [nang]
@@ -197,13 +201,15 @@ void launch_ptr_patch(const PtrPatch& P, hipStream_t stream);
 //                   last sample);
 //   BP_LINEAR_H2/H4/H8/H16 two / four / eight / sixteen FP16 pairs (p_k, d_k), values multiplied by
 //                   row scale / item scale, where the item scale is the largest
-//                   row scale of the slices over their angle list (a power of two).
+//                   row scale of the slices over their angle list (a power of two);
+//   BP_LINEAR_M32   FP16 samples p_k times row scale / item scale, in chunks of
+//                   eight samples x 32 slices ([row][k / 8][slice][8]), zero past N.
 struct PackJob {
     const void* rows[BP_MAX_SLICES_PER_ITEM];      // filtered rows (float / half / bf16)
     const float* scales[BP_MAX_SLICES_PER_ITEM];   // per-row scales (16-bit stores), else nullptr
     const AngleEntry* ang;  // FP16 pairs: the angle list whose rows set the item scale
     int nang;
-    int ns;                 // slices: 2, or 4 / 8 / 16 (FP16 pairs only)
+    int ns;                 // slices: 2, or 4 / 8 / 16 / 32 (FP16 only)
     int half;               // 1: FP16 pairs
     int store;              // 0 f32, 1 f16, 2 bf16
     int nrows, N;           // rows of N samples, row stride N
@@ -212,7 +218,7 @@ struct PackJob {
 };
 // Item scales of FP16-pair jobs (jobs of other kinds are skipped), then the
 // packing of njobs jobs of one kind (0: FP32 pairs; 2 / 4 / 8 / 16: FP16 pairs of
-// that many slices) and one store, in stream order.
+// that many slices; 32: the BP_LINEAR_M32 chunks) and one store, in stream order.
 void launch_pack_scales(const PackJob* d_jobs, int njobs, hipStream_t stream);
 void launch_pack(const PackJob* d_jobs, int njobs, int kind, int store, int max_rows, hipStream_t stream);
 

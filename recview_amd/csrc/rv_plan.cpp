@@ -447,13 +447,14 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
     // FP16 pairs in LDS (classes of equal geometry, in slice order).
     std::vector<std::vector<int>> classes;
     if (opt->slices_per_item != 0 && opt->slices_per_item != 1 && opt->slices_per_item != 2 &&
-        opt->slices_per_item != 4 && opt->slices_per_item != 8 && opt->slices_per_item != 16)
-        return fail(p, "slices_per_item must be 0, 1, 2, 4, 8 or 16");
+        opt->slices_per_item != 4 && opt->slices_per_item != 8 && opt->slices_per_item != 16 &&
+        opt->slices_per_item != 32)
+        return fail(p, "slices_per_item must be 0, 1, 2, 4, 8, 16 or 32");
     // default: two FP32 slices or sixteen FP16-pair slices per item (packed rows;
     // sixteen form on the 32x64 and 32x32 tiles, eight on 64x64; measured per
     // 2048^2 x 1800 slice: FP32 pairs 0.630 ms, FP16 pairs 0.367 / 0.381 / 0.492 ms
     // with sixteen (32x64) / eight (64x64) / four slices per item)
-    const int max_per_item = (p->mode == rv::BP_LINEAR_H) ? 16 : 2;
+    const int max_per_item = (p->mode == rv::BP_LINEAR_H) ? 32 : 2;
     const int per_item = std::min(opt->slices_per_item == 0 ? 16 : opt->slices_per_item, max_per_item);
     if ((p->mode == rv::BP_LINEAR || p->mode == rv::BP_LINEAR_H) && per_item > 1) {
         auto same = [&](int a, int b) {
@@ -501,11 +502,20 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
             const int tile = p->sl[cls[i]].tile;
             const bool tile8 = tile == 0 || tile == 2 || tile == 4;    // eight per item: 64^2, 32^2, 32x64
             const bool tile16 = tile == 2 || tile == 4;                // sixteen: 32^2 and 32x64
-            const int k = (per_item >= 16 && left >= 16 && tile16) ? 16
+            const int k = (per_item >= 32 && left >= 32) ? 32                  // matrix cores, 32x32 tiles
+                        : (per_item >= 16 && left >= 16 && tile16) ? 16
                         : (per_item >= 8 && left >= 8 && tile8) ? 8
                         : (per_item >= 4 && left >= 4) ? 4 : (per_item >= 2 && left >= 2) ? 2 : 1;
             const bool half = (p->mode == rv::BP_LINEAR_H);
-            const int mode = k == 16 ? (int)rv::BP_LINEAR_H16
+            if (k == 32)
+                for (int e = 0; e < k; e++) {
+                    SliceState& T = p->sl[cls[i + e]];
+                    T.tile = 2;
+                    T.tiles_x = (T.out_n + rv::bp_tile_side(2) - 1) / rv::bp_tile_side(2);
+                    T.tiles_y = (T.out_n + rv::bp_tile_h(2) - 1) / rv::bp_tile_h(2);
+                }
+            const int mode = k == 32 ? (int)rv::BP_LINEAR_M32
+                           : k == 16 ? (int)rv::BP_LINEAR_H16
                            : k == 8 ? (int)rv::BP_LINEAR_H8
                            : k == 4 ? (int)rv::BP_LINEAR_H4
                            : k == 2 ? (half ? (int)rv::BP_LINEAR_H2 : (int)rv::BP_LINEAR2)
@@ -553,17 +563,22 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
         !rv::hip_ok(hipMemcpy(p->d_ang, ha.data(), sizeof(rv::AngleEntry) * nang_total, hipMemcpyHostToDevice), "angles"))
         return fail(p, std::string("plan upload failed: ") + RecviewLastMessage());
 
-    // packed rows of the multi-slice work items: FP32 pairs 16 B, FP16 pairs 8 B
-    // (two slices) or 16 B (four slices) per sample
+    // packed rows of the multi-slice work items: FP32 pairs 16 B, FP16 pairs 8 / 16 /
+    // 32 / 64 B per sample (two / four / eight / sixteen slices); 32 FP16 slices in
+    // chunks of eight samples
     auto packed_size = [](int mode) -> size_t {
-        return mode == rv::BP_LINEAR_H16 ? 64 : mode == rv::BP_LINEAR_H8 ? 32
+        return mode == rv::BP_LINEAR_M32 ? 64 : mode == rv::BP_LINEAR_H16 ? 64 : mode == rv::BP_LINEAR_H8 ? 32
              : mode == rv::BP_LINEAR2 || mode == rv::BP_LINEAR_H4 ? 16 : mode == rv::BP_LINEAR_H2 ? 8 : 0;
+    };
+    auto packed_row = [&](int mode, int N) -> size_t {     // bytes per angle row
+        return mode == rv::BP_LINEAR_M32 ? (size_t)rv::bp_m32_chunks(N) * 8 * packed_size(mode)
+                                         : (size_t)N * packed_size(mode);
     };
     size_t packed_bytes = 0, n_half = 0;
     for (const auto& G : p->groups)
         if (packed_size(G.mode))
             for (int s : G.slices) {
-                packed_bytes += align_up((size_t)p->sl[s].nang * p->sl[s].g.N * packed_size(G.mode), 256);
+                packed_bytes += align_up((size_t)p->sl[s].nang * packed_row(G.mode, p->sl[s].g.N), 256);
                 n_half += G.mode != rv::BP_LINEAR2;
             }
     std::vector<rv::PackJob> pack_jobs;
@@ -642,9 +657,9 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
                 pack_jobs.push_back(J);
                 p->pack_half = p->pack_half || J.half;
                 B.rows = J.out;                    // the kernel reads the packed rows
-                B.row_stride = S.g.N;
+                B.row_stride = G.mode == rv::BP_LINEAR_M32 ? rv::bp_m32_chunks(S.g.N) : S.g.N;
                 B.gscale = J.gscale;
-                packed_off += align_up((size_t)S.nang * S.g.N * packed_size(G.mode), 256);
+                packed_off += align_up((size_t)S.nang * packed_row(G.mode, S.g.N), 256);
             }
         }
         if (!rv::hip_ok(hipMalloc(&G.d_slices, sizeof(rv::BpSlice) * G.h.size()), "slice table") ||

@@ -268,6 +268,46 @@ def test_plan_half_sixteen_per_item(store, rms_tol):
         plan.close()
 
 
+@pytest.mark.parametrize("store,rms_tol,persistent", [("f16", 1e-4, True), ("bf16", 2e-3, False)])
+def test_plan_mfma_thirty_two_per_item(store, rms_tol, persistent):
+    """Thirty-two FP16 slices per work item on the matrix cores (BP_LINEAR_M32,
+    32x32 tiles): 35 slices of one geometry form one such item, a pair and a
+    single; the image side (200) leaves the last tile row and column partly
+    outside and 150 angles a partial last chunk.  Every image matches the FP64
+    oracle to the store's tolerance and the one slice per item FP16 path to a
+    fraction of it (inside the reconstruction circle: the matrix kernel
+    interpolates the zero-extended row, the others hold the last sample), also
+    through graph replay."""
+    print(f"case store={store} persistent={persistent}", flush=True)
+    W, P = 200, 150
+    specs, sinos, refs = [], [], []
+    for i in range(35):
+        s, fdeg, c = _sino(W, P, 2100 + i, photons=1e4)
+        specs.append(fbp.SliceSpec(W, fdeg, c))
+        sinos.append(torch.from_numpy(s))
+        g = O.geometry(W, 2, c)
+        refs.append(O.recon_of64(s, fdeg, g, O.set_filter(g.ndim, 0)))
+    m = _circle(W)
+    _, single = _run(specs, sinos, store=store, lds_half=True, slices_per_item=1)
+    plan = fbp.FbpPlan(specs, store=store, lds_half=True, slices_per_item=32, persistent=persistent)
+    assert plan.info(0)["tile_w"] == 32
+    outs = [torch.empty(W, W, device="cuda") for _ in specs]
+    worst = 0.0
+    for graph in (False, True, True):
+        for o in outs:
+            o.fill_(float("nan"))
+        plan.execute([x.cuda() for x in sinos], outs, graph=graph)
+        torch.cuda.synchronize()
+        for i, ref in enumerate(refs):
+            got = outs[i].cpu().numpy()
+            rms, _ = _errors(got, ref, m)
+            diff, _ = _errors(got, single[i].astype(np.float64), m)
+            worst = max(worst, rms)
+            assert rms <= rms_tol and diff <= 0.5 * rms_tol, (graph, i, rms, diff)
+    print(f"worst rms {worst:.2e}")
+    plan.close()
+
+
 @pytest.mark.parametrize("persistent,graph", [(True, False), (False, True)])
 def test_plan_half_groups_mixed_geometries_and_tiles(persistent, graph):
     """One FP16-pair plan holding two geometries on different tiles: nine slices
@@ -301,19 +341,19 @@ def test_plan_half_groups_mixed_geometries_and_tiles(persistent, graph):
 
 
 def test_plan_slices_per_item_options():
-    """slices_per_item: values other than 0, 1, 2, 4, 8, 16 are rejected at plan
-    creation; FP32 plans clamp 4, 8 and 16 to pairs (the FP32 pair kernel), so their
-    images equal the default's exactly."""
+    """slices_per_item: values other than 0, 1, 2, 4, 8, 16, 32 are rejected at plan
+    creation; FP32 plans clamp 4, 8, 16 and 32 to pairs (the FP32 pair kernel), so
+    their images equal the default's exactly."""
     specs, sinos = [], []
     for i in range(4):
         s, fdeg, c = _sino(96, 60, 1700 + i)
         specs.append(fbp.SliceSpec(96, fdeg, c))
         sinos.append(torch.from_numpy(s))
-    for bad in (3, 32, -1):
+    for bad in (3, 64, -1):
         with pytest.raises(RuntimeError):
             fbp.FbpPlan(specs, slices_per_item=bad)
     _, ref = _run(specs, sinos)
-    for k in (4, 8, 16):
+    for k in (4, 8, 16, 32):
         _, got = _run(specs, sinos, slices_per_item=k)
         for i in range(len(specs)):
             assert np.array_equal(got[i], ref[i]), (k, i)

@@ -594,21 +594,22 @@ __device__ __forceinline__ float stored_val(const void* rows, long long i)
 
 // Item scale of the FP16-pair jobs: the largest row scale of the job's slices
 // over its angle list (row scales are powers of two, so 1 / scale is exact).
-__global__ void __launch_bounds__(256) pack_scale_kernel(const PackJob* jobs)
+__global__ void __launch_bounds__(1024) pack_scale_kernel(const PackJob* jobs)
 {
-    __shared__ float red[4];
+    __shared__ float red[16];
     const PackJob& J = jobs[blockIdx.x];
     if (!J.half) return;
     float m = 0.f;
-    for (int a = threadIdx.x; a < J.nang; a += 256) {
-        const int row = J.ang[a].row;
-        for (int e = 0; e < J.ns; e++) m = fmaxf(m, J.scales[e][row]);
+    const int ns = J.ns;
+    for (int i = threadIdx.x; i < J.nang * ns; i += 1024) {
+        const int a = i / ns, e = i - a * ns;
+        m = fmaxf(m, J.scales[e][J.ang[a].row]);
     }
     for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
     __syncthreads();
     if (threadIdx.x == 0) {
-        m = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+        for (int w = 1; w < 16; w++) m = fmaxf(m, red[w]);
         *J.gscale = (m > 0.f) ? m : 1.f;
     }
 }
@@ -851,23 +852,41 @@ __global__ void __launch_bounds__(m32::kThreads, 4) bp_m32_kernel(BpLaunch L)
             // selector entry of integer part i at selb[i] (one shift-add per lookup)
             const uint4* selb = sel - (kShift - 9);
             const float2* bq = (const float2*)blk + 2 * kBlocksPerWave * wave + h;
+            // per angle: the blocks' coordinates, then every selector and sample read,
+            // then the next angle's tables (LDS reads complete in order, so these do not
+            // delay this angle's operands), then the permutes and products
+            float4 gn = ang[0];
+            float2 qn[kBlocksPerWave];
+#pragma unroll
+            for (int b = 0; b < kBlocksPerWave; b++) qn[b] = bq[2 * b];
             for (int a = 0; a < CH; a++) {
-                const float4 g = ang[a];
-                const float lc = fmaf(yl, g.z, fmaf(xl, g.y, g.x));
+                const float lc = fmaf(yl, gn.z, fmaf(xl, gn.y, gn.x));
                 const unsigned char* aa = arow + a * 32 * kPitch;
+                float fr[kBlocksPerWave];
+                uint4 sv[kBlocksPerWave];
+                uint2 av0[kBlocksPerWave], av1[kBlocksPerWave];
 #pragma unroll
                 for (int b = 0; b < kBlocksPerWave; b++) {
-                    const float2 q = bq[2 * (a * 32 + b)];
-                    const float tt = lc + q.x;                       // kShift + u - j0 - 8 h, on the 2^-11 grid
-                    const float f = __builtin_amdgcn_fractf(tt);
+                    const float tt = lc + qn[b].x;                   // kShift + u - j0 - 8 h, on the 2^-11 grid
+                    fr[b] = __builtin_amdgcn_fractf(tt);
                     // entry (int)tt - (kShift - 9): tt - kShift lies in [-8, 12) (window bounds)
-                    const uint4 sv = selb[(int)tt];
+                    sv[b] = selb[(int)tt];
+                    const uint2* ap = (const uint2*)(aa + __float_as_int(qn[b].y));
+                    av0[b] = ap[0];
+                    av1[b] = ap[1];
+                }
+                if (a + 1 < CH) {
+                    gn = ang[a + 1];
+#pragma unroll
+                    for (int b = 0; b < kBlocksPerWave; b++) qn[b] = bq[2 * ((a + 1) * 32 + b)];
+                }
+#pragma unroll
+                for (int b = 0; b < kBlocksPerWave; b++) {
+                    const float f = fr[b];
                     const unsigned int P = __builtin_bit_cast(unsigned int, __builtin_amdgcn_cvt_pkrtz(1.f - f, f));
-                    const uint4 bw = make_uint4(__builtin_amdgcn_perm(P, P, sv.x), __builtin_amdgcn_perm(P, P, sv.y),
-                                                __builtin_amdgcn_perm(P, P, sv.z), __builtin_amdgcn_perm(P, P, sv.w));
-                    const uint2* ap = (const uint2*)(aa + __float_as_int(q.y));
-                    const uint2 av0 = ap[0], av1 = ap[1];
-                    const half8v A = __builtin_bit_cast(half8v, make_uint4(av0.x, av0.y, av1.x, av1.y));
+                    const uint4 bw = make_uint4(__builtin_amdgcn_perm(P, P, sv[b].x), __builtin_amdgcn_perm(P, P, sv[b].y),
+                                                __builtin_amdgcn_perm(P, P, sv[b].z), __builtin_amdgcn_perm(P, P, sv[b].w));
+                    const half8v A = __builtin_bit_cast(half8v, make_uint4(av0[b].x, av0[b].y, av1[b].x, av1[b].y));
                     const half8v B = __builtin_bit_cast(half8v, bw);
                     acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(A, B, acc[b], 0, 0, 0);
                 }
@@ -944,7 +963,7 @@ static void launch_pack_kind(const PackJob* d_jobs, int njobs, int store, int ma
 
 void launch_pack_scales(const PackJob* d_jobs, int njobs, hipStream_t stream)
 {
-    if (njobs > 0) hipLaunchKernelGGL(pack_scale_kernel, dim3((unsigned)njobs), dim3(256), 0, stream, d_jobs);
+    if (njobs > 0) hipLaunchKernelGGL(pack_scale_kernel, dim3((unsigned)njobs), dim3(1024), 0, stream, d_jobs);
 }
 
 void launch_pack(const PackJob* d_jobs, int njobs, int kind, int store, int max_rows, hipStream_t stream)

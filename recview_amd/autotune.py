@@ -4,6 +4,7 @@ The plan picks one back-projection kernel per work-item kind (the mode):
 
   linear2    FP32 filtered rows, two slices of one geometry per work item (default)
   linear     FP32, one slice per work item (slices without a partner of equal geometry)
+  m32        FP16 filtered rows, 32 slices per work item on the matrix cores (32x32 tiles)
   h16        FP16 filtered rows and FP16 pairs in LDS, sixteen slices per work item
   h8         the same with eight slices per work item
   h4         the same with four slices per work item
@@ -11,7 +12,8 @@ The plan picks one back-projection kernel per work-item kind (the mode):
 
 For every (mode, image side N, projections P) the tuner times each valid tile
 configuration (64x64, 64x128, 32x32, 128x64, 32x64; eight-slice items exist on
-64x64, 32x32 and 32x64, sixteen-slice items on 32x32 and 32x64) and each candidate chunk on a batch that fills the GPU, back-projection
+64x64, 32x32 and 32x64, sixteen-slice items on 32x32 and 32x64, 32-slice items on
+32x32 only) and each candidate chunk on a batch that fills the GPU, back-projection
 stage only (pack included), and keeps the fastest.  The table (tuned_tiles.json
 next to this module) is keyed "mode:N:P"; `tile_for` / `chunk_for` return the
 entry of the nearest tuned shape of the same mode (log distance), and `apply`
@@ -24,22 +26,23 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE = os.path.join(HERE, "tuned_tiles.json")
 
-MODES = ("linear2", "linear", "h16", "h8", "h4", "lut16_i32")
-TILES = {"linear2": (0, 1, 2, 3, 4), "linear": (0, 1, 2, 3, 4), "h16": (2, 4), "h8": (0, 2, 4),
+MODES = ("linear2", "linear", "m32", "h16", "h8", "h4", "lut16_i32")
+TILES = {"linear2": (0, 1, 2, 3, 4), "linear": (0, 1, 2, 3, 4), "m32": (2,), "h16": (2, 4), "h8": (0, 2, 4),
          "h4": (0, 1, 2, 3, 4), "lut16_i32": (0, 1, 2, 3, 4)}
-CHUNKS = {"linear2": (8, 16, 24, 32), "linear": (16, 32, 48), "h16": (8, 12, 16), "h8": (16, 24, 32),
-          "h4": (16, 24, 32), "lut16_i32": (16, 32, 48)}
+CHUNKS = {"linear2": (8, 16, 24, 32), "linear": (16, 32, 48), "m32": (8, 12, 16), "h16": (8, 12, 16),
+          "h8": (16, 24, 32), "h4": (16, 24, 32), "lut16_i32": (16, 32, 48)}
 # plan options of each mode (the slices of a tuning batch share one geometry, so
 # they group as the mode says)
 PLAN_KW = {
     "linear2": dict(interp="linear", store="f32", slices_per_item=2),
     "linear": dict(interp="linear", store="f32", slices_per_item=1),
+    "m32": dict(interp="linear", store="f16", lds_half=True, slices_per_item=32),
     "h16": dict(interp="linear", store="f16", lds_half=True, slices_per_item=16),
     "h8": dict(interp="linear", store="f16", lds_half=True, slices_per_item=8),
     "h4": dict(interp="linear", store="f16", lds_half=True, slices_per_item=4),
     "lut16_i32": dict(interp="lut16_i32"),
 }
-GROUP = {"linear2": 2, "linear": 1, "h16": 16, "h8": 8, "h4": 4, "lut16_i32": 1}
+GROUP = {"linear2": 2, "linear": 1, "m32": 32, "h16": 16, "h8": 8, "h4": 4, "lut16_i32": 1}
 
 
 def mode_of(interp="linear", store="f32", lds_half=False, slices_per_item=0):
@@ -49,7 +52,7 @@ def mode_of(interp="linear", store="f32", lds_half=False, slices_per_item=0):
     if interp != "linear":
         return None
     if store != "f32" and lds_half:
-        return {4: "h4", 8: "h8"}.get(slices_per_item, "h16")
+        return {4: "h4", 8: "h8", 16: "h16"}.get(slices_per_item, "m32")
     if store == "f32":
         return "linear" if slices_per_item == 1 else "linear2"
     return None
@@ -89,9 +92,9 @@ def chunk_for(N, P, mode="linear2", table=None):
 
 def _class_mode(mode, count):
     """The kernel a class of `count` slices of one geometry runs under a plan mode."""
-    if mode in ("h16", "h8", "h4"):
+    if mode in ("m32", "h16", "h8", "h4"):
         top = GROUP[mode]
-        for m in ("h16", "h8", "h4"):
+        for m in ("m32", "h16", "h8", "h4"):
             if GROUP[m] <= top and count >= GROUP[m]:
                 return m
         return "h4"
@@ -104,8 +107,8 @@ def apply(specs, sides, mode, table=None):
     """Sets tile and chunk of every slice spec from the table: sides[i] is the image
     side slice i reconstructs (2W for an offset-axis scan), its projection count the
     rows it reconstructs.  Slices of one geometry form a class; a class takes the
-    entry of the kernel its size leads the plan to (sixteen, eight or four FP16-pair
-    slices per item, FP32 pairs or singles)."""
+    entry of the kernel its size leads the plan to (32 FP16 slices per item on the
+    matrix cores, sixteen, eight or four FP16 pairs, FP32 pairs or singles)."""
     t = load() if table is None else table
     key = lambda s: (s.W, s.P, s.center, s.iInterpolation, s.offset_ct, s.out_n, s.tilt,
                      s.fdeg.tobytes())

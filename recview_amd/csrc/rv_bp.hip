@@ -1030,7 +1030,7 @@ int bp_default_chunk(int mode, int tile)
     if (mode == BP_LINEAR_H4) return (tile == 2) ? 32 : 16;
     if (mode == BP_LINEAR_H8) return (tile == 2) ? 32 : 24;
     if (mode == BP_LINEAR_H16) return 16;
-    if (mode == BP_LINEAR_M32) return 8;
+    if (mode == BP_LINEAR_M32) return 16;
     return (tile == 2) ? 64 : 32;
 }
 

@@ -29,7 +29,8 @@ def test_lookup_per_mode():
 def test_mode_of_plan_options():
     assert autotune.mode_of("linear", "f32") == "linear2"
     assert autotune.mode_of("linear", "f32", slices_per_item=1) == "linear"
-    assert autotune.mode_of("linear", "f16", lds_half=True) == "h16"
+    assert autotune.mode_of("linear", "f16", lds_half=True) == "m32"
+    assert autotune.mode_of("linear", "f16", lds_half=True, slices_per_item=16) == "h16"
     assert autotune.mode_of("linear", "f16", lds_half=True, slices_per_item=8) == "h8"
     assert autotune.mode_of("linear", "bf16", lds_half=True, slices_per_item=4) == "h4"
     assert autotune.mode_of("lut16_i32") == "lut16_i32"
@@ -51,17 +52,21 @@ def test_apply_sets_specs():
 
 def test_apply_picks_the_kernel_of_each_class_size():
     """Classes of one geometry take the entry of the kernel their size leads to:
-    sixteen FP16 slices, eight, fewer; FP32 pairs, a single."""
+    32 FP16 slices (matrix cores), sixteen, eight, fewer; FP32 pairs, a single."""
     import numpy as np
     table = dict(TABLE)
     table["h4:2048:1800"] = {"tile": 3, "chunk": 32}
     table["linear:2048:1800"] = {"tile": 2, "chunk": 48}
     fd = np.arange(1800, dtype=np.float32) * 0.1
     mk = lambda c, n: [fbp.SliceSpec(2048, fd, c) for _ in range(n)]
-    for n, want in ((16, (2, 16)), (9, (0, 24)), (3, (3, 32))):
-        table["h16:2048:1800"] = {"tile": 2, "chunk": 16}
-        specs = autotune.apply(mk(1024.37, n), [2048] * n, "h16", table)
-        assert (specs[0].tile, specs[0].chunk) == want, n
+    table["m32:2048:1800"] = {"tile": 2, "chunk": 12}
+    table["h16:2048:1800"] = {"tile": 2, "chunk": 16}
+    for mode in ("m32", "h16"):
+        for n, want in ((40, (2, 12)), (16, (2, 16)), (9, (0, 24)), (3, (3, 32))):
+            if mode == "h16" and n == 40:
+                want = (2, 16)                     # the plan's limit is sixteen
+            specs = autotune.apply(mk(1024.37, n), [2048] * n, mode, table)
+            assert (specs[0].tile, specs[0].chunk) == want, (mode, n)
     specs = autotune.apply(mk(1024.37, 2) + mk(1030.0, 1), [2048] * 3, "linear2", table)
     assert [(s.tile, s.chunk) for s in specs] == [(1, 16), (1, 16), (2, 48)]
 

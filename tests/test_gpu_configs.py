@@ -80,7 +80,8 @@ def _errors(got, ref):
 def test_c2_grouped_parity():
     """C2 geometry (1800 projections x 2048 samples -> 2048^2) with 16 slices of
     one geometry and different noisy phantoms, as the plan groups them: FP32
-    slice pairs (the default) and FP16 pairs eight per work item (lds_half).
+    slice pairs (the default) and FP16 pairs sixteen per work item (lds_half; a
+    class below 32 slices does not reach the matrix-core kernel).
     Every slice against FP64 filtering + FP64 linear back-projection on six rows
     across the reconstruction circle."""
     W, P, S = 2048, 1800, 16
@@ -118,34 +119,42 @@ def test_c2_stack_parity():
     """The headline C2 inputs themselves (bench.make_stack: a 64-slice stack of a
     40-ellipsoid phantom with the 2% white-noise texture and Poisson noise at 1e4
     photons), reconstructed the way the benchmark does (FP32 slice pairs, tuned tile
-    and chunk, one hipGraph): eight slices across the stack against FP64 filtering +
-    FP64 linear back-projection on sampled rows, RMS <= 2e-6 of the range; and one
-    slice in LUT16 float against the reference FP32 path R-f32 (projx64 index rule,
-    CPU LUT formula) over the whole image, max <= 1e-5 of the range."""
+    and chunk, one hipGraph; and its FP16 line: 16-bit filtered rows, two 32-slice
+    items on the matrix cores): eight slices across the stack against FP64 filtering
+    + FP64 linear back-projection on sampled rows, RMS <= 2e-6 (FP32) / 1e-4 (FP16)
+    of the range; and one slice in LUT16 float against the reference FP32 path R-f32
+    (projx64 index rule, CPU LUT formula) over the whole image, max <= 1e-5 of the
+    range."""
     import bench
     from recview_amd import autotune
     W, P, S = 2048, 1800, 64
     sinos, fdeg, c_det = bench.make_stack(S, P, W, 1002, torch.device("cuda"))
-    specs = [fbp.SliceSpec(W, fdeg, c_det) for _ in range(S)]
-    autotune.apply(specs, [W] * S, "linear2")
-    plan = fbp.FbpPlan(specs, persistent=True)
-    outs = [torch.empty(W, W, device="cuda") for _ in range(S)]
-    plan.execute([sinos[i] for i in range(S)], outs, graph=True)
-    torch.cuda.synchronize()
-    plan.close()
     g = O.geometry(W, 2, c_det)
     filt = O.set_filter(g.ndim, 0)
     rows = np.linspace(0.1 * W, 0.9 * W, 5).astype(int)
     ys, xs = np.meshgrid(rows, np.arange(W), indexing="ij")
     inside = ((xs - W / 2) ** 2 + (ys - W / 2) ** 2) <= (0.45 * W) ** 2
-    worst = 0.0
+    refs = {}
     for i in range(0, S, 9):
         px64 = O.filter_rows_f64(sinos[i].cpu().numpy(), g, filt)
-        ref = O.bp_points_linear_f64(px64, fdeg, g, xs[inside], ys[inside])
-        rms, mx = _errors(outs[i].cpu().numpy()[ys[inside], xs[inside]], ref)
-        worst = max(worst, rms)
-        assert rms <= 2e-6, (i, rms, mx)
-    print(f"C2 stack (texture): worst sampled-slice rms {worst:.2e} of range vs FP64", flush=True)
+        refs[i] = O.bp_points_linear_f64(px64, fdeg, g, xs[inside], ys[inside])
+    for store, lds_half, mode, tol in (("f32", False, "linear2", 2e-6), ("f16", True, "m32", 1e-4)):
+        specs = [fbp.SliceSpec(W, fdeg, c_det) for _ in range(S)]
+        autotune.apply(specs, [W] * S, mode)
+        plan = fbp.FbpPlan(specs, store=store, lds_half=lds_half, persistent=True)
+        if lds_half:
+            assert plan.info(0)["tile_w"] == 32                  # the matrix-core items
+        outs = [torch.empty(W, W, device="cuda") for _ in range(S)]
+        plan.execute([sinos[i] for i in range(S)], outs, graph=True)
+        torch.cuda.synchronize()
+        plan.close()
+        worst = 0.0
+        for i, ref in refs.items():
+            rms, mx = _errors(outs[i].cpu().numpy()[ys[inside], xs[inside]], ref)
+            worst = max(worst, rms)
+            assert rms <= tol, (store, i, rms, mx)
+        print(f"C2 stack (texture) {store}: worst sampled-slice rms {worst:.2e} of range vs FP64", flush=True)
+        del outs
     s_np = sinos[S // 2].cpu().numpy()
     spec = fbp.SliceSpec(W, fdeg, c_det)
     _, (lut,) = _run([spec], [sinos[S // 2]], interp="lut16_f32", lut_formula=0, index_rule=0)

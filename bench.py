@@ -431,11 +431,11 @@ def dtype_of(args):
 
 
 def roofline_of(args, S, achieved, traffic):
-    """FP32 kernels against the FP32 vector peak; the matrix-core kernel (FP16 rows,
-    32 or more slices of one geometry) against the dense FP16 MFMA peak.  achieved is
+    """FP32 kernels against the FP32 vector peak; the opt-in matrix-core kernel (FP16
+    rows, --slices-per-item 32) against the dense FP16 MFMA peak.  achieved is
     algorithmic (6 FLOP per update) either way; the MFMA kernel executes 32 FLOP per
     update (16-sample windows), reported beside it."""
-    m32 = (args.store != "f32" and args.lds_half and args.slices_per_item in (0, 32) and S >= 32
+    m32 = (args.store != "f32" and args.lds_half and args.slices_per_item == 32 and S >= 32
            and args.interp == "linear")
     if m32:
         return {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP16_PEAK_TFLOPS, "unit": "TFLOP/s",

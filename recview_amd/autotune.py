@@ -4,7 +4,8 @@ The plan picks one back-projection kernel per work-item kind (the mode):
 
   linear2    FP32 filtered rows, two slices of one geometry per work item (default)
   linear     FP32, one slice per work item (slices without a partner of equal geometry)
-  m32        FP16 filtered rows, 32 slices per work item on the matrix cores (32x32 tiles)
+  m32        FP16 filtered rows, 32 slices per work item on the matrix cores (32x32
+             tiles; opt-in, slices_per_item = 32)
   h16        FP16 filtered rows and FP16 pairs in LDS, sixteen slices per work item
   h8         the same with eight slices per work item
   h4         the same with four slices per work item
@@ -52,7 +53,7 @@ def mode_of(interp="linear", store="f32", lds_half=False, slices_per_item=0):
     if interp != "linear":
         return None
     if store != "f32" and lds_half:
-        return {4: "h4", 8: "h8", 16: "h16"}.get(slices_per_item, "m32")
+        return {4: "h4", 8: "h8", 32: "m32"}.get(slices_per_item, "h16")
     if store == "f32":
         return "linear" if slices_per_item == 1 else "linear2"
     return None

@@ -450,14 +450,13 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
         opt->slices_per_item != 4 && opt->slices_per_item != 8 && opt->slices_per_item != 16 &&
         opt->slices_per_item != 32)
         return fail(p, "slices_per_item must be 0, 1, 2, 4, 8, 16 or 32");
-    // default: two FP32 slices per item, or with FP16 samples 32 per item on the
-    // matrix cores and sixteen / eight / four FP16 pairs for what is left of a class
-    // (packed rows; sixteen form on the 32x64 and 32x32 tiles, eight on 64x64;
-    // measured per 2048^2 x 1800 slice: FP32 pairs 0.630 ms, FP16 0.343 ms with 32
-    // per item, FP16 pairs 0.367 / 0.381 / 0.492 ms with sixteen (32x64) / eight
-    // (64x64) / four per item)
+    // default: two FP32 slices or sixteen FP16-pair slices per item, all on the
+    // vector ALU (packed rows; sixteen form on the 32x64 and 32x32 tiles, eight on
+    // 64x64; measured per 2048^2 x 1800 slice: FP32 pairs 0.630 ms, FP16 pairs
+    // 0.367 / 0.381 / 0.492 ms with sixteen (32x64) / eight (64x64) / four per
+    // item).  32 per item (opt-in) runs the matrix-core kernel: 0.343 ms.
     const int max_per_item = (p->mode == rv::BP_LINEAR_H) ? 32 : 2;
-    const int per_item = std::min(opt->slices_per_item == 0 ? 32 : opt->slices_per_item, max_per_item);
+    const int per_item = std::min(opt->slices_per_item == 0 ? 16 : opt->slices_per_item, max_per_item);
     if ((p->mode == rv::BP_LINEAR || p->mode == rv::BP_LINEAR_H) && per_item > 1) {
         auto same = [&](int a, int b) {
             const SliceState& A = p->sl[a];

@@ -94,10 +94,10 @@ class FbpPlan:
         operation order (bit-identical filtered rows) instead of two rows per transform.
         slices_per_item: LINEAR modes back-project slices of identical geometry
         together (0: two per work item with FP32 LDS pairs -- the same results as
-        one per item -- and with FP16 samples (lds_half) 32 per item on the matrix
-        cores (32x32 tiles), the rest of a class sixteen FP16 pairs on the 32x64 and
-        32x32 tiles, eight on 64x64, four; or 1, 2, 4, 8, 16, 32, where 4 and up need
-        FP16 samples, 16 the 32x32 or 32x64 tile)."""
+        one per item -- and with FP16 pairs (lds_half) sixteen on the 32x64 and
+        32x32 tiles, eight on 64x64; or 1, 2, 4, 8, 16, 32, where 4 and up need
+        FP16 pairs, 16 the 32x32 or 32x64 tile, and 32 selects the matrix-core
+        kernel for every 32 slices of one geometry)."""
         self.slices = list(slices)
         L = _lib.lib()
         specs = (_lib.RvSliceSpec * len(self.slices))(*[s.to_c() for s in self.slices])

@@ -29,8 +29,8 @@ def test_lookup_per_mode():
 def test_mode_of_plan_options():
     assert autotune.mode_of("linear", "f32") == "linear2"
     assert autotune.mode_of("linear", "f32", slices_per_item=1) == "linear"
-    assert autotune.mode_of("linear", "f16", lds_half=True) == "m32"
-    assert autotune.mode_of("linear", "f16", lds_half=True, slices_per_item=16) == "h16"
+    assert autotune.mode_of("linear", "f16", lds_half=True) == "h16"
+    assert autotune.mode_of("linear", "f16", lds_half=True, slices_per_item=32) == "m32"
     assert autotune.mode_of("linear", "f16", lds_half=True, slices_per_item=8) == "h8"
     assert autotune.mode_of("linear", "bf16", lds_half=True, slices_per_item=4) == "h4"
     assert autotune.mode_of("lut16_i32") == "lut16_i32"

@@ -80,8 +80,7 @@ def _errors(got, ref):
 def test_c2_grouped_parity():
     """C2 geometry (1800 projections x 2048 samples -> 2048^2) with 16 slices of
     one geometry and different noisy phantoms, as the plan groups them: FP32
-    slice pairs (the default) and FP16 pairs sixteen per work item (lds_half; a
-    class below 32 slices does not reach the matrix-core kernel).
+    slice pairs (the default) and FP16 pairs sixteen per work item (lds_half).
     Every slice against FP64 filtering + FP64 linear back-projection on six rows
     across the reconstruction circle."""
     W, P, S = 2048, 1800, 16
@@ -119,8 +118,9 @@ def test_c2_stack_parity():
     """The headline C2 inputs themselves (bench.make_stack: a 64-slice stack of a
     40-ellipsoid phantom with the 2% white-noise texture and Poisson noise at 1e4
     photons), reconstructed the way the benchmark does (FP32 slice pairs, tuned tile
-    and chunk, one hipGraph; and its FP16 line: 16-bit filtered rows, two 32-slice
-    items on the matrix cores): eight slices across the stack against FP64 filtering
+    and chunk, one hipGraph; its FP16 line: 16-bit filtered rows, sixteen FP16 pairs
+    per work item; and the opt-in matrix-core kernel, two 32-slice items): eight
+    slices across the stack against FP64 filtering
     + FP64 linear back-projection on sampled rows, RMS <= 2e-6 (FP32) / 1e-4 (FP16)
     of the range; and one slice in LUT16 float against the reference FP32 path R-f32
     (projx64 index rule, CPU LUT formula) over the whole image, max <= 1e-5 of the
@@ -138,11 +138,11 @@ def test_c2_stack_parity():
     for i in range(0, S, 9):
         px64 = O.filter_rows_f64(sinos[i].cpu().numpy(), g, filt)
         refs[i] = O.bp_points_linear_f64(px64, fdeg, g, xs[inside], ys[inside])
-    for store, lds_half, mode, tol in (("f32", False, "linear2", 2e-6), ("f16", True, "m32", 1e-4)):
+    for store, lds_half, spi, tol in (("f32", False, 0, 2e-6), ("f16", True, 0, 1e-4), ("f16", True, 32, 1e-4)):
         specs = [fbp.SliceSpec(W, fdeg, c_det) for _ in range(S)]
-        autotune.apply(specs, [W] * S, mode)
-        plan = fbp.FbpPlan(specs, store=store, lds_half=lds_half, persistent=True)
-        if lds_half:
+        autotune.apply(specs, [W] * S, autotune.mode_of("linear", store, lds_half, spi))
+        plan = fbp.FbpPlan(specs, store=store, lds_half=lds_half, slices_per_item=spi, persistent=True)
+        if spi == 32:
             assert plan.info(0)["tile_w"] == 32                  # the matrix-core items
         outs = [torch.empty(W, W, device="cuda") for _ in range(S)]
         plan.execute([sinos[i] for i in range(S)], outs, graph=True)
@@ -153,7 +153,8 @@ def test_c2_stack_parity():
             rms, mx = _errors(outs[i].cpu().numpy()[ys[inside], xs[inside]], ref)
             worst = max(worst, rms)
             assert rms <= tol, (store, i, rms, mx)
-        print(f"C2 stack (texture) {store}: worst sampled-slice rms {worst:.2e} of range vs FP64", flush=True)
+        print(f"C2 stack (texture) {store} slices_per_item={spi}: worst sampled-slice rms {worst:.2e} "
+              f"of range vs FP64", flush=True)
         del outs
     s_np = sinos[S // 2].cpu().numpy()
     spec = fbp.SliceSpec(W, fdeg, c_det)

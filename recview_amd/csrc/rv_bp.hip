@@ -56,6 +56,10 @@
 namespace rv {
 
 typedef _Float16 half2v __attribute__((ext_vector_type(2)));
+// the tile geometry table, read with global (vector-memory) loads: their counter
+// is separate from the LDS reads', so a load in flight does not hold those up
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef const __attribute__((address_space(1))) f4v* gf4p;
 
 // smallest integer >= sqrt(v)
 constexpr int isqrt_ceil(int v)
@@ -139,7 +143,10 @@ __device__ __forceinline__ float2 linear_pair(const void* rows, const float* sca
     return make_float2(av, dv);
 }
 
-template <int MODE, int PX, int PY, int NW = 4>
+// TG (BP_LINEAR2 only): the per-angle tile geometry comes from BpSlice::tgeom,
+// built once per plan, so the LDS holds only the staged samples and a chunk
+// needs one barrier less.
+template <int MODE, int PX, int PY, int NW = 4, bool TG = false>
 __global__ void __launch_bounds__(64 * NW, NW == 8 ? 4 : 1) bp_tiled_kernel(BpLaunch L)
 {
     using C = TileCfg<PX, PY, NW>;
@@ -172,7 +179,7 @@ __global__ void __launch_bounds__(64 * NW, NW == 8 ? 4 : 1) bp_tiled_kernel(BpLa
         if (L.counter) {
             if (tid == 0) item_sh = (int)atomicAdd(L.counter, 1u);
             __syncthreads();
-            item = item_sh;
+            item = __builtin_amdgcn_readfirstlane(item_sh);     // uniform: slice data in scalar registers
             __syncthreads();
         } else {
             item = (iter == 0) ? (int)blockIdx.x : L.total_items;
@@ -193,8 +200,10 @@ __global__ void __launch_bounds__(64 * NW, NW == 8 ? 4 : 1) bp_tiled_kernel(BpLa
             const int mid = (lo + hi + 1) >> 1;
             if (L.slices[mid].tile_base <= item) lo = mid; else hi = mid - 1;
         }
+        lo = __builtin_amdgcn_readfirstlane(lo);           // uniform: the slice's fields in scalar registers
         const BpSlice S = L.slices[lo];
         const int t = item - S.tile_base;
+        const float4* tgv = TG ? S.tgeom + (size_t)t * S.nang : nullptr;   // this tile's row of the table
         const int X0 = (t % S.tiles_x) * C::TW;
         const int Y0 = (t / S.tiles_x) * C::TH;
         const int N = S.row_len;
@@ -223,8 +232,9 @@ __global__ void __launch_bounds__(64 * NW, NW == 8 ? 4 : 1) bp_tiled_kernel(BpLa
             a_end = min(w1, a_begin + per);
         }
         for (int a0 = a_begin; a0 < a_end; a0 += chunk) {
+            const int na = TG ? min(chunk, a_end - a0) : chunk;     // angles of this chunk
             // ---- per-angle tile geometry
-            if (tid < chunk) {
+            if (!TG && tid < chunk) {
                 const int ga = a0 + tid;
                 if (ga < a_end) {
                     const AngleEntry e = S.ang[ga];
@@ -259,7 +269,7 @@ __global__ void __launch_bounds__(64 * NW, NW == 8 ? 4 : 1) bp_tiled_kernel(BpLa
                     if (tid == 0) chunk_scale = (m > 0.f) ? m : 1.f;
                 }
             }
-            __syncthreads();
+            if (!TG) __syncthreads();
 
             // ---- stage detector segments into LDS
             if (MODE == BP_LINEAR) {
@@ -279,17 +289,32 @@ __global__ void __launch_bounds__(64 * NW, NW == 8 ? 4 : 1) bp_tiled_kernel(BpLa
                     const float fj = -(float)j;
                     quads[a * SEG + j] = make_float4(fmaf(fj, v.y, v.x), v.y, fmaf(fj, v.w, v.z), v.w);
                 };
+                // row and segment origin of angle a of the chunk
+                auto origin = [&](int a, int& row, int& s0) {
+                    if (TG) {
+                        const bool ok = a < na;
+                        row = ok ? S.ang[a0 + a].row : -1;
+                        s0 = ok ? __float_as_int(tgv[a0 + a].w) : 0;
+                    } else {
+                        row = rws[a];
+                        s0 = s0s[a];
+                    }
+                };
                 if (C::T % chunk == 0) {
                     // threads of one angle: its row and segment origin read once per
                     // thread (C2, FP32 pairs: back-projection 40.85 -> 39.87 ms)
                     const int tpa = C::T / chunk;
                     const int a = tid / tpa, j0 = tid - a * tpa;
-                    const int row = rws[a], s0 = s0s[a];
-                    for (int j = j0; j < SEG; j += tpa) stage(a, j, row, s0);
+                    int row, s0;
+                    origin(a, row, s0);
+                    if (!TG || a < na)
+                        for (int j = j0; j < SEG; j += tpa) stage(a, j, row, s0);
                 } else {
-                    for (int idx = tid; idx < chunk * SEG; idx += C::T) {
+                    for (int idx = tid; idx < na * SEG; idx += C::T) {
                         const int a = idx / SEG, j = idx - a * SEG;
-                        stage(a, j, rws[a], s0s[a]);
+                        int row, s0;
+                        origin(a, row, s0);
+                        stage(a, j, row, s0);
                     }
                 }
             } else if (MODE == BP_LINEAR_H) {
@@ -392,8 +417,7 @@ __global__ void __launch_bounds__(64 * NW, NW == 8 ? 4 : 1) bp_tiled_kernel(BpLa
                 }
             } else if (MODE == BP_LINEAR2) {
                 const float4* quads = (const float4*)data;
-                for (int a = 0; a < chunk; a++) {
-                    const float4 tb = tab[a];
+                auto angle = [&](int a, const float4 tb) {
                     const float4* seg = quads + a * SEG * NQ;
                     const float base = fmaf(dyl, tb.y, fmaf(dxl, tb.x, tb.z));
 #pragma unroll
@@ -412,6 +436,23 @@ __global__ void __launch_bounds__(64 * NW, NW == 8 ? 4 : 1) bp_tiled_kernel(BpLa
                             }
                         }
                     }
+                };
+                if constexpr (TG) {
+                    // the next angle's entry loads (vector memory, counted apart from
+                    // the LDS reads) while this angle's updates run.  The compiler
+                    // schedules this form for six waves per SIMD (79 VGPRs, two LDS
+                    // reads in flight per wave); measured faster than forms scheduled
+                    // for four (64-slice C2 stack: 37.8 ms vs 39.4 ms per-chunk geometry,
+                    // 41.2 ms table read at the top of each angle)
+                    const gf4p tgg = (gf4p)tgv;
+                    f4v tn = tgg[a0];
+                    for (int a = 0; a < na; a++) {
+                        const float4 tb = make_float4(tn.x, tn.y, tn.z, tn.w);
+                        if (a + 1 < na) tn = tgg[a0 + a + 1];
+                        angle(a, tb);
+                    }
+                } else {
+                    for (int a = 0; a < na; a++) angle(a, tab[a]);
                 }
             } else if (is_half_multi(MODE)) {
                 // FP16 pairs of 2 / 4 / 8 / 16 slices per sample (8, 16: two / four 16-byte planes).
@@ -582,6 +623,33 @@ __global__ void __launch_bounds__(64 * NW, NW == 8 ? 4 : 1) bp_tiled_kernel(BpLa
         }
         if (!L.counter) break;
     }
+}
+
+// BpSlice::tgeom: the tile origin of every (tile, angle), formed exactly as
+// bp_tiled_kernel forms it per chunk (same FP64 operations, so the images are
+// identical with and without the table).
+__global__ void __launch_bounds__(256) tile_geometry_kernel(const AngleEntry* ang, int nang, int tiles_x, int ntiles,
+                                                            int tw, int th, float4* out)
+{
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (long long)nang * ntiles) return;
+    const int t = (int)(i / nang);
+    const int a = (int)(i - (long long)t * nang);
+    const AngleEntry e = ang[a];
+    const int X0 = (t % tiles_x) * tw, Y0 = (t / tiles_x) * th;
+    const double ut = e.u0 + (double)X0 * e.dux + (double)Y0 * e.duy;
+    const double umin = ut + fmin(0.0, (tw - 1) * e.dux) + fmin(0.0, (th - 1) * e.duy);
+    const int s0 = (int)floor(umin) - 1;
+    out[i] = make_float4((float)e.dux, (float)e.duy, (float)(ut - (double)s0), __int_as_float(s0));
+}
+
+void launch_tile_geometry(const AngleEntry* ang, int nang, int tiles_x, int ntiles, int tw, int th, float4* out,
+                          hipStream_t stream)
+{
+    const long long n = (long long)nang * ntiles;
+    if (n <= 0) return;
+    hipLaunchKernelGGL(tile_geometry_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, ang, nang,
+                       tiles_x, ntiles, tw, th, out);
 }
 
 template <int STORE>
@@ -786,6 +854,7 @@ __global__ void __launch_bounds__(m32::kThreads, 4) bp_m32_kernel(BpLaunch L)
             const int mid = (lo + hi + 1) >> 1;
             if (L.slices[mid].tile_base <= item) lo = mid; else hi = mid - 1;
         }
+        lo = __builtin_amdgcn_readfirstlane(lo);
         const BpSlice& S = L.slices[lo];
         const int t = item - S.tile_base;
         const int tiles_x = S.tiles_x;
@@ -1104,6 +1173,17 @@ static void allow_lds(const void* f, size_t lds)
     if (nseen < 64) { seen_f[nseen] = f; seen_lds[nseen] = lds; nseen++; }
 }
 
+static const void* kernel_tg(int tile)
+{
+    switch (tile) {
+    case 1: return (const void*)&bp_tiled_kernel<BP_LINEAR2, 4, 8, 4, true>;
+    case 2: return (const void*)&bp_tiled_kernel<BP_LINEAR2, 2, 2, 4, true>;
+    case 3: return (const void*)&bp_tiled_kernel<BP_LINEAR2, 8, 4, 4, true>;
+    case 4: return (const void*)&bp_tiled_kernel<BP_LINEAR2, 2, 4, 4, true>;
+    default: return (const void*)&bp_tiled_kernel<BP_LINEAR2, 4, 4, 4, true>;
+    }
+}
+
 int bp_blocks_per_cu(int mode, int tile, int chunk)
 {
     const void* f = kernel_for(mode, tile);
@@ -1115,10 +1195,34 @@ int bp_blocks_per_cu(int mode, int tile, int chunk)
     return nb;
 }
 
+int bp_blocks_per_cu_tg(int mode, int tile, int chunk, bool tgeom)
+{
+    if (!(tgeom && mode == BP_LINEAR2)) return bp_blocks_per_cu(mode, tile, chunk);
+    const void* f = kernel_tg(tile);
+    const size_t lds = bp_lds_bytes(mode, tile, chunk);
+    allow_lds(f, lds);
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, 256, lds) != hipSuccess || nb < 1) nb = 1;
+    return nb;
+}
+
 template <int MODE>
 static void launch_mode(const BpLaunch& L, int grid, hipStream_t stream)
 {
     const size_t lds = bp_lds_bytes(MODE, L.tile, L.chunk);
+    if constexpr (MODE == BP_LINEAR2) {
+        if (L.tgeom) {
+            allow_lds(kernel_tg(L.tile), lds);
+            switch (L.tile) {
+            case 1: hipLaunchKernelGGL((bp_tiled_kernel<MODE, 4, 8, 4, true>), dim3(grid), dim3(256), lds, stream, L); break;
+            case 2: hipLaunchKernelGGL((bp_tiled_kernel<MODE, 2, 2, 4, true>), dim3(grid), dim3(256), lds, stream, L); break;
+            case 3: hipLaunchKernelGGL((bp_tiled_kernel<MODE, 8, 4, 4, true>), dim3(grid), dim3(256), lds, stream, L); break;
+            case 4: hipLaunchKernelGGL((bp_tiled_kernel<MODE, 2, 4, 4, true>), dim3(grid), dim3(256), lds, stream, L); break;
+            default: hipLaunchKernelGGL((bp_tiled_kernel<MODE, 4, 4, 4, true>), dim3(grid), dim3(256), lds, stream, L); break;
+            }
+            return;
+        }
+    }
     allow_lds(kernel_ptr<MODE>(L.tile), lds);
     if constexpr (MODE == BP_LINEAR_H8) {
         if (L.tile == 2) hipLaunchKernelGGL((bp_tiled_kernel<MODE, 2, 2>), dim3(grid), dim3(256), lds, stream, L);

@@ -161,6 +161,10 @@ struct BpSlice {
     const float* scales_x[BP_MAX_SLICES_PER_ITEM - 1]; // of the item's other slices
     void* out_x[BP_MAX_SLICES_PER_ITEM - 1];
     const float* gscale;    // FP16 multi-slice modes: the item scale (device, set by the pack pass)
+    // BP_LINEAR2: per (tile, angle) the tile geometry (dux, duy, ut - s0, s0 as int
+    // bits), [tiles][nang], built once per plan (the kernel then reads it with
+    // scalar loads instead of forming it per chunk); nullptr: formed per chunk
+    const float4* tgeom;
 };
 
 struct BpLaunch {
@@ -178,6 +182,7 @@ struct BpLaunch {
                             // image (order-free, so bit-exact); 0 or 1: one item per tile
     int ang_begin, ang_end; // ang_end > 0: only the slice's angles [ang_begin, ang_end)
                             // (a segment of a launch sequence); 0: all of them
+    int tgeom;              // BP_LINEAR2: 1 = every slice has BpSlice::tgeom
 };
 
 void launch_bp(const BpLaunch& L, int grid, hipStream_t stream);
@@ -239,8 +244,13 @@ int bp_threads(int mode, int tile);         // workgroup size of a mode's kernel
 int bp_tile_side(int tile);                 // tile width in pixels for a config id
 int bp_tile_h(int tile);                    // tile height in pixels
 int bp_default_chunk(int mode, int tile);   // angles per chunk
+int bp_blocks_per_cu_tg(int mode, int tile, int chunk, bool tgeom);   // resident workgroups per CU
 int bp_max_chunk(int mode);                 // largest chunk a mode supports
 size_t bp_lds_bytes(int mode, int tile, int chunk);
 void launch_zero_u32(unsigned int* p, int n, hipStream_t stream);
+// Tile geometry table of BpSlice::tgeom for tiles of tw x th pixels, ntiles of them
+// in rows of tiles_x, over nang angle entries.
+void launch_tile_geometry(const AngleEntry* ang, int nang, int tiles_x, int ntiles, int tw, int th, float4* out,
+                          hipStream_t stream);
 
 }  // namespace rv

@@ -48,6 +48,7 @@ struct SliceState {
 struct Group {
     int tile = 0, chunk = 0;
     int mode = 0;                // BP mode of the group's launch
+    bool tgeom = false;          // BP_LINEAR2: every entry has a tile geometry table
     std::vector<int> slices;     // first slice of each work-item entry
     std::vector<std::array<int, rv::BP_MAX_SLICES_PER_ITEM - 1>> others;   // BP_LINEAR2: other slices (-1: none)
     std::vector<rv::BpSlice> h;
@@ -77,6 +78,7 @@ struct rv_plan_opaque {
     int* d_pairs = nullptr;
     unsigned char* d_packed = nullptr;   // multi-slice work items: packed rows
     float* d_gscale = nullptr;           // FP16-pair items: item scales
+    unsigned char* d_tgeom = nullptr;    // FP32 pair groups: tile geometry tables
     rv::PackJob* d_pack_jobs = nullptr;
     int n_pack = 0;
     bool pack_half = false;
@@ -116,6 +118,7 @@ void plan_free(rv_plan_t p)
     (void)hipFree(p->d_pairs);
     (void)hipFree(p->d_packed);
     (void)hipFree(p->d_gscale);
+    (void)hipFree(p->d_tgeom);
     (void)hipFree(p->d_pack_jobs);
     (void)hipFree(p->d_fft_work);
     delete p;
@@ -285,6 +288,7 @@ int launch_backprojection(rv_plan_t p, hipStream_t st)
         L.rule = p->opt.index_rule ? 1 : 0;
         L.chunk = G.chunk;
         L.tile = G.tile;
+        L.tgeom = G.tgeom ? 1 : 0;
         if (L.counter) rv::launch_zero_u32(p->d_counter + gi, 1, st);
         rv::launch_bp(L, G.grid, st);
     }
@@ -590,6 +594,43 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
         return fail(p, std::string("plan allocation failed: ") + RecviewLastMessage());
     size_t packed_off = 0, half_idx = 0;
 
+    // Tile geometry tables of the FP32 pair groups, one per class of identical
+    // geometry (its slices share every angle entry): [tiles][angles] of 16 B,
+    // formed once here; at most 2 GiB, groups beyond that form theirs per chunk.
+    std::vector<int> cls_of(nslices, -1);
+    for (size_t c = 0; c < classes.size(); c++)
+        for (int s : classes[c]) cls_of[s] = (int)c;
+    std::vector<size_t> tg_off(classes.size(), (size_t)-1);
+    size_t tg_total = 0;
+    for (auto& G : p->groups) {
+        if (G.mode != rv::BP_LINEAR2) continue;
+        G.tgeom = true;
+        for (int s : G.slices) {
+            const int c = cls_of[s];
+            if (c < 0) { G.tgeom = false; break; }
+            if (tg_off[c] != (size_t)-1) continue;
+            const SliceState& S = p->sl[s];
+            // + 64 B: the kernel reads four entries at a time past a chunk's end
+            const size_t bytes = align_up((size_t)S.tiles_x * S.tiles_y * S.nang * 16 + 64, 256);
+            if (tg_total + bytes > ((size_t)2 << 30)) { G.tgeom = false; break; }
+            tg_off[c] = tg_total;
+            tg_total += bytes;
+        }
+    }
+    if (tg_total) {
+        if (!rv::hip_ok(hipMalloc(&p->d_tgeom, tg_total), "tile geometry tables"))
+            return fail(p, std::string("plan allocation failed: ") + RecviewLastMessage());
+        for (size_t c = 0; c < classes.size(); c++) {
+            if (tg_off[c] == (size_t)-1) continue;
+            const SliceState& S = p->sl[classes[c][0]];
+            rv::launch_tile_geometry(p->d_ang + S.ang_off, S.nang, S.tiles_x, S.tiles_x * S.tiles_y,
+                                     rv::bp_tile_side(S.tile), rv::bp_tile_h(S.tile),
+                                     (float4*)(p->d_tgeom + tg_off[c]), nullptr);
+        }
+        if (!rv::hip_ok(hipGetLastError(), "tile geometry") || !rv::hip_ok(hipDeviceSynchronize(), "tile geometry"))
+            return fail(p, std::string("plan set-up failed: ") + RecviewLastMessage());
+    }
+
     for (auto& G : p->groups) {
         // slice order inside the group: longest projection lists first
         std::vector<int> ord = G.slices;
@@ -626,6 +667,7 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
             B.lut_formula = opt->lut_formula ? 1 : 0;
             B.store = p->store;
             B.out_scale = opt->normalize ? (float)(6.283185307179586 / S.nang) : 1.f;
+            B.tgeom = G.tgeom ? (const float4*)(p->d_tgeom + tg_off[cls_of[G.slices[q]]]) : nullptr;
             for (int e = 0; e + 1 < rv::BP_MAX_SLICES_PER_ITEM; e++) {
                 const int s2 = G.others[q][e];
                 if (s2 < 0) continue;
@@ -669,9 +711,9 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
             !rv::hip_ok(hipMemcpy(G.d_slices, G.h.data(), sizeof(rv::BpSlice) * G.h.size(), hipMemcpyHostToDevice),
                         "slices"))
             return fail(p, std::string("plan upload failed: ") + RecviewLastMessage());
-        (void)rv::bp_blocks_per_cu(G.mode, G.tile, G.chunk);     // sets LDS attributes outside capture
+        (void)rv::bp_blocks_per_cu_tg(G.mode, G.tile, G.chunk, G.tgeom);     // sets LDS attributes outside capture
         if (opt->persistent) {
-            const int per_cu = rv::bp_blocks_per_cu(G.mode, G.tile, G.chunk);
+            const int per_cu = rv::bp_blocks_per_cu_tg(G.mode, G.tile, G.chunk, G.tgeom);
             G.grid = std::max(1, std::min(per_cu * rv::device_cu_count(device), G.total_items));
         } else {
             G.grid = G.total_items;

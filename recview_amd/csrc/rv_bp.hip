@@ -272,11 +272,24 @@ __global__ void __launch_bounds__(64 * NW, NW == 8 ? 4 : 1) bp_tiled_kernel(BpLa
             if (!TG) __syncthreads();
 
             // ---- stage detector segments into LDS
+            // row and segment origin of angle a of the chunk
+            auto origin = [&](int a, int& row, int& s0) {
+                if (TG) {
+                    const bool ok = a < na;
+                    row = ok ? S.ang[a0 + a].row : -1;
+                    s0 = ok ? __float_as_int(tgv[a0 + a].w) : 0;
+                } else {
+                    row = rws[a];
+                    s0 = s0s[a];
+                }
+            };
             if (MODE == BP_LINEAR) {
                 float2* pairs = (float2*)data;
-                for (int idx = tid; idx < chunk * SEG; idx += C::T) {
+                for (int idx = tid; idx < na * SEG; idx += C::T) {
                     const int a = idx / SEG, j = idx - a * SEG;
-                    pairs[idx] = linear_pair(S.rows, S.scales, S, rws[a], s0s[a] + j, j);
+                    int row, s0;
+                    origin(a, row, s0);
+                    pairs[idx] = linear_pair(S.rows, S.scales, S, row, s0 + j, j);
                 }
             } else if (MODE == BP_LINEAR2) {
                 // a_j = p_k - j d_k as linear_pair forms it (d = 0 at the last sample)
@@ -288,17 +301,6 @@ __global__ void __launch_bounds__(64 * NW, NW == 8 ? 4 : 1) bp_tiled_kernel(BpLa
                     if (row >= 0 && k >= 0 && k < N) v = packed[(long long)row * S.row_stride + k];
                     const float fj = -(float)j;
                     quads[a * SEG + j] = make_float4(fmaf(fj, v.y, v.x), v.y, fmaf(fj, v.w, v.z), v.w);
-                };
-                // row and segment origin of angle a of the chunk
-                auto origin = [&](int a, int& row, int& s0) {
-                    if (TG) {
-                        const bool ok = a < na;
-                        row = ok ? S.ang[a0 + a].row : -1;
-                        s0 = ok ? __float_as_int(tgv[a0 + a].w) : 0;
-                    } else {
-                        row = rws[a];
-                        s0 = s0s[a];
-                    }
                 };
                 if (C::T % chunk == 0) {
                     // threads of one angle: its row and segment origin read once per
@@ -336,9 +338,11 @@ __global__ void __launch_bounds__(64 * NW, NW == 8 ? 4 : 1) bp_tiled_kernel(BpLa
                 }
             } else if (is_half_multi(MODE)) {
                 // packed FP16 pairs of the item's slices: one 8- / 16-byte load per sample
-                for (int idx = tid; idx < chunk * SEG; idx += C::T) {
+                for (int idx = tid; idx < na * SEG; idx += C::T) {
                     const int a = idx / SEG, j = idx - a * SEG;
-                    const int row = rws[a], k = s0s[a] + j;
+                    int row, s0;
+                    origin(a, row, s0);
+                    const int k = s0 + j;
                     const bool in = row >= 0 && k >= 0 && k < N;
                     const long long off = (long long)row * S.row_stride + k;
                     if (NS >= 8) {
@@ -396,10 +400,29 @@ __global__ void __launch_bounds__(64 * NW, NW == 8 ? 4 : 1) bp_tiled_kernel(BpLa
             __syncthreads();
 
             // ---- accumulate
+            // fn(a, tile geometry) for every angle of the chunk.  With the table, the
+            // next angle's entry loads (vector memory, counted apart from the LDS
+            // reads) while this angle's updates run.  For the FP32 pairs the compiler
+            // schedules this form for six waves per SIMD (79 VGPRs, two LDS reads in
+            // flight per wave); measured faster than forms scheduled for four (64-slice
+            // C2 stack: 37.8 ms vs 39.4 ms per-chunk geometry, 41.2 ms table read at the
+            // top of each angle)
+            auto for_angles = [&](auto&& fn) {
+                if constexpr (TG) {
+                    const gf4p tgg = (gf4p)tgv;
+                    f4v tn = tgg[a0];
+                    for (int a = 0; a < na; a++) {
+                        const float4 tb = make_float4(tn.x, tn.y, tn.z, tn.w);
+                        if (a + 1 < na) tn = tgg[a0 + a + 1];
+                        fn(a, tb);
+                    }
+                } else {
+                    for (int a = 0; a < na; a++) fn(a, tab[a]);
+                }
+            };
             if (MODE == BP_LINEAR) {
                 const float2* pairs = (const float2*)data;
-                for (int a = 0; a < chunk; a++) {
-                    const float4 tb = tab[a];
+                for_angles([&](int a, const float4 tb) {
                     const float2* seg = pairs + a * SEG;
                     // u(i, j) = base + 8 i cos + 8 j sin, base at the lane's first pixel
                     const float base = fmaf(dyl, tb.y, fmaf(dxl, tb.x, tb.z));
@@ -414,7 +437,7 @@ __global__ void __launch_bounds__(64 * NW, NW == 8 ? 4 : 1) bp_tiled_kernel(BpLa
                             acc[j][i] += fmaf(u, ad.y, ad.x);
                         }
                     }
-                }
+                });
             } else if (MODE == BP_LINEAR2) {
                 const float4* quads = (const float4*)data;
                 auto angle = [&](int a, const float4 tb) {
@@ -437,23 +460,7 @@ __global__ void __launch_bounds__(64 * NW, NW == 8 ? 4 : 1) bp_tiled_kernel(BpLa
                         }
                     }
                 };
-                if constexpr (TG) {
-                    // the next angle's entry loads (vector memory, counted apart from
-                    // the LDS reads) while this angle's updates run.  The compiler
-                    // schedules this form for six waves per SIMD (79 VGPRs, two LDS
-                    // reads in flight per wave); measured faster than forms scheduled
-                    // for four (64-slice C2 stack: 37.8 ms vs 39.4 ms per-chunk geometry,
-                    // 41.2 ms table read at the top of each angle)
-                    const gf4p tgg = (gf4p)tgv;
-                    f4v tn = tgg[a0];
-                    for (int a = 0; a < na; a++) {
-                        const float4 tb = make_float4(tn.x, tn.y, tn.z, tn.w);
-                        if (a + 1 < na) tn = tgg[a0 + a + 1];
-                        angle(a, tb);
-                    }
-                } else {
-                    for (int a = 0; a < na; a++) angle(a, tab[a]);
-                }
+                for_angles(angle);
             } else if (is_half_multi(MODE)) {
                 // FP16 pairs of 2 / 4 / 8 / 16 slices per sample (8, 16: two / four 16-byte planes).
                 using QT = typename std::conditional<NS >= 4, uint4, uint2>::type;
@@ -488,8 +495,7 @@ __global__ void __launch_bounds__(64 * NW, NW == 8 ? 4 : 1) bp_tiled_kernel(BpLa
                             }
                         }
                 };
-                for (int a = 0; a < chunk; a++) {
-                    const float4 tb = tab[a];
+                for_angles([&](int a, const float4 tb) {
                     const float base = fmaf(dyl, tb.y, fmaf(dxl, tb.x, tb.z));
                     const QT* seg = ents + a * NPL * SEG;
                     if constexpr (NS == 8 && NW == 4) {
@@ -527,7 +533,7 @@ __global__ void __launch_bounds__(64 * NW, NW == 8 ? 4 : 1) bp_tiled_kernel(BpLa
                         for (int j = 0; j < PY; j++) dot_row(j, q[j], w[j]);
                         __builtin_amdgcn_sched_barrier(0);
                     }
-                }
+                });
             } else if (MODE == BP_LINEAR_H) {
                 const __half2* pairs = (const __half2*)data;
                 float cacc[PY][PX];
@@ -1124,30 +1130,40 @@ int bp_threads(int mode, int tile)
                ? 512 : 256;
 }
 
-template <int MODE>
+// Modes that read the tile geometry table: the FP32 pairs.  The kernel also
+// builds the table form for single FP32 slices and FP16 pairs, but the sixteen-
+// slice FP16 kernel measured slower with it (64-slice C2 stack: 26.4 vs 24.0 ms;
+// it spills and the compiler's schedule changes), and singles are not measured.
+bool bp_has_tgeom(int mode)
+{
+    return mode == BP_LINEAR2;
+}
+
+template <int MODE, bool TG = false>
 static const void* kernel_ptr(int tile)
 {
     if constexpr (MODE == BP_LINEAR_H8) {
-        if (tile == 2) return (const void*)&bp_tiled_kernel<MODE, 2, 2>;
-        if (tile == 4) return (const void*)&bp_tiled_kernel<MODE, 2, 4>;
-        return tile == 0 ? (const void*)&bp_tiled_kernel<MODE, 4, 2, 8> : nullptr;
+        if (tile == 2) return (const void*)&bp_tiled_kernel<MODE, 2, 2, 4, TG>;
+        if (tile == 4) return (const void*)&bp_tiled_kernel<MODE, 2, 4, 4, TG>;
+        return tile == 0 ? (const void*)&bp_tiled_kernel<MODE, 4, 2, 8, TG> : nullptr;
     } else if constexpr (MODE == BP_LINEAR_H16) {
-        if (tile == 2) return (const void*)&bp_tiled_kernel<MODE, 2, 2>;
-        return tile == 4 ? (const void*)&bp_tiled_kernel<MODE, 2, 2, 8> : nullptr;
+        if (tile == 2) return (const void*)&bp_tiled_kernel<MODE, 2, 2, 4, TG>;
+        return tile == 4 ? (const void*)&bp_tiled_kernel<MODE, 2, 2, 8, TG> : nullptr;
     } else {
         switch (tile) {
-        case 1: return (const void*)&bp_tiled_kernel<MODE, 4, 8>;
-        case 2: return (const void*)&bp_tiled_kernel<MODE, 2, 2>;
-        case 3: return (const void*)&bp_tiled_kernel<MODE, 8, 4>;
-        case 4: return (const void*)&bp_tiled_kernel<MODE, 2, 4>;
-        default: return (const void*)&bp_tiled_kernel<MODE, 4, 4>;
+        case 1: return (const void*)&bp_tiled_kernel<MODE, 4, 8, 4, TG>;
+        case 2: return (const void*)&bp_tiled_kernel<MODE, 2, 2, 4, TG>;
+        case 3: return (const void*)&bp_tiled_kernel<MODE, 8, 4, 4, TG>;
+        case 4: return (const void*)&bp_tiled_kernel<MODE, 2, 4, 4, TG>;
+        default: return (const void*)&bp_tiled_kernel<MODE, 4, 4, 4, TG>;
         }
     }
 }
 
-static const void* kernel_for(int mode, int tile)
+static const void* kernel_for(int mode, int tile, bool tg = false)
 {
     if (mode == BP_LINEAR_M32) return tile == 2 ? (const void*)&bp_m32_kernel : nullptr;
+    if (tg && mode == BP_LINEAR2) return kernel_ptr<BP_LINEAR2, true>(tile);
     if (mode == BP_LINEAR) return kernel_ptr<BP_LINEAR>(tile);
     if (mode == BP_LINEAR2) return kernel_ptr<BP_LINEAR2>(tile);
     if (mode == BP_LINEAR_H) return kernel_ptr<BP_LINEAR_H>(tile);
@@ -1173,17 +1189,6 @@ static void allow_lds(const void* f, size_t lds)
     if (nseen < 64) { seen_f[nseen] = f; seen_lds[nseen] = lds; nseen++; }
 }
 
-static const void* kernel_tg(int tile)
-{
-    switch (tile) {
-    case 1: return (const void*)&bp_tiled_kernel<BP_LINEAR2, 4, 8, 4, true>;
-    case 2: return (const void*)&bp_tiled_kernel<BP_LINEAR2, 2, 2, 4, true>;
-    case 3: return (const void*)&bp_tiled_kernel<BP_LINEAR2, 8, 4, 4, true>;
-    case 4: return (const void*)&bp_tiled_kernel<BP_LINEAR2, 2, 4, 4, true>;
-    default: return (const void*)&bp_tiled_kernel<BP_LINEAR2, 4, 4, 4, true>;
-    }
-}
-
 int bp_blocks_per_cu(int mode, int tile, int chunk)
 {
     const void* f = kernel_for(mode, tile);
@@ -1197,13 +1202,38 @@ int bp_blocks_per_cu(int mode, int tile, int chunk)
 
 int bp_blocks_per_cu_tg(int mode, int tile, int chunk, bool tgeom)
 {
-    if (!(tgeom && mode == BP_LINEAR2)) return bp_blocks_per_cu(mode, tile, chunk);
-    const void* f = kernel_tg(tile);
+    if (!(tgeom && bp_has_tgeom(mode))) return bp_blocks_per_cu(mode, tile, chunk);
+    const void* f = kernel_for(mode, tile, true);
     const size_t lds = bp_lds_bytes(mode, tile, chunk);
     allow_lds(f, lds);
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, 256, lds) != hipSuccess || nb < 1) nb = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, bp_threads(mode, tile), lds) != hipSuccess || nb < 1)
+        nb = 1;
     return nb;
+}
+
+template <int MODE, bool TG>
+static void launch_mode_tg(const BpLaunch& L, int grid, size_t lds, hipStream_t stream)
+{
+    allow_lds(kernel_ptr<MODE, TG>(L.tile), lds);
+    if constexpr (MODE == BP_LINEAR_H8) {
+        if (L.tile == 2) hipLaunchKernelGGL((bp_tiled_kernel<MODE, 2, 2, 4, TG>), dim3(grid), dim3(256), lds, stream, L);
+        else if (L.tile == 4) hipLaunchKernelGGL((bp_tiled_kernel<MODE, 2, 4, 4, TG>), dim3(grid), dim3(256), lds, stream, L);
+        else if (L.tile == 0)
+            hipLaunchKernelGGL((bp_tiled_kernel<MODE, 4, 2, 8, TG>), dim3(grid), dim3(512), lds, stream, L);
+    } else if constexpr (MODE == BP_LINEAR_H16) {
+        if (L.tile == 2) hipLaunchKernelGGL((bp_tiled_kernel<MODE, 2, 2, 4, TG>), dim3(grid), dim3(256), lds, stream, L);
+        else if (L.tile == 4)
+            hipLaunchKernelGGL((bp_tiled_kernel<MODE, 2, 2, 8, TG>), dim3(grid), dim3(512), lds, stream, L);
+    } else {
+        switch (L.tile) {
+        case 1: hipLaunchKernelGGL((bp_tiled_kernel<MODE, 4, 8, 4, TG>), dim3(grid), dim3(256), lds, stream, L); break;
+        case 2: hipLaunchKernelGGL((bp_tiled_kernel<MODE, 2, 2, 4, TG>), dim3(grid), dim3(256), lds, stream, L); break;
+        case 3: hipLaunchKernelGGL((bp_tiled_kernel<MODE, 8, 4, 4, TG>), dim3(grid), dim3(256), lds, stream, L); break;
+        case 4: hipLaunchKernelGGL((bp_tiled_kernel<MODE, 2, 4, 4, TG>), dim3(grid), dim3(256), lds, stream, L); break;
+        default: hipLaunchKernelGGL((bp_tiled_kernel<MODE, 4, 4, 4, TG>), dim3(grid), dim3(256), lds, stream, L); break;
+        }
+    }
 }
 
 template <int MODE>
@@ -1212,36 +1242,11 @@ static void launch_mode(const BpLaunch& L, int grid, hipStream_t stream)
     const size_t lds = bp_lds_bytes(MODE, L.tile, L.chunk);
     if constexpr (MODE == BP_LINEAR2) {
         if (L.tgeom) {
-            allow_lds(kernel_tg(L.tile), lds);
-            switch (L.tile) {
-            case 1: hipLaunchKernelGGL((bp_tiled_kernel<MODE, 4, 8, 4, true>), dim3(grid), dim3(256), lds, stream, L); break;
-            case 2: hipLaunchKernelGGL((bp_tiled_kernel<MODE, 2, 2, 4, true>), dim3(grid), dim3(256), lds, stream, L); break;
-            case 3: hipLaunchKernelGGL((bp_tiled_kernel<MODE, 8, 4, 4, true>), dim3(grid), dim3(256), lds, stream, L); break;
-            case 4: hipLaunchKernelGGL((bp_tiled_kernel<MODE, 2, 4, 4, true>), dim3(grid), dim3(256), lds, stream, L); break;
-            default: hipLaunchKernelGGL((bp_tiled_kernel<MODE, 4, 4, 4, true>), dim3(grid), dim3(256), lds, stream, L); break;
-            }
+            launch_mode_tg<MODE, true>(L, grid, lds, stream);
             return;
         }
     }
-    allow_lds(kernel_ptr<MODE>(L.tile), lds);
-    if constexpr (MODE == BP_LINEAR_H8) {
-        if (L.tile == 2) hipLaunchKernelGGL((bp_tiled_kernel<MODE, 2, 2>), dim3(grid), dim3(256), lds, stream, L);
-        else if (L.tile == 4) hipLaunchKernelGGL((bp_tiled_kernel<MODE, 2, 4>), dim3(grid), dim3(256), lds, stream, L);
-        else if (L.tile == 0)
-            hipLaunchKernelGGL((bp_tiled_kernel<MODE, 4, 2, 8>), dim3(grid), dim3(512), lds, stream, L);
-    } else if constexpr (MODE == BP_LINEAR_H16) {
-        if (L.tile == 2) hipLaunchKernelGGL((bp_tiled_kernel<MODE, 2, 2>), dim3(grid), dim3(256), lds, stream, L);
-        else if (L.tile == 4)
-            hipLaunchKernelGGL((bp_tiled_kernel<MODE, 2, 2, 8>), dim3(grid), dim3(512), lds, stream, L);
-    } else {
-        switch (L.tile) {
-        case 1: hipLaunchKernelGGL((bp_tiled_kernel<MODE, 4, 8>), dim3(grid), dim3(256), lds, stream, L); break;
-        case 2: hipLaunchKernelGGL((bp_tiled_kernel<MODE, 2, 2>), dim3(grid), dim3(256), lds, stream, L); break;
-        case 3: hipLaunchKernelGGL((bp_tiled_kernel<MODE, 8, 4>), dim3(grid), dim3(256), lds, stream, L); break;
-        case 4: hipLaunchKernelGGL((bp_tiled_kernel<MODE, 2, 4>), dim3(grid), dim3(256), lds, stream, L); break;
-        default: hipLaunchKernelGGL((bp_tiled_kernel<MODE, 4, 4>), dim3(grid), dim3(256), lds, stream, L); break;
-        }
-    }
+    launch_mode_tg<MODE, false>(L, grid, lds, stream);
 }
 
 __global__ void __launch_bounds__(64) ptr_patch_kernel(PtrPatch P)

@@ -162,8 +162,8 @@ struct BpSlice {
     void* out_x[BP_MAX_SLICES_PER_ITEM - 1];
     const float* gscale;    // FP16 multi-slice modes: the item scale (device, set by the pack pass)
     // BP_LINEAR2: per (tile, angle) the tile geometry (dux, duy, ut - s0, s0 as int
-    // bits), [tiles][nang], built once per plan (the kernel then reads it with
-    // scalar loads instead of forming it per chunk); nullptr: formed per chunk
+    // bits), [tiles][nang], built once per plan, read one angle ahead instead of
+    // formed in LDS per chunk; nullptr: formed per chunk
     const float4* tgeom;
 };
 
@@ -245,6 +245,7 @@ int bp_tile_side(int tile);                 // tile width in pixels for a config
 int bp_tile_h(int tile);                    // tile height in pixels
 int bp_default_chunk(int mode, int tile);   // angles per chunk
 int bp_blocks_per_cu_tg(int mode, int tile, int chunk, bool tgeom);   // resident workgroups per CU
+bool bp_has_tgeom(int mode);                // the mode has a tile geometry table variant
 int bp_max_chunk(int mode);                 // largest chunk a mode supports
 size_t bp_lds_bytes(int mode, int tile, int chunk);
 void launch_zero_u32(unsigned int* p, int n, hipStream_t stream);

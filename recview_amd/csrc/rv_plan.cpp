@@ -597,35 +597,41 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
     // Tile geometry tables of the FP32 pair groups, one per class of identical
     // geometry (its slices share every angle entry): [tiles][angles] of 16 B,
     // formed once here; at most 2 GiB, groups beyond that form theirs per chunk.
+    // A table depends on the geometry and the tile configuration: the matrix-core
+    // grouping moves its slices to 32x32 tiles after the classes are formed, so a
+    // class can run on two tile configurations.
     std::vector<int> cls_of(nslices, -1);
     for (size_t c = 0; c < classes.size(); c++)
         for (int s : classes[c]) cls_of[s] = (int)c;
-    std::vector<size_t> tg_off(classes.size(), (size_t)-1);
+    std::map<std::pair<int, int>, std::pair<size_t, int>> tg_tab;   // (class, tile) -> (offset, slice)
     size_t tg_total = 0;
     for (auto& G : p->groups) {
-        if (G.mode != rv::BP_LINEAR2) continue;
+        if (!rv::bp_has_tgeom(G.mode)) continue;
         G.tgeom = true;
         for (int s : G.slices) {
             const int c = cls_of[s];
             if (c < 0) { G.tgeom = false; break; }
-            if (tg_off[c] != (size_t)-1) continue;
             const SliceState& S = p->sl[s];
-            // + 64 B: the kernel reads four entries at a time past a chunk's end
+            const auto key = std::make_pair(c, S.tile);
+            if (tg_tab.count(key)) continue;
+            // + 64 B: a chunk's prefetch may read one entry past the tile's last
             const size_t bytes = align_up((size_t)S.tiles_x * S.tiles_y * S.nang * 16 + 64, 256);
             if (tg_total + bytes > ((size_t)2 << 30)) { G.tgeom = false; break; }
-            tg_off[c] = tg_total;
+            tg_tab[key] = std::make_pair(tg_total, s);
             tg_total += bytes;
         }
     }
+    auto tg_of = [&](int s) -> const float4* {
+        return (const float4*)(p->d_tgeom + tg_tab.at(std::make_pair(cls_of[s], p->sl[s].tile)).first);
+    };
     if (tg_total) {
         if (!rv::hip_ok(hipMalloc(&p->d_tgeom, tg_total), "tile geometry tables"))
             return fail(p, std::string("plan allocation failed: ") + RecviewLastMessage());
-        for (size_t c = 0; c < classes.size(); c++) {
-            if (tg_off[c] == (size_t)-1) continue;
-            const SliceState& S = p->sl[classes[c][0]];
+        for (const auto& kv : tg_tab) {
+            const SliceState& S = p->sl[kv.second.second];
             rv::launch_tile_geometry(p->d_ang + S.ang_off, S.nang, S.tiles_x, S.tiles_x * S.tiles_y,
                                      rv::bp_tile_side(S.tile), rv::bp_tile_h(S.tile),
-                                     (float4*)(p->d_tgeom + tg_off[c]), nullptr);
+                                     (float4*)(p->d_tgeom + kv.second.first), nullptr);
         }
         if (!rv::hip_ok(hipGetLastError(), "tile geometry") || !rv::hip_ok(hipDeviceSynchronize(), "tile geometry"))
             return fail(p, std::string("plan set-up failed: ") + RecviewLastMessage());
@@ -667,7 +673,7 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
             B.lut_formula = opt->lut_formula ? 1 : 0;
             B.store = p->store;
             B.out_scale = opt->normalize ? (float)(6.283185307179586 / S.nang) : 1.f;
-            B.tgeom = G.tgeom ? (const float4*)(p->d_tgeom + tg_off[cls_of[G.slices[q]]]) : nullptr;
+            B.tgeom = G.tgeom ? tg_of(G.slices[q]) : nullptr;
             for (int e = 0; e + 1 < rv::BP_MAX_SLICES_PER_ITEM; e++) {
                 const int s2 = G.others[q][e];
                 if (s2 < 0) continue;

@@ -596,7 +596,8 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
 
     // Tile geometry tables of the FP32 pair groups, one per class of identical
     // geometry (its slices share every angle entry): [tiles][angles] of 16 B,
-    // formed once here; at most 2 GiB, groups beyond that form theirs per chunk.
+    // formed once here; at most 8 GiB (the 256-slice batch of uneven shapes up to
+    // 8192^2 x 3600 needs 3.8 GiB), groups beyond that form theirs per chunk.
     // A table depends on the geometry and the tile configuration: the matrix-core
     // grouping moves its slices to 32x32 tiles after the classes are formed, so a
     // class can run on two tile configurations.
@@ -616,7 +617,7 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
             if (tg_tab.count(key)) continue;
             // + 64 B: a chunk's prefetch may read one entry past the tile's last
             const size_t bytes = align_up((size_t)S.tiles_x * S.tiles_y * S.nang * 16 + 64, 256);
-            if (tg_total + bytes > ((size_t)2 << 30)) { G.tgeom = false; break; }
+            if (tg_total + bytes > ((size_t)8 << 30)) { G.tgeom = false; break; }
             tg_tab[key] = std::make_pair(tg_total, s);
             tg_total += bytes;
         }

@@ -531,19 +531,24 @@ def main():
         step()
     torch.cuda.synchronize()
 
-    # Stage timing with HIP events on the launch stream (eager passes).
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
-    bp_ms = []
+    # Stage timing with HIP events on the launch stream (eager passes): filter,
+    # pack, and the back-projection kernels alone (the roofline's kernel).
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    bp_ms, pack_ms = [], []
     for _ in range(max(1, min(3, args.steps))):
         ev[0].record(stream)
         plan.filter(sino_list, stream=stream)
         ev[1].record(stream)
-        plan.backproject(outs, stream=stream)
+        plan.pack(stream=stream)
         ev[2].record(stream)
+        plan.backproject_packed(outs, stream=stream)
+        ev[3].record(stream)
         torch.cuda.synchronize()
-        bp_ms.append(ev[1].elapsed_time(ev[2]))
+        pack_ms.append(ev[1].elapsed_time(ev[2]))
+        bp_ms.append(ev[2].elapsed_time(ev[3]))
     filt_ms = ev[0].elapsed_time(ev[1])
     bp_ms = sum(bp_ms) / len(bp_ms)
+    pack_ms = sum(pack_ms) / len(pack_ms)
 
     torch.cuda.synchronize()
     t_start = time.perf_counter()
@@ -606,6 +611,7 @@ def main():
         "tau_ns": round(tau_ns, 7),
         "tau_bp_ns": round(bp_ms * 1e-3 / updates * 1e9, 7),
         "filter_ms": round(filt_ms, 3),
+        "pack_ms": round(pack_ms, 3),
         "bp_ms": round(bp_ms, 3),
         "config": config,
         "d2h_inclusive": d2h,

@@ -321,9 +321,14 @@ int rvPlanExecute(rv_plan_t plan, const void* const* d_sino, void* const* d_out,
 /* Captures rvPlanExecute with these pointers into a hipGraph once and
  * replays it on every later call with the same pointers. */
 int rvPlanExecuteGraph(rv_plan_t plan, const void* const* d_sino, void* const* d_out, rv_stream_t stream);
-/* Stage-level access for tests and benchmarks. */
+/* Stage-level access for tests and benchmarks.  rvPlanBackproject packs the
+ * grouped slices' filtered rows and back-projects them; rvPlanPack and
+ * rvPlanBackprojectPacked are its two halves (the second reads the rows the
+ * last pack wrote, so that the back-projection kernels can be timed alone). */
 int rvPlanFilter(rv_plan_t plan, const void* const* d_sino, rv_stream_t stream);
 int rvPlanBackproject(rv_plan_t plan, void* const* d_out, rv_stream_t stream);
+int rvPlanPack(rv_plan_t plan, rv_stream_t stream);
+int rvPlanBackprojectPacked(rv_plan_t plan, void* const* d_out, rv_stream_t stream);
 /* Device pointer of slice s's filtered rows (FP32 [P][N], or the FP16/BF16
  * rows with their per-row scales), and their row length. */
 int rvPlanFilteredRows(rv_plan_t plan, int s, void** rows, float** scales, int* row_len);

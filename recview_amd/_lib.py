@@ -70,6 +70,7 @@ EXPORTS = [
     "CUDA_MALLOC_HOST_INT", "CUDA_FREE_HOST", "blocksize", "RecviewSetDlgCtrl",
     "RecviewGetErrors", "RecviewClearErrors", "RecviewLastMessage", "RecviewSetCompatOptions",
     "rvPlanCreate", "rvPlanExecute", "rvPlanExecuteGraph", "rvPlanFilter", "rvPlanBackproject",
+    "rvPlanPack", "rvPlanBackprojectPacked",
     "rvPlanFilteredRows", "rvPlanSetFilteredRows", "rvPlanInfo", "rvPlanRows", "rvPlanDestroy",
     "RecviewSetFilter", "RecviewGeometry", "RecviewOffsetCTPairs", "RecviewOffsetCTStitch",
     "rvUnetCreate", "rvUnetParamCount", "rvUnetGetParams", "rvUnetSetParams", "rvUnetForward",
@@ -121,6 +122,8 @@ def _declare(L):
     L.rvPlanExecuteGraph.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), vp]
     L.rvPlanFilter.argtypes = [vp, C.POINTER(vp), vp]
     L.rvPlanBackproject.argtypes = [vp, C.POINTER(vp), vp]
+    L.rvPlanPack.argtypes = [vp, vp]
+    L.rvPlanBackprojectPacked.argtypes = [vp, C.POINTER(vp), vp]
     L.rvPlanFilteredRows.argtypes = [vp, C.c_int, C.POINTER(vp), C.POINTER(vp), ip]
     L.rvPlanSetFilteredRows.argtypes = [vp, C.c_int, vp, vp]
     L.rvPlanInfo.argtypes = [vp, C.c_int, ip, ip, ip, ip, ip]

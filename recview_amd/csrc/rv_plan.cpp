@@ -271,11 +271,16 @@ int launch_filters(rv_plan_t p, const void* const* d_sino, hipStream_t st)
     return rv::hip_ok(err, "filter launch") && rv::hip_ok(hipGetLastError(), "filter launch") ? 0 : -1;
 }
 
-int launch_backprojection(rv_plan_t p, hipStream_t st)
+void launch_packing(rv_plan_t p, hipStream_t st)
 {
     if (p->pack_half) rv::launch_pack_scales(p->d_pack_jobs, p->n_pack, st);
     for (const auto& R : p->pack_ranges)
         rv::launch_pack(p->d_pack_jobs + R.first, R.count, R.kind, p->store, R.rows, st);
+}
+
+int launch_backprojection(rv_plan_t p, hipStream_t st, bool pack = true)
+{
+    if (pack) launch_packing(p, st);
     for (size_t gi = 0; gi < p->groups.size(); gi++) {
         const Group& G = p->groups[gi];
         rv::BpLaunch L{};
@@ -785,6 +790,22 @@ int rvPlanBackproject(rv_plan_t p, void* const* d_out, rv_stream_t stream)
     if (!rv::hip_ok(hipSetDevice(p->dev), "hipSetDevice")) return -1;
     if (!upload_slices(p, d_out, (hipStream_t)stream, false)) return -1;
     return launch_backprojection(p, (hipStream_t)stream);
+}
+
+int rvPlanPack(rv_plan_t p, rv_stream_t stream)
+{
+    if (!p) return -1;
+    if (!rv::hip_ok(hipSetDevice(p->dev), "hipSetDevice")) return -1;
+    launch_packing(p, (hipStream_t)stream);
+    return rv::hip_ok(hipGetLastError(), "pack launch") ? 0 : -1;
+}
+
+int rvPlanBackprojectPacked(rv_plan_t p, void* const* d_out, rv_stream_t stream)
+{
+    if (!p || !all_set(p, (const void* const*)d_out, "image")) return -1;
+    if (!rv::hip_ok(hipSetDevice(p->dev), "hipSetDevice")) return -1;
+    if (!upload_slices(p, d_out, (hipStream_t)stream, false)) return -1;
+    return launch_backprojection(p, (hipStream_t)stream, false);
 }
 
 int rvPlanExecute(rv_plan_t p, const void* const* d_sino, void* const* d_out, rv_stream_t stream)

@@ -147,6 +147,17 @@ class FbpPlan:
         _lib.check(_lib.lib().rvPlanBackproject(self._h, self._count(outs, "images"), _stream_handle(stream)),
                    "rvPlanBackproject")
 
+    def pack(self, stream=None):
+        """The first half of backproject(): grouped slices' filtered rows into the
+        form their kernels stage."""
+        _lib.check(_lib.lib().rvPlanPack(self._h, _stream_handle(stream)), "rvPlanPack")
+
+    def backproject_packed(self, outs, stream=None):
+        """The second half of backproject(): the back-projection kernels alone, on
+        the rows the last pack() (or backproject()) wrote."""
+        _lib.check(_lib.lib().rvPlanBackprojectPacked(self._h, self._count(outs, "images"), _stream_handle(stream)),
+                   "rvPlanBackprojectPacked")
+
     def filtered_rows(self, s):
         rows, scales, n = C.c_void_p(), C.c_void_p(), C.c_int()
         _lib.check(_lib.lib().rvPlanFilteredRows(self._h, s, C.byref(rows), C.byref(scales), C.byref(n)),

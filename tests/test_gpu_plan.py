@@ -638,3 +638,27 @@ def test_plan_non_pow2_grouped_vs_fp64(W, nsl, store, lds_half, rms_tol):
         rms, mx = _errors(imgs[i], ref, mask)
         print(f"W={W} slice {i}/{nsl} store={store} lds_half={lds_half}: rms {rms:.2e} max {mx:.2e}")
         assert rms <= rms_tol, (W, i, store)
+
+
+@pytest.mark.parametrize("store,lds_half", [("f32", False), ("f16", True)])
+def test_plan_pack_then_backproject_packed(store, lds_half):
+    """rvPlanPack + rvPlanBackprojectPacked (the two halves the benchmark times
+    apart) give exactly the images of rvPlanBackproject: five slices of one
+    geometry (grouped, packed rows) and one of another (single)."""
+    specs, sinos = [], []
+    for i in range(6):
+        W, P = (128, 90) if i < 5 else (96, 60)
+        s, fdeg, c = _sino(W, P, 2300 + i)
+        specs.append(fbp.SliceSpec(W, fdeg, c))
+        sinos.append(torch.from_numpy(s).cuda())
+    plan = fbp.FbpPlan(specs, store=store, lds_half=lds_half, persistent=True)
+    plan.filter(sinos)
+    a = [torch.empty(plan.out_n(i), plan.out_n(i), device="cuda") for i in range(6)]
+    b = [torch.full_like(x, float("nan")) for x in a]
+    plan.backproject(a)
+    plan.pack()
+    plan.backproject_packed(b)
+    torch.cuda.synchronize()
+    plan.close()
+    for i in range(6):
+        assert torch.equal(a[i], b[i]), i

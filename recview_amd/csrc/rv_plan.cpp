@@ -16,6 +16,7 @@
 #include <array>
 #include <cmath>
 #include <cstring>
+#include <cstdlib>
 #include <map>
 #include <numeric>
 #include <string>
@@ -611,8 +612,12 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
         for (int s : classes[c]) cls_of[s] = (int)c;
     std::map<std::pair<int, int>, std::pair<size_t, int>> tg_tab;   // (class, tile) -> (offset, slice)
     size_t tg_total = 0;
+    // RECVIEW_GEOMETRY_TABLES=0 (diagnostics, tests): every group forms its geometry
+    // per chunk, as when the tables would exceed the budget
+    const char* tg_env = std::getenv("RECVIEW_GEOMETRY_TABLES");
+    const bool tg_allowed = !(tg_env && tg_env[0] == '0');
     for (auto& G : p->groups) {
-        if (!rv::bp_has_tgeom(G.mode)) continue;
+        if (!tg_allowed || !rv::bp_has_tgeom(G.mode)) continue;
         G.tgeom = true;
         for (int s : G.slices) {
             const int c = cls_of[s];

@@ -662,3 +662,26 @@ def test_plan_pack_then_backproject_packed(store, lds_half):
     plan.close()
     for i in range(6):
         assert torch.equal(a[i], b[i]), i
+
+
+def test_plan_geometry_tables_equal_per_chunk_geometry(monkeypatch):
+    """FP32 pairs read their tile geometry from tables built at plan creation; with
+    the tables switched off (RECVIEW_GEOMETRY_TABLES=0, the path plans take beyond
+    the table budget) the kernel forms it per chunk.  Both give the same images bit
+    for bit: four slices of one geometry (two pairs) on each tile configuration,
+    persistent and one block per item, and a pair and a single of another geometry."""
+    specs, sinos = [], []
+    for i in range(7):
+        W, P = (160, 97) if i < 4 else (96, 60)
+        s, fdeg, c = _sino(W, P, 2400 + i)
+        specs.append(fbp.SliceSpec(W, fdeg, c))
+        sinos.append(torch.from_numpy(s))
+    for tile in range(5):
+        for persistent in (True, False):
+            kw = dict(tile=tile, persistent=persistent)
+            monkeypatch.setenv("RECVIEW_GEOMETRY_TABLES", "0")
+            _, per_chunk = _run(specs, sinos, **kw)
+            monkeypatch.delenv("RECVIEW_GEOMETRY_TABLES")
+            _, tables = _run(specs, sinos, **kw)
+            for i in range(len(specs)):
+                assert np.array_equal(per_chunk[i], tables[i]), (tile, persistent, i)

@@ -14,8 +14,10 @@ The plan picks one back-projection kernel per work-item kind (the mode):
 For every (mode, image side N, projections P) the tuner times each valid tile
 configuration (64x64, 64x128, 32x32, 128x64, 32x64; eight-slice items exist on
 64x64, 32x32 and 32x64, sixteen-slice items on 32x32 and 32x64, 32-slice items on
-32x32 only) and each candidate chunk on a batch that fills the GPU, back-projection
-stage only (pack included), and keeps the fastest.  The table (tuned_tiles.json
+32x32 only) and each candidate chunk on a batch of at least 16384 64x64-tile items
+(up to 64 slices: ten or more rounds of the resident blocks, so the last, partly
+filled round does not decide the choice), back-projection stage only (pack
+included), and keeps the fastest.  The table (tuned_tiles.json
 next to this module) is keyed "mode:N:P"; `tile_for` / `chunk_for` return the
 entry of the nearest tuned shape of the same mode (log distance), and `apply`
 writes them into slice specs before a plan is built.
@@ -109,23 +111,44 @@ def apply(specs, sides, mode, table=None):
     side slice i reconstructs (2W for an offset-axis scan), its projection count the
     rows it reconstructs.  Slices of one geometry form a class; a class takes the
     entry of the kernel its size leads the plan to (32 FP16 slices per item on the
-    matrix cores, sixteen, eight or four FP16 pairs, FP32 pairs or singles)."""
+    matrix cores, sixteen, eight or four FP16 pairs, FP32 pairs or singles).
+
+    A plan runs one launch per (tile, kernel), with the angle chunk of the first
+    slice it groups there, so a batch of several shapes gets one (tile, chunk) per
+    kernel: the configuration with the least summed tuned time over the slices that
+    kernel runs (each slice priced by its nearest entry's timing of every
+    configuration).  For a batch of one shape this is that shape's best entry; an
+    entry without the per-configuration timings keeps its own tile and chunk."""
     t = load() if table is None else table
     key = lambda s: (s.W, s.P, s.center, s.iInterpolation, s.offset_ct, s.out_n, s.tilt,
                      s.fdeg.tobytes())
     counts = {}
     for spec in specs:
         counts[key(spec)] = counts.get(key(spec), 0) + 1
+    per_kernel = {}
     for spec, N in zip(specs, sides):
         P = (spec.P + 1) // 2 if spec.offset_ct else spec.P
-        e = _nearest(N, P, _class_mode(mode, counts[key(spec)]), t)
+        km = _class_mode(mode, counts[key(spec)])
+        e = _nearest(N, P, km, t)
         if e:
             spec.tile = int(e["tile"])
             spec.chunk = int(e.get("chunk", 0))
+            per_kernel.setdefault(km, []).append((spec, e))
+    for members in per_kernel.values():
+        grids = [e.get("all") for _, e in members]
+        if not all(grids):
+            continue
+        common = set.intersection(*(set(g) for g in grids))
+        if not common:
+            continue
+        best = min(sorted(common), key=lambda c: sum(g[c] for g in grids))
+        tile, chunk = (int(v) for v in best.split("/"))
+        for spec, _ in members:
+            spec.tile, spec.chunk = tile, chunk
     return specs
 
 
-def tune_shape(N, P, mode, reps=3, min_items=4096, device=0, log=None):
+def tune_shape(N, P, mode, reps=3, min_items=16384, device=0, log=None):
     """Times every (tile, chunk) of a mode for slices of side N with P projections."""
     import torch
     from . import fbp, phantom
@@ -170,12 +193,12 @@ def _log(msg):
     print(msg, flush=True)
 
 
-def tune(shapes, modes=MODES, path=None, log=_log):
+def tune(shapes, modes=MODES, path=None, log=_log, min_items=16384):
     path = path or TABLE
     table = load(path)
     for mode in modes:
         for N, P in shapes:
-            table[f"{mode}:{N}:{P}"] = tune_shape(N, P, mode, log=log)
+            table[f"{mode}:{N}:{P}"] = tune_shape(N, P, mode, min_items=min_items, log=log)
             with open(path, "w") as f:          # written as it goes (a long sweep keeps its results)
                 json.dump(table, f, indent=1, sort_keys=True)
     return table
@@ -188,6 +211,8 @@ if __name__ == "__main__":
     ap.add_argument("--proj", default="450,900,1800,3600")
     ap.add_argument("--modes", default=",".join(MODES))
     ap.add_argument("--out", default=None, help="write the table here instead of the package")
+    ap.add_argument("--min-items", type=int, default=16384,
+                    help="64x64-tile work items a tuning batch holds at least (capped at 64 slices)")
     a = ap.parse_args()
     shapes = [(int(n), int(p)) for n in a.sides.split(",") for p in a.proj.split(",")]
-    tune(shapes, modes=a.modes.split(","), path=a.out)
+    tune(shapes, modes=a.modes.split(","), path=a.out, min_items=a.min_items)

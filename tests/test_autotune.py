@@ -71,6 +71,40 @@ def test_apply_picks_the_kernel_of_each_class_size():
     assert [(s.tile, s.chunk) for s in specs] == [(1, 16), (1, 16), (2, 48)]
 
 
+def test_apply_one_configuration_per_kernel_in_a_mixed_batch():
+    """Slices of several shapes that run one kernel share its launch, so they get the
+    (tile, chunk) of least summed tuned time; each shape alone keeps its own best."""
+    import numpy as np
+    table = {
+        # 2048-wide: tile 1 chunk 16 best by a little; 512-wide: tile 3 chunk 8 best
+        "linear2:2048:1800": {"tile": 1, "chunk": 16,
+                              "all": {"0/16": 0.62, "1/16": 0.60, "3/8": 0.70, "0/8": 0.63}},
+        "linear2:512:450": {"tile": 3, "chunk": 8,
+                            "all": {"0/16": 0.013, "1/16": 0.014, "3/8": 0.011, "0/8": 0.012}},
+        "linear:2048:1800": {"tile": 2, "chunk": 48},          # no timings: keeps its entry
+    }
+    big = lambda n: [fbp.SliceSpec(2048, np.arange(1800, dtype=np.float32) * 0.1, 1024.37)
+                     for _ in range(n)]
+    small = lambda n: [fbp.SliceSpec(512, np.arange(450, dtype=np.float32) * 0.4, 256.37)
+                       for _ in range(n)]
+    specs = autotune.apply(big(4), [2048] * 4, "linear2", table)
+    assert {(s.tile, s.chunk) for s in specs} == {(1, 16)}
+    specs = autotune.apply(small(4), [512] * 4, "linear2", table)
+    assert {(s.tile, s.chunk) for s in specs} == {(3, 8)}
+    # mixed: 4 x (0.62, 0.60, 0.70, 0.63) + 20 x (0.013, 0.014, 0.011, 0.012)
+    #      = 2.74 (0/16), 2.68 (1/16), 3.02 (3/8), 2.76 (0/8) -> 1/16 for every pair slice
+    specs = autotune.apply(big(4) + small(20), [2048] * 4 + [512] * 20, "linear2", table)
+    assert {(s.tile, s.chunk) for s in specs} == {(1, 16)}
+    # with many small slices the small shape's choice wins: 4 x 0.70 + 400 x 0.011 = 7.2
+    # against 4 x 0.60 + 400 x 0.014 = 8.0 and 4 x 0.62 + 400 x 0.013 = 7.68
+    specs = autotune.apply(big(4) + small(400), [2048] * 4 + [512] * 400, "linear2", table)
+    assert {(s.tile, s.chunk) for s in specs} == {(3, 8)}
+    # a single (class of one) runs the singles kernel, whose entry has no timings
+    one = [fbp.SliceSpec(2048, np.arange(1800, dtype=np.float32) * 0.1, 1030.0)]
+    specs = autotune.apply(big(2) + one, [2048] * 3, "linear2", table)
+    assert [(s.tile, s.chunk) for s in specs] == [(1, 16), (1, 16), (2, 48)]
+
+
 def test_shipped_table_is_per_mode():
     t = autotune.load()
     assert t, "tuned table missing"

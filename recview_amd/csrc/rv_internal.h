@@ -172,7 +172,9 @@ struct BpLaunch {
     int nslices;
     int total_items;        // sum of tiles
     const int* item_order;  // optional permutation of work items (cost-sorted)
-    unsigned int* counter;  // persistent work counter (nullptr: one item per block)
+    unsigned int* counter;  // persistent work: kXcds counters kCounterStride apart, zeroed
+                            // before the launch (nullptr: one item per block)
+    int run;                // persistent work: list positions dealt to the XCDs in runs of this length
     int mode;               // BpMode
     int rule;               // LUT: 0 floor (float mask), 1 trunc
     int chunk;              // angles per LDS chunk
@@ -184,6 +186,23 @@ struct BpLaunch {
                             // (a segment of a launch sequence); 0: all of them
     int tgeom;              // BP_LINEAR2: 1 = every slice has BpSlice::tgeom
 };
+
+// Persistent launches take work-list positions from kXcds counters: the list is
+// dealt to the eight XCDs in runs of consecutive positions, run j belonging to XCD
+// j % 8, whose blocks (blockIdx % 8 is the XCD a block is dispatched to) take its
+// runs from its own counter, then the other XCDs' runs from theirs.  Neighbouring
+// tiles of one slice read overlapping sinogram segments, so an XCD working through
+// a run finds them in its own L2.  Kernels that do not gain from it
+// (bp_deals_runs) get a single run of the whole list: one queue in list order.
+constexpr int kXcds = 8;
+constexpr int kCounterStride = 32;      // counters 128 B apart
+constexpr int kMaxRun = 256;
+inline int work_run(int total_items, bool deal)
+{
+    if (!deal) return total_items < 1 ? 1 : total_items;    // one run: one queue in list order
+    const int r = (total_items + kXcds - 1) / kXcds;
+    return r < 1 ? 1 : r > kMaxRun ? kMaxRun : r;
+}
 
 void launch_bp(const BpLaunch& L, int grid, hipStream_t stream);
 
@@ -246,6 +265,7 @@ int bp_tile_h(int tile);                    // tile height in pixels
 int bp_default_chunk(int mode, int tile);   // angles per chunk
 int bp_blocks_per_cu_tg(int mode, int tile, int chunk, bool tgeom);   // resident workgroups per CU
 bool bp_has_tgeom(int mode);                // the mode has a tile geometry table variant
+bool bp_deals_runs(int mode);               // persistent work dealt to the XCDs in runs
 int bp_max_chunk(int mode);                 // largest chunk a mode supports
 size_t bp_lds_bytes(int mode, int tile, int chunk);
 void launch_zero_u32(unsigned int* p, int n, hipStream_t stream);

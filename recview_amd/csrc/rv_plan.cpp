@@ -74,7 +74,7 @@ struct rv_plan_opaque {
     size_t filt_total = 0;
     bool pair_filter = false;            // LINEAR modes without exact_filter
     rv::AngleEntry* d_ang = nullptr;
-    unsigned int* d_counter = nullptr;   // one per group
+    unsigned int* d_counter = nullptr;   // kXcds counters per group
     void* d_stitch = nullptr;            // stitched offset-CT rows
     int* d_pairs = nullptr;
     unsigned char* d_packed = nullptr;   // multi-slice work items: packed rows
@@ -289,13 +289,15 @@ int launch_backprojection(rv_plan_t p, hipStream_t st, bool pack = true)
         L.nslices = (int)G.slices.size();
         L.total_items = G.total_items;
         L.item_order = G.d_order;
-        L.counter = p->opt.persistent ? p->d_counter + gi : nullptr;
+        unsigned int* ctr = p->d_counter + gi * (size_t)(rv::kXcds * rv::kCounterStride);
+        L.counter = p->opt.persistent ? ctr : nullptr;
+        L.run = rv::work_run(G.total_items, rv::bp_deals_runs(G.mode));
         L.mode = G.mode;
         L.rule = p->opt.index_rule ? 1 : 0;
         L.chunk = G.chunk;
         L.tile = G.tile;
         L.tgeom = G.tgeom ? 1 : 0;
-        if (L.counter) rv::launch_zero_u32(p->d_counter + gi, 1, st);
+        if (L.counter) rv::launch_zero_u32(ctr, rv::kXcds * rv::kCounterStride, st);
         rv::launch_bp(L, G.grid, st);
     }
     return rv::hip_ok(hipGetLastError(), "back-projection launch") ? 0 : -1;
@@ -561,7 +563,8 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
         !rv::hip_ok(hipMalloc(&p->d_scales, sizeof(float) * nang_total), "row scales") ||
         !rv::hip_ok(hipMalloc(&p->d_filts, sizeof(float) * 2 * filt_total), "filters") ||
         !rv::hip_ok(hipMalloc(&p->d_ang, sizeof(rv::AngleEntry) * nang_total), "angle table") ||
-        !rv::hip_ok(hipMalloc(&p->d_counter, 64 * (p->groups.size() + 1)), "work counters"))
+        !rv::hip_ok(hipMalloc(&p->d_counter, sizeof(unsigned int) * rv::kXcds * rv::kCounterStride *
+                                     (p->groups.size() + 1)), "work counters"))
         return fail(p, std::string("plan allocation failed: ") + RecviewLastMessage());
     if (stitch_elems) {
         if (!rv::hip_ok(hipMalloc(&p->d_stitch, in_esz * stitch_elems), "stitch buffer") ||

@@ -477,29 +477,118 @@ hipError_t launch_filter_large(const FilterJob& J, void* work, size_t work_rows,
     return hipGetLastError();
 }
 
-// ------------------------------------------------ paired rows, radix-4 transform
+// ------------------------------------------------ paired rows, radix-16 transform
 // Production filter for the LINEAR modes: rows a and b become the real and
 // imaginary parts of one complex row; with the Hermitian-symmetrised filter
 // Hs[k] = (H[k] + H[n-k]) / 2 (real and even) the result is
 //   y = IFFT(Hs * FFT(a + i b)) = a (*) h + i (b (*) h),  h = IFFT(Hs) = h_re,
 // the same circular convolution the CFft pipeline computes (one-sided filter,
 // real part), in the standard DFT convention.  The transforms are Stockham
-// autosort passes of radix 4 (radix 2 first when log2 n is odd), after
-// Govindaraju et al., "High performance discrete Fourier transforms on
-// graphics processors" (SC08): pass with sub-length Ns reads x[j + r*n/R],
-// twiddles by exp(-2 pi i r (j mod Ns) / (Ns R)), does an R-point DFT and
-// writes x'[(j / Ns) * Ns * R + j mod Ns + r * Ns].  The inverse is
-// conj(FFT(conj(.))) / n, the conjugation folded into the filter multiply.
+// autosort passes after Govindaraju et al., "High performance discrete Fourier
+// transforms on graphics processors" (SC08): a pass of radix R and sub-length
+// Ns reads x[j + r*n/R], twiddles by exp(-2 pi i r (j mod Ns) / (Ns R)), does an
+// R-point DFT and writes x'[(j / Ns) * Ns * R + j mod Ns + r * Ns].  Passes are
+// radix 16 (a thread's 16 points in registers, 4 x 4 Cooley-Tukey inside), the
+// last one radix 8, 4 or 2 when log2 n is not a multiple of 4: a 4096-point
+// transform is three LDS round trips.  The inverse is conj(FFT(conj(.))) / n,
+// the conjugation folded into the filter multiply.
 __device__ __forceinline__ float2 cmulf(float2 a, float2 b)
 {
-    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+    return make_float2(fmaf(a.x, b.x, -(a.y * b.y)), fmaf(a.x, b.y, a.y * b.x));
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+
+// LDS position of element i of a transform: inside each 16-element block the
+// element index is XORed with the low four bits of the block index, and the
+// blocks of each pair swap when bit 4 of the block index is set.  A wave's reads
+// at stride n/R and writes at stride Ns (every pass of every length up to 2^14),
+// and its contiguous loads and stores, then meet each LDS bank the minimum number
+// of times for 64 8-byte accesses, with 32 or 64 banks.
+__device__ __forceinline__ int lds_pos(int i)
+{
+    const int blk = i >> 4;
+    return ((blk ^ ((blk >> 4) & 1)) << 4) | ((i & 15) ^ (blk & 15));
 }
 
-// Forward DFT of buf, n = 2^NP points fixed at compile time (every stride,
-// twiddle step and loop bound is a constant); from 1024 points on T = n/16
-// threads cover each pass's butterflies exactly (no lane guards).
-template <int R, int NP, int NS>
-__device__ __forceinline__ void stockham_pass_ct(float2* buf, const float2* twf, int tid)
+// 4-point forward DFT in place (W4 = -i)
+__device__ __forceinline__ void dft4(float2& x0, float2& x1, float2& x2, float2& x3)
+{
+    const float2 t0 = cadd(x0, x2), t1 = csub(x0, x2), t2 = cadd(x1, x3), t3 = csub(x1, x3);
+    x0 = cadd(t0, t2);
+    x1 = make_float2(t1.x + t3.y, t1.y - t3.x);          // t1 - i t3
+    x2 = csub(t0, t2);
+    x3 = make_float2(t1.x - t3.y, t1.y + t3.x);          // t1 + i t3
+}
+
+// R-point forward DFT of v in place; output k sits in v[dft_slot<R>(k)].
+// R = 16: input p = q + 4 s; 4-point DFTs over s, twiddles W16^(q k1), 4-point
+// DFTs over q, X[k1 + 4 k2] in v[4 k1 + k2].  R = 8: p = q + 2 s, twiddles
+// W8^(q k1), X[k1 + 4 k2] in v[2 k1 + k2].
+template <int R>
+__host__ __device__ constexpr int dft_slot(int k)
+{
+    return R == 16 ? 4 * (k & 3) + (k >> 2) : R == 8 ? 2 * (k & 3) + (k >> 2) : k;
+}
+
+template <int R>
+__device__ __forceinline__ void dft_r(float2 (&v)[R])
+{
+    constexpr float C1 = 0.92387953251128674f, S1 = 0.38268343236508978f, RH = 0.70710678118654752f;
+    if constexpr (R == 16) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) dft4(v[q], v[q + 4], v[q + 8], v[q + 12]);
+        // W16^m = (cos 2 pi m / 16, -sin 2 pi m / 16) for m = q k1
+        v[5] = cmulf(v[5], make_float2(C1, -S1));        // q 1, k1 1: m 1
+        v[9] = cmulf(v[9], make_float2(RH, -RH));        // q 1, k1 2: m 2
+        v[13] = cmulf(v[13], make_float2(S1, -C1));      // q 1, k1 3: m 3
+        v[6] = cmulf(v[6], make_float2(RH, -RH));        // q 2, k1 1: m 2
+        v[10] = make_float2(v[10].y, -v[10].x);          // q 2, k1 2: m 4 (-i)
+        v[14] = cmulf(v[14], make_float2(-RH, -RH));     // q 2, k1 3: m 6
+        v[7] = cmulf(v[7], make_float2(S1, -C1));        // q 3, k1 1: m 3
+        v[11] = cmulf(v[11], make_float2(-RH, -RH));     // q 3, k1 2: m 6
+        v[15] = cmulf(v[15], make_float2(-C1, S1));      // q 3, k1 3: m 9
+#pragma unroll
+        for (int k1 = 0; k1 < 4; k1++) dft4(v[4 * k1], v[4 * k1 + 1], v[4 * k1 + 2], v[4 * k1 + 3]);
+    } else if constexpr (R == 8) {
+        dft4(v[0], v[2], v[4], v[6]);
+        dft4(v[1], v[3], v[5], v[7]);
+        v[3] = cmulf(v[3], make_float2(RH, -RH));        // k1 1: W8^1
+        v[5] = make_float2(v[5].y, -v[5].x);             // k1 2: W8^2 = -i
+        v[7] = cmulf(v[7], make_float2(-RH, -RH));       // k1 3: W8^3
+#pragma unroll
+        for (int k1 = 0; k1 < 4; k1++) {
+            const float2 a = v[2 * k1], b = v[2 * k1 + 1];
+            v[2 * k1] = cadd(a, b);
+            v[2 * k1 + 1] = csub(a, b);
+        }
+    } else if constexpr (R == 4) {
+        dft4(v[0], v[1], v[2], v[3]);
+    } else {
+        const float2 a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    }
+}
+
+// Element access of a pass: the transform's points in LDS (positions lds_pos).
+struct LdsLoad {
+    const float2* buf;
+    __device__ __forceinline__ float2 operator()(int m) const { return buf[lds_pos(m)]; }
+};
+struct LdsStore {
+    float2* buf;
+    __device__ __forceinline__ void operator()(int k, float2 v) const { buf[lds_pos(k)] = v; }
+};
+
+// One Stockham pass of radix R and sub-length NS over n = 2^NP points, by
+// fft_threads(NP) threads: from 1024 points on they cover the pass's n / R
+// butterflies exactly (16 / R each, no lane guards).  Point m comes from ld(m),
+// output k goes to st(k, value); IN_LDS / OUT_LDS say whether those touch the
+// LDS buffer, which decides the barriers: LDS reads finish before LDS writes
+// (the passes work in place), and LDS writes before the next pass reads.
+template <int R, int NP, int NS, bool IN_LDS, bool OUT_LDS, class Ld, class St>
+__device__ __forceinline__ void stockham_pass(const float2* twf, int tid, const Ld& ld, const St& st)
 {
     constexpr int n = 1 << NP, T = fft_threads(NP), nb = n / R, NB = (nb + T - 1) / T;
     constexpr int tstep = n / (NS * R);
@@ -510,62 +599,60 @@ __device__ __forceinline__ void stockham_pass_ct(float2* buf, const float2* twf,
         if (nb % T == 0 || j < nb) {
             const int jm = j & (NS - 1);
 #pragma unroll
-            for (int r = 0; r < R; r++) {
-                float2 x = buf[j + r * nb];
-                if (r > 0) x = cmulf(x, twf[(r * jm * tstep) & (n - 1)]);
-                v[i][r] = x;
+            for (int r = 0; r < R; r++) v[i][r] = ld(j + r * nb);
+            if constexpr (NS > 1) {
+                // w^r for w = exp(-2 pi i jm tstep / n): w, w^2, w^4, w^8 from the table
+                // (rounded from double), the others as products of at most four of them
+                float2 w[R];
+                const int e = jm * tstep;                       // (R - 1) e < n
+#pragma unroll
+                for (int b = 1; b < R; b <<= 1) w[b] = twf[b * e];
+#pragma unroll
+                for (int r = 3; r < R; r++)
+                    if (r & (r - 1)) w[r] = cmulf(w[r & (r - 1)], w[r & -r]);
+#pragma unroll
+                for (int r = 1; r < R; r++) v[i][r] = cmulf(v[i][r], w[r]);
             }
         }
     }
-    __syncthreads();
+    if constexpr (IN_LDS && OUT_LDS) __syncthreads();
 #pragma unroll
     for (int i = 0; i < NB; i++) {
         const int j = tid + i * T;
         if (nb % T == 0 || j < nb) {
             const int jm = j & (NS - 1);
             const int d = (j - jm) * R + jm;
-            if (R == 2) {
-                buf[d] = make_float2(v[i][0].x + v[i][1].x, v[i][0].y + v[i][1].y);
-                buf[d + NS] = make_float2(v[i][0].x - v[i][1].x, v[i][0].y - v[i][1].y);
-            } else {
-                const float2 t0 = make_float2(v[i][0].x + v[i][2].x, v[i][0].y + v[i][2].y);
-                const float2 t1 = make_float2(v[i][0].x - v[i][2].x, v[i][0].y - v[i][2].y);
-                const float2 t2 = make_float2(v[i][1].x + v[i][3].x, v[i][1].y + v[i][3].y);
-                const float2 t3 = make_float2(v[i][1].x - v[i][3].x, v[i][1].y - v[i][3].y);
-                buf[d] = make_float2(t0.x + t2.x, t0.y + t2.y);
-                buf[d + NS] = make_float2(t1.x + t3.y, t1.y - t3.x);          // t1 - i t3
-                buf[d + 2 * NS] = make_float2(t0.x - t2.x, t0.y - t2.y);
-                buf[d + 3 * NS] = make_float2(t1.x - t3.y, t1.y + t3.x);      // t1 + i t3
-            }
+            dft_r<R>(v[i]);
+#pragma unroll
+            for (int r = 0; r < R; r++) st(d + r * NS, v[i][dft_slot<R>(r)]);
         }
     }
-    __syncthreads();
+    if constexpr (OUT_LDS) __syncthreads();
 }
 
-template <int NP, int NS>
-__device__ __forceinline__ void radix4_passes_ct(float2* buf, const float2* twf, int tid)
+// Forward DFT of n = 2^NP points: radix-16 passes, then one of radix 8, 4 or 2.
+// The first pass reads its points through ld (IN_LDS: from buf), the last pass
+// writes through st (OUT_LDS: into buf); the passes between use buf.
+template <int NP, int NS, int REM, bool IN_LDS, bool OUT_LDS, class Ld, class St>
+__device__ __forceinline__ void fft_r16(float2* buf, const float2* twf, int tid, const Ld& ld, const St& st)
 {
-    if constexpr (NS < (1 << NP)) {
-        stockham_pass_ct<4, NP, NS>(buf, twf, tid);
-        radix4_passes_ct<NP, NS * 4>(buf, twf, tid);
-    }
-}
-
-template <int NP>
-__device__ __forceinline__ void fft_radix4_ct(float2* buf, const float2* twf, int tid)
-{
-    if constexpr (NP & 1) {
-        stockham_pass_ct<2, NP, 1>(buf, twf, tid);
-        radix4_passes_ct<NP, 2>(buf, twf, tid);
-    } else {
-        radix4_passes_ct<NP, 1>(buf, twf, tid);
-    }
+    constexpr int LR = REM >= 4 ? 4 : REM, R = 1 << LR;
+    constexpr bool first = NS == 1, last = REM == LR;
+    const LdsLoad lds_ld{buf};
+    const LdsStore lds_st{buf};
+    if constexpr (first && last) stockham_pass<R, NP, NS, IN_LDS, OUT_LDS>(twf, tid, ld, st);
+    else if constexpr (first) stockham_pass<R, NP, NS, IN_LDS, true>(twf, tid, ld, lds_st);
+    else if constexpr (last) stockham_pass<R, NP, NS, true, OUT_LDS>(twf, tid, lds_ld, st);
+    else stockham_pass<R, NP, NS, true, true>(twf, tid, lds_ld, lds_st);
+    if constexpr (!last) fft_r16<NP, (NS << LR), REM - LR, IN_LDS, OUT_LDS>(buf, twf, tid, ld, st);
 }
 
 // NP: log2 of the transform length (compile time); launched with
-// fft_threads(NP) threads.
+// fft_threads(NP) threads.  The forward transform reads the prepared rows from
+// global memory in its first pass and applies conj(Hs * X) in its last; with
+// FP32 rows the inverse transform's last pass writes the cropped columns.
 template <int IN_KIND, int NP>
-__global__ void __launch_bounds__(1024) filter_pair_kernel(FilterJob J, FilterBatch FB)
+__global__ void __launch_bounds__(fft_threads(NP), 4) filter_pair_kernel(FilterJob J, FilterBatch FB)
 {
     constexpr int n = 1 << NP, T = fft_threads(NP);
     extern __shared__ float2 buf[];
@@ -584,28 +671,32 @@ __global__ void __launch_bounds__(1024) filter_pair_kernel(FilterJob J, FilterBa
     const void* row_b = (IN_KIND == 0)
         ? (const void*)((const int16_t*)sino + (long long)rin_b * J.in_stride)
         : (const void*)((const float*)sino + (long long)rin_b * J.in_stride);
-    for (int m = tid; m < n; m += T)
-        buf[m] = make_float2(prep_value<IN_KIND>(J, row, m), hasb ? prep_value<IN_KIND>(J, row_b, m) : 0.f);
-    __syncthreads();
-    fft_radix4_ct<NP>(buf, twf, tid);
-    for (int k = tid; k < n; k += T) {            // conj(Hs * X)
-        const float h = J.filt[k];
-        const float2 x = buf[k];
-        buf[k] = make_float2(x.x * h, -(x.y * h));
-    }
-    __syncthreads();
-    fft_radix4_ct<NP>(buf, twf, tid);
+    const float* filt = J.filt;
+    auto prep = [&](int m) {
+        return make_float2(prep_value<IN_KIND>(J, row, m), hasb ? prep_value<IN_KIND>(J, row_b, m) : 0.f);
+    };
+    auto filter = [&](int k, float2 x) {          // conj(Hs * X)
+        const float h = filt[k];
+        buf[lds_pos(k)] = make_float2(x.x * h, -(x.y * h));
+    };
+    fft_r16<NP, 1, NP, false, true>(buf, twf, tid, prep, filter);
     // y = conj(buf) / n; rows: real part -> a, imaginary part -> b
     const float a = 1.f / (float)n;
+    const LdsLoad lds_ld{buf};
     if (J.store == 0) {
         float* o = FB.out[bi] + (long long)rout * J.out_stride;
         float* ob = FB.out[bi] + (long long)rout_b * J.out_stride;
-        for (int c = tid; c < J.N; c += T) {
-            const float2 y = buf[c + J.crop];
-            o[c] = y.x * a;
-            if (hasb) ob[c] = -y.y * a;
-        }
+        const int crop = J.crop, N = J.N;
+        auto out = [&](int k, float2 y) {
+            const int c = k - crop;
+            if (c >= 0 && c < N) {
+                o[c] = y.x * a;
+                if (hasb) ob[c] = -y.y * a;
+            }
+        };
+        fft_r16<NP, 1, NP, true, false>(buf, twf, tid, lds_ld, out);
     } else {
+        fft_r16<NP, 1, NP, true, true>(buf, twf, tid, lds_ld, LdsStore{buf});
         // The cropped width N is below n, so n / T values per thread cover
         // every column (N > n/2 for any width that is not a power of two).
         constexpr int NV = (n + T - 1) / T;
@@ -614,7 +705,7 @@ __global__ void __launch_bounds__(1024) filter_pair_kernel(FilterJob J, FilterBa
         for (int i = 0; i < NV; i++) {
             const int c = tid + i * T;
             if (c < J.N) {
-                const float2 y = buf[c + J.crop];
+                const float2 y = buf[lds_pos(c + J.crop)];
                 va[i] = y.x * a;
                 vb[i] = -y.y * a;
             }

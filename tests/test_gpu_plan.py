@@ -616,6 +616,30 @@ def test_plan_filtered_rows_non_pow2(W, interp):
                 assert np.abs(got - ref64).max() <= 4 * e_ref, (np.abs(got - ref64).max(), e_ref)
 
 
+def test_plan_pair_filter_every_transform_length():
+    """The paired-row filter at every transform length it runs in LDS, 2^3 .. 2^14
+    points (radix-16 passes, then a radix-8, -4 or -2 pass, or none), both rows of
+    each pair against the FP64 filter and as accurate as the reference's own FP32
+    filter; an odd row count leaves the last transform with one row."""
+    P = 7
+    for W in (5, 9, 20, 40, 70, 130, 260, 520, 1040, 2080, 4100, 8192):
+        s, fdeg, c = _sino(W, P, 2300 + W)
+        g = O.geometry(W, 2, c)
+        ref64 = O.filter_rows_f64(s, g, O.set_filter(g.ndim, 0))
+        rmax = np.abs(ref64).max(axis=1, keepdims=True)
+        plan = fbp.FbpPlan([fbp.SliceSpec(W, fdeg, c)], store="f32")
+        plan.filter([torch.from_numpy(s).cuda()])
+        torch.cuda.synchronize()
+        got = plan.filtered_rows_tensor(0).cpu().numpy()
+        plan.close()
+        err = np.abs(got - ref64) / rmax
+        e_ref = np.abs(O.filter_rows(s, g, O.set_filter(g.ndim, 0), kind=1) - ref64).max()
+        print(f"W={W} n={g.ndim}: max {err.max():.2e} of the row maximum, abs "
+              f"{np.abs(got - ref64).max():.2e} vs reference FP32 {e_ref:.2e}")
+        assert err.max() <= 1e-4, W
+        assert np.abs(got - ref64).max() <= 4 * e_ref, W
+
+
 @pytest.mark.parametrize("W,nsl,store,lds_half,rms_tol", [
     (1000, 8, "f16", True, 1e-4), (1000, 8, "bf16", True, 2e-3), (1000, 4, "f16", False, 1e-4),
     (2000, 2, "f16", True, 1e-4), (2000, 2, "bf16", False, 2e-3), (2000, 2, "f32", False, 2e-6),

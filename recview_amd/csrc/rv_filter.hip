@@ -578,13 +578,14 @@ struct LdsLoad {
 };
 struct LdsStore {
     float2* buf;
-    __device__ __forceinline__ void operator()(int k, float2 v) const { buf[lds_pos(k)] = v; }
+    __device__ __forceinline__ void operator()(int, int, int k, float2 v) const { buf[lds_pos(k)] = v; }
 };
 
 // One Stockham pass of radix R and sub-length NS over n = 2^NP points, by
 // fft_threads(NP) threads: from 1024 points on they cover the pass's n / R
 // butterflies exactly (16 / R each, no lane guards).  Point m comes from ld(m),
-// output k goes to st(k, value); IN_LDS / OUT_LDS say whether those touch the
+// output k of butterfly i, slot r goes to st(i, r, k, value) (i, r compile-time
+// after unrolling, so a store may keep it in registers); IN_LDS / OUT_LDS say whether those touch the
 // LDS buffer, which decides the barriers: LDS reads finish before LDS writes
 // (the passes work in place), and LDS writes before the next pass reads.
 template <int R, int NP, int NS, bool IN_LDS, bool OUT_LDS, class Ld, class St>
@@ -624,7 +625,7 @@ __device__ __forceinline__ void stockham_pass(const float2* twf, int tid, const 
             const int d = (j - jm) * R + jm;
             dft_r<R>(v[i]);
 #pragma unroll
-            for (int r = 0; r < R; r++) st(d + r * NS, v[i][dft_slot<R>(r)]);
+            for (int r = 0; r < R; r++) st(i, r, d + r * NS, v[i][dft_slot<R>(r)]);
         }
     }
     if constexpr (OUT_LDS) __syncthreads();
@@ -649,9 +650,11 @@ __device__ __forceinline__ void fft_r16(float2* buf, const float2* twf, int tid,
 
 // NP: log2 of the transform length (compile time); launched with
 // fft_threads(NP) threads.  The forward transform reads the prepared rows from
-// global memory in its first pass and applies conj(Hs * X) in its last; with
-// FP32 rows the inverse transform's last pass writes the cropped columns.
-template <int IN_KIND, int NP>
+// global memory in its first pass and applies conj(Hs * X) in its last; the
+// inverse transform's last pass writes the cropped columns (LP false: FP32 rows,
+// J.store 0) or keeps them in registers for the row scales (LP true: 16-bit
+// rows, J.store 1 or 2).
+template <int IN_KIND, int NP, bool LP>
 __global__ void __launch_bounds__(fft_threads(NP), 4) filter_pair_kernel(FilterJob J, FilterBatch FB)
 {
     constexpr int n = 1 << NP, T = fft_threads(NP);
@@ -675,7 +678,7 @@ __global__ void __launch_bounds__(fft_threads(NP), 4) filter_pair_kernel(FilterJ
     auto prep = [&](int m) {
         return make_float2(prep_value<IN_KIND>(J, row, m), hasb ? prep_value<IN_KIND>(J, row_b, m) : 0.f);
     };
-    auto filter = [&](int k, float2 x) {          // conj(Hs * X)
+    auto filter = [&](int, int, int k, float2 x) {          // conj(Hs * X)
         const float h = filt[k];
         buf[lds_pos(k)] = make_float2(x.x * h, -(x.y * h));
     };
@@ -683,11 +686,11 @@ __global__ void __launch_bounds__(fft_threads(NP), 4) filter_pair_kernel(FilterJ
     // y = conj(buf) / n; rows: real part -> a, imaginary part -> b
     const float a = 1.f / (float)n;
     const LdsLoad lds_ld{buf};
-    if (J.store == 0) {
+    if constexpr (!LP) {
         float* o = FB.out[bi] + (long long)rout * J.out_stride;
         float* ob = FB.out[bi] + (long long)rout_b * J.out_stride;
         const int crop = J.crop, N = J.N;
-        auto out = [&](int k, float2 y) {
+        auto out = [&](int, int, int k, float2 y) {
             const int c = k - crop;
             if (c >= 0 && c < N) {
                 o[c] = y.x * a;
@@ -696,31 +699,69 @@ __global__ void __launch_bounds__(fft_threads(NP), 4) filter_pair_kernel(FilterJ
         };
         fft_r16<NP, 1, NP, true, false>(buf, twf, tid, lds_ld, out);
     } else {
-        fft_r16<NP, 1, NP, true, true>(buf, twf, tid, lds_ld, LdsStore{buf});
-        // The cropped width N is below n, so n / T values per thread cover
-        // every column (N > n/2 for any width that is not a power of two).
-        constexpr int NV = (n + T - 1) / T;
-        float va[NV], vb[NV];
+        // 16-bit rows: the inverse's last pass keeps its outputs in registers; per row
+        // a power-of-two scale so that |value| / scale <= 2^14 (the largest cropped
+        // value over the workgroup), then the scaled values stored from registers
+        constexpr int LRL = NP % 4 == 0 ? 4 : NP % 4, RL = 1 << LRL;     // the last pass
+        constexpr int NSL = n / RL, NBL = (NSL + T - 1) / T;
+        float2 keep[NBL][RL];
+        auto hold = [&](int i, int r, int, float2 y) { keep[i][r] = y; };
+        fft_r16<NP, 1, NP, true, false>(buf, twf, tid, lds_ld, hold);
+        // output k of butterfly i, slot r of the last pass: j = tid + i T < NSL, k = j + r NSL
+        const int crop = J.crop, N = J.N;
+        float ma = 0.f, mb = 0.f;
 #pragma unroll
-        for (int i = 0; i < NV; i++) {
-            const int c = tid + i * T;
-            if (c < J.N) {
-                const float2 y = buf[lds_pos(c + J.crop)];
-                va[i] = y.x * a;
-                vb[i] = -y.y * a;
+        for (int i = 0; i < NBL; i++) {
+            const int j = tid + i * T;
+#pragma unroll
+            for (int r = 0; r < RL; r++) {
+                const int c = j + r * NSL - crop;
+                if ((NSL % T == 0 || j < NSL) && c >= 0 && c < N) {
+                    ma = fmaxf(ma, fabsf(keep[i][r].x * a));
+                    mb = fmaxf(mb, fabsf(-keep[i][r].y * a));
+                }
             }
         }
-        __syncthreads();
-        float* outf = (float*)buf;
-        float* outf_b = outf + J.N;
-#pragma unroll
-        for (int i = 0; i < NV; i++) {
-            const int c = tid + i * T;
-            if (c < J.N) { outf[c] = va[i]; outf_b[c] = vb[i]; }
+        for (int off = 32; off > 0; off >>= 1) {
+            ma = fmaxf(ma, __shfl_xor(ma, off));
+            mb = fmaxf(mb, __shfl_xor(mb, off));
         }
-        __shared__ float red[32];
-        store_lp_row(J, FB.out_lp[bi], FB.out_scale[bi], outf, rout, red);
-        if (hasb) store_lp_row(J, FB.out_lp[bi], FB.out_scale[bi], outf_b, rout_b, red);
+        __shared__ float red[2 * 16 + 2];
+        if ((tid & 63) == 0) { red[tid >> 6] = ma; red[16 + (tid >> 6)] = mb; }
+        __syncthreads();
+        if (tid == 0) {
+            float m2a = 0.f, m2b = 0.f;
+            for (int w = 0; w < T / 64; w++) { m2a = fmaxf(m2a, red[w]); m2b = fmaxf(m2b, red[16 + w]); }
+            int ea = 0, eb = 0;
+            if (m2a > 0.f) frexpf(m2a / 16384.f, &ea);
+            if (m2b > 0.f) frexpf(m2b / 16384.f, &eb);
+            red[32] = ldexpf(1.f, ea);
+            red[33] = ldexpf(1.f, eb);
+            FB.out_scale[bi][rout] = red[32];
+            if (hasb) FB.out_scale[bi][rout_b] = red[33];
+        }
+        __syncthreads();
+        const float inva = 1.f / red[32], invb = 1.f / red[33];      // powers of two: exact
+#pragma unroll
+        for (int i = 0; i < NBL; i++) {
+            const int j = tid + i * T;
+#pragma unroll
+            for (int r = 0; r < RL; r++) {
+                const int c = j + r * NSL - crop;
+                if ((NSL % T == 0 || j < NSL) && c >= 0 && c < N) {
+                    const float va = keep[i][r].x * a, vb = -keep[i][r].y * a;
+                    if (J.store == 1) {
+                        __half* o = (__half*)FB.out_lp[bi];
+                        o[(long long)rout * J.out_stride + c] = __float2half_rn(va * inva);
+                        if (hasb) o[(long long)rout_b * J.out_stride + c] = __float2half_rn(vb * invb);
+                    } else {
+                        __hip_bfloat16* o = (__hip_bfloat16*)FB.out_lp[bi];
+                        o[(long long)rout * J.out_stride + c] = __float2bfloat16(va * inva);
+                        if (hasb) o[(long long)rout_b * J.out_stride + c] = __float2bfloat16(vb * invb);
+                    }
+                }
+            }
+        }
     }
 }
 
@@ -740,9 +781,13 @@ void prepare_filter(int n)
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     (void)hipFuncSetAttribute((const void*)&filter_rows_kernel<1, 14>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    (void)hipFuncSetAttribute((const void*)&filter_pair_kernel<0, 14>,
+    (void)hipFuncSetAttribute((const void*)&filter_pair_kernel<0, 14, false>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    (void)hipFuncSetAttribute((const void*)&filter_pair_kernel<1, 14>,
+    (void)hipFuncSetAttribute((const void*)&filter_pair_kernel<1, 14, false>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    (void)hipFuncSetAttribute((const void*)&filter_pair_kernel<0, 14, true>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    (void)hipFuncSetAttribute((const void*)&filter_pair_kernel<1, 14, true>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     done[dev] = n;
 }
@@ -753,10 +798,15 @@ static void launch_np(const FilterJob& job, const FilterBatch& fb, int count, si
     const dim3 T(fft_threads(NP));
     if (job.pair) {
         const dim3 g((job.nrows + 1) / 2, count);
-        if (job.in_kind == 0)
-            hipLaunchKernelGGL((filter_pair_kernel<0, NP>), g, T, lds, stream, job, fb);
+        const bool lp = job.store != 0;
+        if (job.in_kind == 0 && !lp)
+            hipLaunchKernelGGL((filter_pair_kernel<0, NP, false>), g, T, lds, stream, job, fb);
+        else if (job.in_kind == 0)
+            hipLaunchKernelGGL((filter_pair_kernel<0, NP, true>), g, T, lds, stream, job, fb);
+        else if (!lp)
+            hipLaunchKernelGGL((filter_pair_kernel<1, NP, false>), g, T, lds, stream, job, fb);
         else
-            hipLaunchKernelGGL((filter_pair_kernel<1, NP>), g, T, lds, stream, job, fb);
+            hipLaunchKernelGGL((filter_pair_kernel<1, NP, true>), g, T, lds, stream, job, fb);
     } else {
         const dim3 g(job.nrows, count);
         if (job.in_kind == 0)

@@ -19,6 +19,7 @@
 // the upsampled channels first, so e_l is written straight into its slice.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -251,17 +252,32 @@ static bool forward(Unet* u, const float* in, float scale, hipStream_t s, bool m
     const long long n0 = (long long)u->B * u->H * u->W;
     if (!ok(launch_scale(in, scale, u->x, n0, s), "scale")) return false;
     const ULayer& Li = u->layers[u->li_in()];
-    if (!ok(launch_edge_in(u->x, u->P + Li.w, u->P + Li.b, u->C(0), u->ea[0], u->C(0), 0, u->B, u->H,
-                           u->W, s), "edge conv"))
-        return false;
     // e_l into the skip slice of cat[l] (or the bottleneck tensor)
     auto e_ptr = [&](int l) { return l < Lv - 1 ? u->cat[l] : u->bott; };
     auto e_stride = [&](int l) { return l < Lv - 1 ? u->cat_c(l) : u->C(l); };
     auto e_off = [&](int l) { return l < Lv - 1 ? u->C(l + 1) : 0; };
-    // e_l and, below the bottleneck, its 2x2 average pool come out of one launch
-    if (!conv_fwd(u, u->li_enc(0, 1), u->ea[0], u->C(0), 0, e_ptr(0), e_stride(0), e_off(0), 0, s,
-                  Lv > 1 ? u->pool[0] : nullptr))
+    // e_0 and, below the bottleneck, its 2x2 average pool come out of one launch;
+    // inference with 32 base channels also computes the edge convolution in that
+    // launch (training keeps its output for the backward pass)
+    const char* fused_env = std::getenv("RECVIEW_UNET_FUSED");
+    const bool fuse = !materialize && u->C(0) == 32 && !(fused_env && std::strcmp(fused_env, "0") == 0);
+    if (fuse) {
+        const ULayer& L = u->layers[u->li_enc(0, 1)];
+        ConvArgs a{};
+        a.in = nullptr; a.in_stride = 0; a.in_off = 0; a.Cin = L.cin;
+        a.wfrag = L.wf; a.bias = u->P + L.b;
+        a.out = e_ptr(0); a.out_stride = e_stride(0); a.out_off = e_off(0); a.Cout = L.cout;
+        a.relu = 1;
+        a.B = u->B; a.H = u->H; a.W = u->W;
+        a.pool_out = Lv > 1 ? u->pool[0] : nullptr;
+        if (!ok(launch_edge_conv3x3_c32(a, u->x, u->P + Li.w, u->P + Li.b, s), "edge conv + conv3x3"))
+            return false;
+    } else if (!ok(launch_edge_in(u->x, u->P + Li.w, u->P + Li.b, u->C(0), u->ea[0], u->C(0), 0, u->B,
+                                  u->H, u->W, s), "edge conv") ||
+               !conv_fwd(u, u->li_enc(0, 1), u->ea[0], u->C(0), 0, e_ptr(0), e_stride(0), e_off(0), 0, s,
+                         Lv > 1 ? u->pool[0] : nullptr)) {
         return false;
+    }
     for (int l = 1; l < Lv; l++) {
         if (!conv_fwd(u, u->li_enc(l, 0), u->pool[l - 1], u->C(l - 1), 0, u->ea[l], u->C(l), 0, l, s) ||
             !conv_fwd(u, u->li_enc(l, 1), u->ea[l], u->C(l), 0, e_ptr(l), e_stride(l), e_off(l), l, s,

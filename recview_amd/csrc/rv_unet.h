@@ -33,6 +33,12 @@ struct ConvArgs {
     uint16_t* pool_out;
 };
 hipError_t launch_conv3x3(const ConvArgs& a, hipStream_t s);
+// The edge convolution (1 -> 32, weights ew [32][3][3], biases eb, ReLU) of the
+// FP32 image x (B x H x W) fused into the 32 -> 32 convolution a that reads its
+// output: a.in is not read, a.Cin = a.Cout = 32, no mask and no upsampled input.
+// Bit-identical to launch_edge_in followed by launch_conv3x3.
+hipError_t launch_edge_conv3x3_c32(const ConvArgs& a, const float* x, const float* ew, const float* eb,
+                                   hipStream_t s);
 
 // Weight gradient of the same convolution: for every split s (grid z) writes
 //   part[s][co][tap][ci] = sum over the split's pixels of dY[p][co] * X[p + d(tap)][ci]

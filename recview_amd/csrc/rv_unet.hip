@@ -344,6 +344,184 @@ __global__ __launch_bounds__(256, 2) void conv3x3_c32_kernel(ConvArgs a) {
     }
 }
 
+// Encoder level 0 in one pass for 32 base channels: the edge convolution
+// (1 -> 32, ReLU) of each patch's 18 x 18 halo is computed from a 20 x 20 tile
+// of the input image in LDS, in edge_in_kernel's operation order (bias, the
+// nine taps by fmaf, ReLU, bf16), straight into the halo tile of the 32 -> 32
+// convolution; the MFMA loop and the epilogue (bias, ReLU, 2x2 pool, stores)
+// are conv3x3_c32_kernel's.  The 32-channel edge output never goes to HBM.
+// Halo item i = t + 256 k covers pixel i / 4 and channel group i % 4 = t % 4,
+// so a thread keeps one channel group's 72 edge weights in registers.
+template <int PPB>
+__global__ __launch_bounds__(256, 2) void edge_c32_kernel(ConvArgs a, const float* x, const float* ew,
+                                                          const float* eb) {
+    constexpr int NF = 2;
+    constexpr int HSZ = HALO_PIX * HALO_PITCH;
+    constexpr int WSZ = NF * 576 * 8;
+    constexpr int TC = 16 * NF;
+    constexpr int OPITCH = TC + 8;
+    constexpr int W_ITEMS = NF * 576;
+    constexpr int W_PER = (W_ITEMS + 255) / 256;
+    constexpr int XW = 20, XPIX = (CONV_ROWS + 4) * XW;     // input tile: halo 2 around the patch
+    constexpr int X_PER = (XPIX + 255) / 256;
+    __shared__ __attribute__((aligned(16))) uint16_t smem[HSZ + WSZ + 256 * OPITCH];
+    __shared__ float xs[XPIX];
+    uint16_t* in_s = smem;
+    uint16_t* w_s = smem + HSZ;
+    uint16_t* o_s = w_s + WSZ;
+    const int tiles_x = (a.W + 15) >> 4;
+    const int ntiles = tiles_x * ((a.H + CONV_ROWS - 1) / CONV_ROWS);
+    const int tile0 = blockIdx.x * PPB;
+    const int npatch = ntiles - tile0 < PPB ? ntiles - tile0 : PPB;
+    const int bb = blockIdx.z;
+    const int t = threadIdx.x, l = t & 63, w = t >> 6;
+    const float* xb = x + (long long)bb * a.H * a.W;
+
+#pragma unroll
+    for (int k = 0; k < W_PER; k++) {
+        const int i = t + 256 * k;
+        if (i < W_ITEMS) st16(w_s + i * 8, ld16(a.wfrag + (long long)i * 8));
+    }
+    const int g = t & 3;
+    float ewr[8][9], ebr[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        ebr[e] = eb[8 * g + e];
+#pragma unroll
+        for (int tap = 0; tap < 9; tap++) ewr[e][tap] = ew[(8 * g + e) * 9 + tap];
+    }
+    float xreg[X_PER];
+    auto fetch = [&](int p) {
+        const int tile = tile0 + p;
+        const int x0 = (tile % tiles_x) * 16, y0 = (tile / tiles_x) * CONV_ROWS;
+#pragma unroll
+        for (int k = 0; k < X_PER; k++) {
+            const int i = t + 256 * k;
+            const int ty = i / XW, tx = i - ty * XW;
+            const int y = y0 - 2 + ty, xx = x0 - 2 + tx;
+            xreg[k] = (i < XPIX && y >= 0 && y < a.H && xx >= 0 && xx < a.W) ? xb[(long long)y * a.W + xx] : 0.f;
+        }
+    };
+    fetch(0);
+    for (int p = 0; p < npatch; p++) {
+        const int tile = tile0 + p;
+        const int x0 = (tile % tiles_x) * 16, y0 = (tile / tiles_x) * CONV_ROWS;
+#pragma unroll
+        for (int k = 0; k < X_PER; k++) {
+            const int i = t + 256 * k;
+            if (i < XPIX) xs[i] = xreg[k];
+        }
+        __syncthreads();
+        if (p + 1 < npatch) fetch(p + 1);
+        // edge convolution of the halo: zero outside the image (the 32 -> 32
+        // convolution's padding), the tile's zeros are the edge layer's padding
+#pragma unroll
+        for (int k = 0; k < HALO_PER; k++) {
+            const int i = t + 256 * k;
+            if (i < HALO_ITEMS) {
+                const int hp = i >> 2;
+                const int hy = hp / 18, hx = hp - hy * 18;
+                const int y = y0 - 1 + hy, xx = x0 - 1 + hx;
+                uint4 r = make_uint4(0, 0, 0, 0);
+                if (y >= 0 && y < a.H && xx >= 0 && xx < a.W) {
+                    float v[9];
+#pragma unroll
+                    for (int tap = 0; tap < 9; tap++) v[tap] = xs[(hy + tap / 3) * XW + hx + tap % 3];
+                    uint16_t* r8 = (uint16_t*)&r;
+#pragma unroll
+                    for (int e = 0; e < 8; e++) {
+                        float s = ebr[e];
+#pragma unroll
+                        for (int tap = 0; tap < 9; tap++) s = fmaf(ewr[e][tap], v[tap], s);
+                        r8[e] = f2bf(fmaxf(s, 0.f));
+                    }
+                }
+                st16(in_s + hp * HALO_PITCH + g * 8, r);
+            }
+        }
+        __syncthreads();
+        floatx4 acc[4][NF];
+#pragma unroll
+        for (int m = 0; m < 4; m++)
+#pragma unroll
+            for (int f = 0; f < NF; f++) acc[m][f] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int tap = 0; tap < 9; tap++) {
+            const int ky = tap / 3, kx = tap % 3;
+            bf16x8 av[4], bv[NF];
+#pragma unroll
+            for (int m = 0; m < 4; m++)
+                av[m] = *(const bf16x8*)(in_s + ((4 * w + m + ky) * 18 + (l & 15) + kx) * HALO_PITCH +
+                                         8 * (l >> 4));
+#pragma unroll
+            for (int f = 0; f < NF; f++) bv[f] = *(const bf16x8*)(w_s + ((f * 9 + tap) * 64 + l) * 8);
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int f = 0; f < NF; f++)
+#pragma unroll
+                for (int m = 0; m < 4; m++)
+                    acc[m][f] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[m], bv[f], acc[m][f], 0, 0, 0);
+        }
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+            const int co = f * 16 + (l & 15);
+            const float bias = a.bias ? a.bias[co] : 0.f;
+#pragma unroll
+            for (int m = 0; m < 4; m++)
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    float v = acc[m][f][i] + bias;
+                    if (a.relu) v = fmaxf(v, 0.f);
+                    o_s[((4 * w + m) * 16 + 4 * (l >> 4) + i) * OPITCH + co] = f2bf(v);
+                }
+        }
+        __syncthreads();
+        if (a.pool_out) {
+            const int Ho = a.H >> 1, Wo = a.W >> 1;
+            for (int it = t; it < 64 * (TC / 8); it += 256) {
+                const int q = it / (TC / 8), gg = it - q * (TC / 8);
+                const int py = q >> 3, pxx = q & 7;
+                const int yo = (y0 >> 1) + py, xo = (x0 >> 1) + pxx;
+                if (yo >= Ho || xo >= Wo) continue;
+                float sum[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const int px = (2 * py + (k >> 1)) * 16 + 2 * pxx + (k & 1);
+                    const uint4 v = ld16(o_s + px * OPITCH + 8 * gg);
+                    const uint16_t* v8 = (const uint16_t*)&v;
+#pragma unroll
+                    for (int e = 0; e < 8; e++) sum[e] += bf2f(v8[e]);
+                }
+                uint4 r;
+                uint16_t* r8 = (uint16_t*)&r;
+#pragma unroll
+                for (int e = 0; e < 8; e++) r8[e] = f2bf(0.25f * sum[e]);
+                st16(a.pool_out + (((long long)bb * Ho + yo) * Wo + xo) * a.Cout + 8 * gg, r);
+            }
+        }
+        for (int it = t; it < 256 * (TC / 8); it += 256) {
+            const int px = it / (TC / 8), gg = it - px * (TC / 8);
+            const int y = y0 + (px >> 4), xx = x0 + (px & 15);
+            if (y >= a.H || xx >= a.W) continue;
+            const long long pix = ((long long)bb * a.H + y) * a.W + xx;
+            st16(a.out + pix * a.out_stride + a.out_off + 8 * gg, ld16(o_s + px * OPITCH + 8 * gg));
+        }
+        // the next patch's tile and halo stores and output-tile writes are each
+        // separated from this patch's reads by one barrier
+    }
+}
+
+hipError_t launch_edge_conv3x3_c32(const ConvArgs& a, const float* x, const float* ew, const float* eb,
+                                   hipStream_t s) {
+    if (a.Cin != 32 || a.Cout != 32 || a.out_stride % 8 || a.out_off % 8 || a.B <= 0 || a.H <= 0 ||
+        a.W <= 0 || a.out_off + a.Cout > a.out_stride || a.up || a.Cup || a.mask ||
+        (a.pool_out && (a.H % 2 || a.W % 2)))
+        return hipErrorInvalidValue;
+    const int tiles = ((a.W + 15) / 16) * ((a.H + CONV_ROWS - 1) / CONV_ROWS);
+    hipLaunchKernelGGL((edge_c32_kernel<4>), dim3((tiles + 3) / 4, 1, a.B), dim3(256), 0, s, a, x, ew, eb);
+    return hipGetLastError();
+}
+
 hipError_t launch_conv3x3(const ConvArgs& a, hipStream_t s) {
     if (a.Cin % 32 || a.Cout % 32 || a.in_stride % 8 || a.in_off % 8 || a.out_stride % 8 ||
         a.out_off % 8 || a.B <= 0 || a.H <= 0 || a.W <= 0 || a.in_off + a.Cin > a.in_stride ||

@@ -151,6 +151,26 @@ def test_inference_forward_equals_training_forward():
     net.close()
 
 
+def test_fused_edge_convolution_bit_identical(monkeypatch):
+    """Inference computes the edge convolution inside the first 32 -> 32
+    convolution's launch (same operation order); the network output equals the
+    two-launch form bit for bit, on shapes whose 16 x 16 patches do not divide
+    the image or fill the last block of four patches, and a batch of three."""
+    net = RU.Unet(32, 4, seed=11)
+    rng = np.random.default_rng(5)
+    for (B, H, W) in ((3, 40, 56), (1, 24, 200), (2, 136, 72)):
+        x = torch.from_numpy(_images(rng, B, H, W).astype(np.float32)).to(DEV).contiguous()
+        fused = net.forward(x, 1.3).clone()
+        torch.cuda.synchronize()
+        monkeypatch.setenv("RECVIEW_UNET_FUSED", "0")
+        split = net.forward(x, 1.3).clone()
+        torch.cuda.synchronize()
+        monkeypatch.delenv("RECVIEW_UNET_FUSED")
+        assert torch.equal(fused, split), (B, H, W, (fused - split).abs().max().item())
+        assert fused.abs().max().item() > 0
+    net.close()
+
+
 def test_network_gradients_match_oracle():
     net = RU.Unet(32, 4, seed=4)
     rng = np.random.default_rng(1)

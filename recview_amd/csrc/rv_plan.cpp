@@ -67,6 +67,7 @@ struct rv_plan_opaque {
     int mode = rv::BP_LINEAR;
     int store = 0;
     std::vector<SliceState> sl;
+    std::vector<int> filter_order;       // slices grouped by filter class (batched launches)
     std::vector<Group> groups;
     void* d_rows = nullptr;
     float* d_scales = nullptr;
@@ -215,7 +216,9 @@ rv::FilterJob filter_job(rv_plan_t p, size_t s)
 }
 
 // Slices of one filter class (same transform, filter table, row count and
-// input kind) are filtered by one launch, up to kFilterBatch slices each.
+// input kind) are filtered by one launch, up to kFilterBatch slices each; the
+// slices are visited class by class (filter_order), whatever their order in
+// the plan.
 int launch_filters(rv_plan_t p, const void* const* d_sino, hipStream_t st)
 {
     rv::FilterJob head{};
@@ -226,9 +229,8 @@ int launch_filters(rv_plan_t p, const void* const* d_sino, hipStream_t st)
         if (count && err == hipSuccess) err = rv::launch_filter_batch(head, fb, count, st);
         count = 0;
     };
-    for (size_t s = 0; s < p->sl.size(); s++) {
+    for (const int s : p->filter_order) {
         const SliceState& S = p->sl[s];
-        if (S.rows_src >= 0) continue;               // back-projects another slice's rows
         const bool is_float = (S.spec.sino_dtype != RV_SINO_I16);
         const void* src = d_sino[s];
         if (S.offset) {
@@ -451,6 +453,19 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
                 S.fclass = t;
                 break;
             }
+        }
+    }
+    // filter launch order: class by class, each in slice order (slices that
+    // back-project another slice's rows are not filtered)
+    {
+        std::vector<char> placed(nslices, 0);
+        for (int s = 0; s < nslices; s++) {
+            if (placed[s] || p->sl[s].rows_src >= 0) continue;
+            for (int t = s; t < nslices; t++)
+                if (!placed[t] && p->sl[t].rows_src < 0 && p->sl[t].fclass == p->sl[s].fclass) {
+                    p->filter_order.push_back(t);
+                    placed[t] = 1;
+                }
         }
     }
 

@@ -94,13 +94,14 @@ def chunk_for(N, P, mode="linear2", table=None):
 
 
 def _class_mode(mode, count):
-    """The kernel a class of `count` slices of one geometry runs under a plan mode."""
+    """The kernel a class of `count` slices of one geometry runs under a plan mode
+    (an FP16-pair plan runs a class of fewer than four on the FP32-pair kernels)."""
     if mode in ("m32", "h16", "h8", "h4"):
         top = GROUP[mode]
         for m in ("m32", "h16", "h8", "h4"):
             if GROUP[m] <= top and count >= GROUP[m]:
                 return m
-        return "h4"
+        return "linear2" if count >= 2 else "linear"
     if mode == "linear2":
         return "linear2" if count >= 2 else "linear"
     return mode
@@ -111,7 +112,9 @@ def apply(specs, sides, mode, table=None):
     side slice i reconstructs (2W for an offset-axis scan), its projection count the
     rows it reconstructs.  Slices of one geometry form a class; a class takes the
     entry of the kernel its size leads the plan to (32 FP16 slices per item on the
-    matrix cores, sixteen, eight or four FP16 pairs, FP32 pairs or singles).
+    matrix cores, sixteen, eight or four FP16 pairs, FP32 pairs or singles; an
+    FP16-pair plan runs the pairs and singles of its classes smaller than four on
+    the FP32-pair kernels).
 
     A plan runs one launch per (tile, kernel), with the angle chunk of the first
     slice it groups there, so a batch of several shapes gets one (tile, chunk) per

@@ -535,6 +535,11 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
                         : (per_item >= 8 && left >= 8 && tile8) ? 8
                         : (per_item >= 4 && left >= 4) ? 4 : (per_item >= 2 && left >= 2) ? 2 : 1;
             const bool half = (p->mode == rv::BP_LINEAR_H);
+            // FP16 pairs in LDS pay from four slices per item on; the pairs and singles
+            // left over by wider items run the FP32-pair kernels on the 16-bit rows
+            // (2048^2 x 1800: 37.6 vs 45.9 ms for two per item, 52.7 vs 76.0 ms for
+            // one), unless the plan asked for at most two per item
+            const bool narrow_fp32 = half && per_item > 2;
             if (k == 32)
                 for (int e = 0; e < k; e++) {
                     SliceState& T = p->sl[cls[i + e]];
@@ -546,8 +551,8 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
                            : k == 16 ? (int)rv::BP_LINEAR_H16
                            : k == 8 ? (int)rv::BP_LINEAR_H8
                            : k == 4 ? (int)rv::BP_LINEAR_H4
-                           : k == 2 ? (half ? (int)rv::BP_LINEAR_H2 : (int)rv::BP_LINEAR2)
-                                    : p->mode;
+                           : k == 2 ? (half && !narrow_fp32 ? (int)rv::BP_LINEAR_H2 : (int)rv::BP_LINEAR2)
+                                    : (narrow_fp32 ? (int)rv::BP_LINEAR : p->mode);
             const int s = cls[i];
             const auto gk = std::make_pair(p->sl[s].tile, mode);
             if (gidx.find(gk) == gidx.end()) {

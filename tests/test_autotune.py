@@ -52,7 +52,8 @@ def test_apply_sets_specs():
 
 def test_apply_picks_the_kernel_of_each_class_size():
     """Classes of one geometry take the entry of the kernel their size leads to:
-    32 FP16 slices (matrix cores), sixteen, eight, fewer; FP32 pairs, a single."""
+    32 FP16 slices (matrix cores), sixteen, eight, four; FP32 pairs, a single (also
+    in an FP16-pair plan, whose classes of fewer than four run the FP32 kernels)."""
     import numpy as np
     table = dict(TABLE)
     table["h4:2048:1800"] = {"tile": 3, "chunk": 32}
@@ -62,7 +63,7 @@ def test_apply_picks_the_kernel_of_each_class_size():
     table["m32:2048:1800"] = {"tile": 2, "chunk": 12}
     table["h16:2048:1800"] = {"tile": 2, "chunk": 16}
     for mode in ("m32", "h16"):
-        for n, want in ((40, (2, 12)), (16, (2, 16)), (9, (0, 24)), (3, (3, 32))):
+        for n, want in ((40, (2, 12)), (16, (2, 16)), (9, (0, 24)), (5, (3, 32)), (3, (1, 16)), (1, (2, 48))):
             if mode == "h16" and n == 40:
                 want = (2, 16)                     # the plan's limit is sixteen
             specs = autotune.apply(mk(1024.37, n), [2048] * n, mode, table)

@@ -302,7 +302,9 @@ typedef struct {
                           * FP16 pairs; 8 on the 64x64, 32x32 and 32x64 tiles, 16 on the 32x32
                           * and 32x64 tiles, else fewer; 32, opt-in: the matrix-core kernel, which
                           * runs 32x32 tiles whatever the slice's tile, for every 32 slices of one
-                          * geometry, the rest of a class as FP16 pairs) */
+                          * geometry, the rest of a class as FP16 pairs).  With FP16 pairs and more
+                          * than two per item, the pairs and singles left over by a class run the
+                          * FP32-pair kernels on the 16-bit rows (faster for them) */
 } RvOptions;
 
 typedef struct rv_plan_opaque* rv_plan_t;

@@ -97,7 +97,9 @@ class FbpPlan:
         one per item -- and with FP16 pairs (lds_half) sixteen on the 32x64 and
         32x32 tiles, eight on 64x64; or 1, 2, 4, 8, 16, 32, where 4 and up need
         FP16 pairs, 16 the 32x32 or 32x64 tile, and 32 selects the matrix-core
-        kernel for every 32 slices of one geometry)."""
+        kernel for every 32 slices of one geometry; with FP16 pairs and more than
+        two per item, the pairs and singles a class leaves over run the FP32-pair
+        kernels on the 16-bit rows)."""
         self.slices = list(slices)
         L = _lib.lib()
         specs = (_lib.RvSliceSpec * len(self.slices))(*[s.to_c() for s in self.slices])

@@ -93,7 +93,8 @@ constexpr bool is_half_multi(int mode)
 
 constexpr bool is_linear(int mode)
 {
-    return mode == BP_LINEAR || mode == BP_LINEAR_H || mode == BP_LINEAR2 || is_half_multi(mode);
+    return mode == BP_LINEAR || mode == BP_LINEAR_H || mode == BP_LINEAR2 || mode == BP_LINEAR4 ||
+           is_half_multi(mode);
 }
 
 // LDS bytes per staged angle
@@ -199,7 +200,14 @@ __global__ void __launch_bounds__(64 * NW, NW == 8 ? 4 : 1) bp_tiled_kernel(BpLa
     while (true) {
         int item;
         if (L.counter) {
-            if (tid == 0) item_sh = next_work_item(L, probe);
+            // only the kernels bp_deals_runs names deal the list to the XCDs; the
+            // others take positions in list order from the first counter (the probe
+            // loop costs the FP32 pair kernel three VGPRs and a wave per SIMD)
+            if constexpr (MODE == BP_LINEAR_H16) {
+                if (tid == 0) item_sh = next_work_item(L, probe);
+            } else {
+                if (tid == 0) item_sh = (int)atomicAdd(L.counter, 1u);
+            }
             __syncthreads();
             item = __builtin_amdgcn_readfirstlane(item_sh);     // uniform: slice data in scalar registers
             __syncthreads();
@@ -313,16 +321,20 @@ __global__ void __launch_bounds__(64 * NW, NW == 8 ? 4 : 1) bp_tiled_kernel(BpLa
                     origin(a, row, s0);
                     pairs[idx] = linear_pair(S.rows, S.scales, S, row, s0 + j, j);
                 }
-            } else if (MODE == BP_LINEAR2) {
-                // a_j = p_k - j d_k as linear_pair forms it (d = 0 at the last sample)
+            } else if (MODE == BP_LINEAR2 || MODE == BP_LINEAR4) {
+                // a_j = p_k - j d_k as linear_pair forms it (d = 0 at the last sample);
+                // NQ float4 planes per sample (two slices each), plane q at q * SEG
                 float4* quads = (float4*)data;
                 const float4* packed = (const float4*)S.rows;
                 auto stage = [&](int a, int j, int row, int s0) {
                     const int k = s0 + j;
-                    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-                    if (row >= 0 && k >= 0 && k < N) v = packed[(long long)row * S.row_stride + k];
                     const float fj = -(float)j;
-                    quads[a * SEG + j] = make_float4(fmaf(fj, v.y, v.x), v.y, fmaf(fj, v.w, v.z), v.w);
+#pragma unroll
+                    for (int q = 0; q < NQ; q++) {
+                        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+                        if (row >= 0 && k >= 0 && k < N) v = packed[((long long)row * S.row_stride + k) * NQ + q];
+                        quads[(a * NQ + q) * SEG + j] = make_float4(fmaf(fj, v.y, v.x), v.y, fmaf(fj, v.w, v.z), v.w);
+                    }
                 };
                 if (C::T % chunk == 0) {
                     // threads of one angle: its row and segment origin read once per
@@ -460,7 +472,7 @@ __global__ void __launch_bounds__(64 * NW, NW == 8 ? 4 : 1) bp_tiled_kernel(BpLa
                         }
                     }
                 });
-            } else if (MODE == BP_LINEAR2) {
+            } else if (MODE == BP_LINEAR2 || MODE == BP_LINEAR4) {
                 const float4* quads = (const float4*)data;
                 auto angle = [&](int a, const float4 tb) {
                     const float4* seg = quads + a * SEG * NQ;
@@ -710,16 +722,18 @@ __global__ void __launch_bounds__(1024) pack_scale_kernel(const PackJob* jobs)
     }
 }
 
-// KIND 0: FP32 pairs of two slices; KIND 2 / 4 / 8 / 16: FP16 pairs of that many
-// slices.  Blocks stride over rows, threads over samples (no index division;
-// each row's scales read once); every slice's loads are issued together.
+// KIND 0: FP32 pairs of two slices, 3: FP32 pairs of four slices (two float4 per
+// sample); KIND 2 / 4 / 8 / 16: FP16 pairs of that many slices.  Blocks stride
+// over rows, threads over samples (no index division; each row's scales read
+// once); every slice's loads are issued together.
 template <int KIND, int STORE>
 __global__ void __launch_bounds__(256) pack_kernel(const PackJob* jobs)
 {
-    constexpr int NS = KIND == 0 ? 2 : KIND;
+    constexpr bool FP32 = KIND == 0 || KIND == 3;
+    constexpr int NS = KIND == 0 ? 2 : KIND == 3 ? 4 : KIND;
     const PackJob J = jobs[blockIdx.y];
     const int N = J.N;
-    const float inv = KIND ? 1.f / *J.gscale : 1.f;
+    const float inv = FP32 ? 1.f : 1.f / *J.gscale;
     for (int row = blockIdx.x; row < J.nrows; row += gridDim.x) {
         float mul[NS];
 #pragma unroll
@@ -735,17 +749,19 @@ __global__ void __launch_bounds__(256) pack_kernel(const PackJob* jobs)
                 p0[e] = stored_val<STORE>(J.rows[e], i);
                 p1[e] = stored_val<STORE>(J.rows[e], i1);
             }
-            if (KIND == 0) {
+            if (FP32) {
                 // the FP32 values and difference of the single-slice staging
-                float v[4];
+                float v[2 * NS];
 #pragma unroll
-                for (int e = 0; e < 2; e++) {
+                for (int e = 0; e < NS; e++) {
                     const float a0 = STORE ? p0[e] * mul[e] : p0[e];
                     const float a1 = STORE ? p1[e] * mul[e] : p1[e];
                     v[2 * e] = a0;
                     v[2 * e + 1] = last ? 0.f : a1 - a0;
                 }
-                ((float4*)J.out)[i] = make_float4(v[0], v[1], v[2], v[3]);
+#pragma unroll
+                for (int q = 0; q < NS / 2; q++)
+                    ((float4*)J.out)[i * (NS / 2) + q] = make_float4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
             } else {
                 uint32_t w[NS];
 #pragma unroll
@@ -1075,6 +1091,7 @@ void launch_pack(const PackJob* d_jobs, int njobs, int kind, int store, int max_
         return;
     }
     if (kind == 0) launch_pack_kind<0>(d_jobs, njobs, store, max_rows, stream);
+    else if (kind == 3) launch_pack_kind<3>(d_jobs, njobs, store, max_rows, stream);
     else if (kind == 2) launch_pack_kind<2>(d_jobs, njobs, store, max_rows, stream);
     else if (kind == 4) launch_pack_kind<4>(d_jobs, njobs, store, max_rows, stream);
     else if (kind == 8) launch_pack_kind<8>(d_jobs, njobs, store, max_rows, stream);
@@ -1122,6 +1139,7 @@ int bp_default_chunk(int mode, int tile)
 {
     if (mode == BP_LINEAR) return (tile == 2) ? 64 : 32;
     if (mode == BP_LINEAR2) return (tile == 2) ? 32 : 16;
+    if (mode == BP_LINEAR4) return (tile == 2) ? 16 : 8;
     if (mode == BP_LINEAR_H) return (tile == 2) ? 64 : 32;
     if (mode == BP_LINEAR_H2) return (tile == 2) ? 64 : 32;
     if (mode == BP_LINEAR_H4) return (tile == 2) ? 32 : 16;
@@ -1158,7 +1176,7 @@ int bp_threads(int mode, int tile)
 // it spills and the compiler's schedule changes), and singles are not measured.
 bool bp_has_tgeom(int mode)
 {
-    return mode == BP_LINEAR2;
+    return mode == BP_LINEAR2 || mode == BP_LINEAR4;
 }
 
 // Persistent launches that deal their work list to the XCDs in runs of
@@ -1172,7 +1190,7 @@ bool bp_has_tgeom(int mode)
 // loses 0.7%, so those keep one queue (a single run of the whole list).
 bool bp_deals_runs(int mode)
 {
-    return mode == BP_LINEAR_H16 || mode == BP_LINEAR_M32;
+    return mode == BP_LINEAR_H16 || mode == BP_LINEAR_M32;      // bp_tiled_kernel and bp_m32_kernel agree
 }
 
 template <int MODE, bool TG = false>
@@ -1200,8 +1218,10 @@ static const void* kernel_for(int mode, int tile, bool tg = false)
 {
     if (mode == BP_LINEAR_M32) return tile == 2 ? (const void*)&bp_m32_kernel : nullptr;
     if (tg && mode == BP_LINEAR2) return kernel_ptr<BP_LINEAR2, true>(tile);
+    if (tg && mode == BP_LINEAR4) return kernel_ptr<BP_LINEAR4, true>(tile);
     if (mode == BP_LINEAR) return kernel_ptr<BP_LINEAR>(tile);
     if (mode == BP_LINEAR2) return kernel_ptr<BP_LINEAR2>(tile);
+    if (mode == BP_LINEAR4) return kernel_ptr<BP_LINEAR4>(tile);
     if (mode == BP_LINEAR_H) return kernel_ptr<BP_LINEAR_H>(tile);
     if (mode == BP_LINEAR_H2) return kernel_ptr<BP_LINEAR_H2>(tile);
     if (mode == BP_LINEAR_H4) return kernel_ptr<BP_LINEAR_H4>(tile);
@@ -1276,7 +1296,7 @@ template <int MODE>
 static void launch_mode(const BpLaunch& L, int grid, hipStream_t stream)
 {
     const size_t lds = bp_lds_bytes(MODE, L.tile, L.chunk);
-    if constexpr (MODE == BP_LINEAR2) {
+    if constexpr (MODE == BP_LINEAR2 || MODE == BP_LINEAR4) {
         if (L.tgeom) {
             launch_mode_tg<MODE, true>(L, grid, lds, stream);
             return;
@@ -1307,6 +1327,7 @@ void launch_bp(const BpLaunch& L, int grid, hipStream_t stream)
     }
     if (L.mode == BP_LINEAR) launch_mode<BP_LINEAR>(L, grid, stream);
     else if (L.mode == BP_LINEAR2) launch_mode<BP_LINEAR2>(L, grid, stream);
+    else if (L.mode == BP_LINEAR4) launch_mode<BP_LINEAR4>(L, grid, stream);
     else if (L.mode == BP_LINEAR_H) launch_mode<BP_LINEAR_H>(L, grid, stream);
     else if (L.mode == BP_LINEAR_H2) launch_mode<BP_LINEAR_H2>(L, grid, stream);
     else if (L.mode == BP_LINEAR_H4) launch_mode<BP_LINEAR_H4>(L, grid, stream);

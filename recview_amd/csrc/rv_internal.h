@@ -116,17 +116,20 @@ inline int output_irad2(int N, double center, int iInterpolation)
 }
 
 // ------------------------------------------------------- back-projection
-// BP_LINEAR2: BP_LINEAR over two slices of identical geometry per work item;
+// BP_LINEAR2: BP_LINEAR over two slices of identical geometry per work item
+// (BP_LINEAR4: four);
 // BP_LINEAR_H2 / H4 / H8 / H16: BP_LINEAR_H over two / four / eight / sixteen;
 // BP_LINEAR_M32: 32 FP16 slices per work item, the updates done as matrix
 // products on the MFMA units (the others in BpSlice::rows_x / scales_x / out_x).
 enum BpMode {
     BP_LINEAR = 0, BP_LUT_F32 = 1, BP_LUT_I32 = 2, BP_LINEAR_H = 3, BP_LINEAR2 = 4,
-    BP_LINEAR_H2 = 5, BP_LINEAR_H4 = 6, BP_LINEAR_H8 = 7, BP_LINEAR_H16 = 8, BP_LINEAR_M32 = 9
+    BP_LINEAR_H2 = 5, BP_LINEAR_H4 = 6, BP_LINEAR_H8 = 7, BP_LINEAR_H16 = 8, BP_LINEAR_M32 = 9,
+    BP_LINEAR4 = 10
 };
 constexpr int bp_slices_per_item(int mode)
 {
-    return (mode == 4 || mode == 5) ? 2 : mode == 6 ? 4 : mode == 7 ? 8 : mode == 8 ? 16 : mode == 9 ? 32 : 1;
+    return (mode == 4 || mode == 5) ? 2 : (mode == 6 || mode == 10) ? 4 : mode == 7 ? 8 : mode == 8 ? 16
+         : mode == 9 ? 32 : 1;
 }
 constexpr int BP_MAX_SLICES_PER_ITEM = 32;
 // BP_LINEAR_M32 packed rows: per angle row, (N + 7) / 8 chunks of eight samples,
@@ -223,6 +226,7 @@ void launch_ptr_patch(const PtrPatch& P, hipStream_t stream);
 // 16-byte load:
 //   BP_LINEAR2      float4 (p_k, d_k, p'_k, d'_k), d_k = p_{k+1} - p_k (0 at the
 //                   last sample);
+//   BP_LINEAR4      two such float4 per sample, slices 0-1 then 2-3 ([row][k][2]);
 //   BP_LINEAR_H2/H4/H8/H16 two / four / eight / sixteen FP16 pairs (p_k, d_k), values multiplied by
 //                   row scale / item scale, where the item scale is the largest
 //                   row scale of the slices over their angle list (a power of two);
@@ -241,8 +245,9 @@ struct PackJob {
     float* gscale;          // FP16 pairs: the item scale, written by the scale pass
 };
 // Item scales of FP16-pair jobs (jobs of other kinds are skipped), then the
-// packing of njobs jobs of one kind (0: FP32 pairs; 2 / 4 / 8 / 16: FP16 pairs of
-// that many slices; 32: the BP_LINEAR_M32 chunks) and one store, in stream order.
+// packing of njobs jobs of one kind (0: FP32 pairs; 3: FP32, four slices; 2 / 4 /
+// 8 / 16: FP16 pairs of that many slices; 32: the BP_LINEAR_M32 chunks) and one
+// store, in stream order.
 void launch_pack_scales(const PackJob* d_jobs, int njobs, hipStream_t stream);
 void launch_pack(const PackJob* d_jobs, int njobs, int kind, int store, int max_rows, hipStream_t stream);
 

@@ -482,8 +482,10 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
     // 64x64; measured per 2048^2 x 1800 slice: FP32 pairs 0.630 ms, FP16 pairs
     // 0.367 / 0.381 / 0.492 ms with sixteen (32x64) / eight (64x64) / four per
     // item).  32 per item (opt-in) runs the matrix-core kernel: 0.343 ms.
-    const int max_per_item = (p->mode == rv::BP_LINEAR_H) ? 32 : 2;
-    const int per_item = std::min(opt->slices_per_item == 0 ? 16 : opt->slices_per_item, max_per_item);
+    // FP32 rows: two slices per item by default, four when asked; FP16 pairs: sixteen
+    const int max_per_item = (p->mode == rv::BP_LINEAR_H) ? 32 : 4;
+    const int per_item = std::min(opt->slices_per_item == 0 ? (p->mode == rv::BP_LINEAR_H ? 16 : 2)
+                                                            : opt->slices_per_item, max_per_item);
     if ((p->mode == rv::BP_LINEAR || p->mode == rv::BP_LINEAR_H) && per_item > 1) {
         auto same = [&](int a, int b) {
             const SliceState& A = p->sl[a];
@@ -550,7 +552,7 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
             const int mode = k == 32 ? (int)rv::BP_LINEAR_M32
                            : k == 16 ? (int)rv::BP_LINEAR_H16
                            : k == 8 ? (int)rv::BP_LINEAR_H8
-                           : k == 4 ? (int)rv::BP_LINEAR_H4
+                           : k == 4 ? (half ? (int)rv::BP_LINEAR_H4 : (int)rv::BP_LINEAR4)
                            : k == 2 ? (half && !narrow_fp32 ? (int)rv::BP_LINEAR_H2 : (int)rv::BP_LINEAR2)
                                     : (narrow_fp32 ? (int)rv::BP_LINEAR : p->mode);
             const int s = cls[i];
@@ -602,6 +604,7 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
     // chunks of eight samples
     auto packed_size = [](int mode) -> size_t {
         return mode == rv::BP_LINEAR_M32 ? 64 : mode == rv::BP_LINEAR_H16 ? 64 : mode == rv::BP_LINEAR_H8 ? 32
+             : mode == rv::BP_LINEAR4 ? 32
              : mode == rv::BP_LINEAR2 || mode == rv::BP_LINEAR_H4 ? 16 : mode == rv::BP_LINEAR_H2 ? 8 : 0;
     };
     auto packed_row = [&](int mode, int N) -> size_t {     // bytes per angle row
@@ -613,7 +616,7 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
         if (packed_size(G.mode))
             for (int s : G.slices) {
                 packed_bytes += align_up((size_t)p->sl[s].nang * packed_row(G.mode, p->sl[s].g.N), 256);
-                n_half += G.mode != rv::BP_LINEAR2;
+                n_half += G.mode != rv::BP_LINEAR2 && G.mode != rv::BP_LINEAR4;
             }
     std::vector<rv::PackJob> pack_jobs;
     if (packed_bytes &&
@@ -727,13 +730,14 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
                 J.ang = B.ang;
                 J.nang = S.nang;
                 J.ns = ns;
-                J.half = G.mode != rv::BP_LINEAR2;
+                J.half = G.mode != rv::BP_LINEAR2 && G.mode != rv::BP_LINEAR4;
                 J.store = p->store;
                 J.nrows = S.nang;
                 J.N = S.g.N;
                 J.out = p->d_packed + packed_off;
                 J.gscale = J.half ? p->d_gscale + half_idx++ : nullptr;
-                if (q == 0) p->pack_ranges.push_back({(int)pack_jobs.size(), 0, J.half ? ns : 0, 0});
+                if (q == 0)
+                    p->pack_ranges.push_back({(int)pack_jobs.size(), 0, J.half ? ns : ns == 4 ? 3 : 0, 0});
                 auto& R = p->pack_ranges.back();
                 R.count++;
                 R.rows = std::max(R.rows, S.nang);

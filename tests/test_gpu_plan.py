@@ -157,9 +157,10 @@ def test_plan_batch_uneven_equals_single():
 @pytest.mark.parametrize("store,persistent,normalize", [("f32", True, False), ("f32", False, True),
                                                          ("f16", True, False), ("bf16", False, False)])
 def test_plan_slice_groups_identical(store, persistent, normalize):
-    """Slices of identical geometry back-projected two per work item give exactly
-    the images of one slice per item: five equal-geometry slices (two pairs and
-    one left over) with different data, plus two slices of other geometries."""
+    """Slices of identical geometry back-projected two or four per work item give
+    exactly the images of one slice per item: five equal-geometry slices (two pairs
+    and one left over, or four and one) with different data, plus two slices of
+    other geometries; four per item on every tile configuration."""
     print(f"case store={store} persistent={persistent} normalize={normalize}", flush=True)
     specs, sinos = [], []
     for i in range(5):
@@ -171,11 +172,16 @@ def test_plan_slice_groups_identical(store, persistent, normalize):
         specs.append(fbp.SliceSpec(W, fdeg, c))
         sinos.append(torch.from_numpy(s))
     kw = dict(store=store, persistent=persistent, normalize=normalize)
-    _, single = _run(specs, sinos, slices_per_item=1, **kw)
-    for k in (0, 2):
-        _, grouped = _run(specs, sinos, slices_per_item=k, **kw)
+    # each tile configuration against one slice per item on the same tiles (the
+    # FP32 coordinate is formed relative to the tile origin)
+    singles = {}
+    for k, tile in ((0, 0), (2, 0), (4, 0), (4, 1), (4, 2), (4, 3), (4, 4)):
+        if tile not in singles:
+            singles[tile] = _run(specs, sinos, slices_per_item=1, tile=tile, **kw)[1]
+        _, grouped = _run(specs, sinos, slices_per_item=k, tile=tile, **kw)
         for i in range(len(specs)):
-            assert np.array_equal(grouped[i], single[i]), (k, i)
+            assert np.array_equal(grouped[i], singles[tile][i]), (k, tile, i)
+    single = singles[0]
     # the data of each slice stays with its own image
     assert not np.array_equal(single[0], single[1])
 
@@ -692,20 +698,20 @@ def test_plan_geometry_tables_equal_per_chunk_geometry(monkeypatch):
     """FP32 pairs read their tile geometry from tables built at plan creation; with
     the tables switched off (RECVIEW_GEOMETRY_TABLES=0, the path plans take beyond
     the table budget) the kernel forms it per chunk.  Both give the same images bit
-    for bit: four slices of one geometry (two pairs) on each tile configuration,
-    persistent and one block per item, and a pair and a single of another geometry."""
+    for bit: four slices of one geometry (two pairs, or one item of four) on each
+    tile configuration, persistent and one block per item, and a pair and a single
+    of another geometry."""
     specs, sinos = [], []
     for i in range(7):
         W, P = (160, 97) if i < 4 else (96, 60)
         s, fdeg, c = _sino(W, P, 2400 + i)
         specs.append(fbp.SliceSpec(W, fdeg, c))
         sinos.append(torch.from_numpy(s))
-    for tile in range(5):
-        for persistent in (True, False):
-            kw = dict(tile=tile, persistent=persistent)
-            monkeypatch.setenv("RECVIEW_GEOMETRY_TABLES", "0")
-            _, per_chunk = _run(specs, sinos, **kw)
-            monkeypatch.delenv("RECVIEW_GEOMETRY_TABLES")
-            _, tables = _run(specs, sinos, **kw)
-            for i in range(len(specs)):
-                assert np.array_equal(per_chunk[i], tables[i]), (tile, persistent, i)
+    for tile, persistent, spi in [(t, p, k) for t in range(5) for p in (True, False) for k in (0, 4)]:
+        kw = dict(tile=tile, persistent=persistent, slices_per_item=spi)
+        monkeypatch.setenv("RECVIEW_GEOMETRY_TABLES", "0")
+        _, per_chunk = _run(specs, sinos, **kw)
+        monkeypatch.delenv("RECVIEW_GEOMETRY_TABLES")
+        _, tables = _run(specs, sinos, **kw)
+        for i in range(len(specs)):
+            assert np.array_equal(per_chunk[i], tables[i]), (tile, persistent, spi, i)

@@ -461,8 +461,8 @@ def main():
     ap.add_argument("--lds-half", type=int, default=0, help="FP16 pairs in LDS for 16-bit stores")
     ap.add_argument("--persistent", type=int, default=1)
     ap.add_argument("--slices-per-item", type=int, default=0,
-                    help="LINEAR: slices of identical geometry per work item (0 = two with FP32 "
-                         "pairs, eight with FP16 pairs; or 1, 2, 4, 8)")
+                    help="LINEAR: slices of identical geometry per work item (0 = four with FP32 "
+                         "rows, sixteen with FP16 pairs; or 1, 2, 4, 8, 16, 32)")
     ap.add_argument("--graph", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--texture", type=float, default=0.02,
@@ -585,9 +585,11 @@ def main():
 
     traffic = None
     if args.workload == "c2" and args.interp == "linear" and args.store == "f32" and S == 64 \
-            and args.proj == 1800 and args.width == 2048 and args.slices_per_item in (0, 2) \
+            and args.proj == 1800 and args.width == 2048 and args.slices_per_item in (0, 2, 4) \
             and args.tile in (-1, 0) and args.chunk == 0:
-        traffic = read_traffic("bp_linear_c2_traffic.md")
+        # the FP32 default runs four slices per work item; two per item has its own figure
+        traffic = read_traffic("bp_linear_c2_traffic.md" if args.slices_per_item == 2
+                               else "bp_linear4_c2_traffic.md")
     config = {"workload": desc, "slices": S, "interp": args.interp, "store": args.store,
               "persistent": bool(args.persistent), "graph": bool(args.graph),
               "lds_half": bool(args.lds_half), "slices_per_item": args.slices_per_item, "updates": updates}

@@ -2,7 +2,8 @@
 
 The plan picks one back-projection kernel per work-item kind (the mode):
 
-  linear2    FP32 filtered rows, two slices of one geometry per work item (default)
+  linear4    FP32 filtered rows, four slices of one geometry per work item (default)
+  linear2    FP32, two slices per work item (a class's pair left over by fours)
   linear     FP32, one slice per work item (slices without a partner of equal geometry)
   m32        FP16 filtered rows, 32 slices per work item on the matrix cores (32x32
              tiles; opt-in, slices_per_item = 32)
@@ -29,14 +30,15 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE = os.path.join(HERE, "tuned_tiles.json")
 
-MODES = ("linear2", "linear", "m32", "h16", "h8", "h4", "lut16_i32")
-TILES = {"linear2": (0, 1, 2, 3, 4), "linear": (0, 1, 2, 3, 4), "m32": (2,), "h16": (2, 4), "h8": (0, 2, 4),
-         "h4": (0, 1, 2, 3, 4), "lut16_i32": (0, 1, 2, 3, 4)}
-CHUNKS = {"linear2": (8, 16, 24, 32), "linear": (16, 32, 48), "m32": (8, 12, 16), "h16": (8, 12, 16),
-          "h8": (16, 24, 32), "h4": (16, 24, 32), "lut16_i32": (16, 32, 48)}
+MODES = ("linear4", "linear2", "linear", "m32", "h16", "h8", "h4", "lut16_i32")
+TILES = {"linear4": (0, 1, 2, 3, 4), "linear2": (0, 1, 2, 3, 4), "linear": (0, 1, 2, 3, 4), "m32": (2,),
+         "h16": (2, 4), "h8": (0, 2, 4), "h4": (0, 1, 2, 3, 4), "lut16_i32": (0, 1, 2, 3, 4)}
+CHUNKS = {"linear4": (8, 12, 16), "linear2": (8, 16, 24, 32), "linear": (16, 32, 48), "m32": (8, 12, 16),
+          "h16": (8, 12, 16), "h8": (16, 24, 32), "h4": (16, 24, 32), "lut16_i32": (16, 32, 48)}
 # plan options of each mode (the slices of a tuning batch share one geometry, so
 # they group as the mode says)
 PLAN_KW = {
+    "linear4": dict(interp="linear", store="f32", slices_per_item=4),
     "linear2": dict(interp="linear", store="f32", slices_per_item=2),
     "linear": dict(interp="linear", store="f32", slices_per_item=1),
     "m32": dict(interp="linear", store="f16", lds_half=True, slices_per_item=32),
@@ -45,7 +47,7 @@ PLAN_KW = {
     "h4": dict(interp="linear", store="f16", lds_half=True, slices_per_item=4),
     "lut16_i32": dict(interp="lut16_i32"),
 }
-GROUP = {"linear2": 2, "linear": 1, "m32": 32, "h16": 16, "h8": 8, "h4": 4, "lut16_i32": 1}
+GROUP = {"linear4": 4, "linear2": 2, "linear": 1, "m32": 32, "h16": 16, "h8": 8, "h4": 4, "lut16_i32": 1}
 
 
 def mode_of(interp="linear", store="f32", lds_half=False, slices_per_item=0):
@@ -57,7 +59,7 @@ def mode_of(interp="linear", store="f32", lds_half=False, slices_per_item=0):
     if store != "f32" and lds_half:
         return {4: "h4", 8: "h8", 32: "m32"}.get(slices_per_item, "h16")
     if store == "f32":
-        return "linear" if slices_per_item == 1 else "linear2"
+        return {1: "linear", 2: "linear2"}.get(slices_per_item, "linear4")
     return None
 
 
@@ -102,6 +104,8 @@ def _class_mode(mode, count):
             if GROUP[m] <= top and count >= GROUP[m]:
                 return m
         return "linear2" if count >= 2 else "linear"
+    if mode == "linear4":
+        return "linear4" if count >= 4 else "linear2" if count >= 2 else "linear"
     if mode == "linear2":
         return "linear2" if count >= 2 else "linear"
     return mode
@@ -112,7 +116,7 @@ def apply(specs, sides, mode, table=None):
     side slice i reconstructs (2W for an offset-axis scan), its projection count the
     rows it reconstructs.  Slices of one geometry form a class; a class takes the
     entry of the kernel its size leads the plan to (32 FP16 slices per item on the
-    matrix cores, sixteen, eight or four FP16 pairs, FP32 pairs or singles; an
+    matrix cores, sixteen, eight or four FP16 pairs, four or two FP32 slices, singles; an
     FP16-pair plan runs the pairs and singles of its classes smaller than four on
     the FP32-pair kernels).
 

@@ -482,9 +482,10 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
     // 64x64; measured per 2048^2 x 1800 slice: FP32 pairs 0.630 ms, FP16 pairs
     // 0.367 / 0.381 / 0.492 ms with sixteen (32x64) / eight (64x64) / four per
     // item).  32 per item (opt-in) runs the matrix-core kernel: 0.343 ms.
-    // FP32 rows: two slices per item by default, four when asked; FP16 pairs: sixteen
+    // FP32 rows: four slices per item by default (2048^2 x 1800, 64 slices: 36.3 vs 37.2
+    // ms for pairs; a class's leftovers run as a pair and a single); FP16 pairs: sixteen
     const int max_per_item = (p->mode == rv::BP_LINEAR_H) ? 32 : 4;
-    const int per_item = std::min(opt->slices_per_item == 0 ? (p->mode == rv::BP_LINEAR_H ? 16 : 2)
+    const int per_item = std::min(opt->slices_per_item == 0 ? (p->mode == rv::BP_LINEAR_H ? 16 : 4)
                                                             : opt->slices_per_item, max_per_item);
     if ((p->mode == rv::BP_LINEAR || p->mode == rv::BP_LINEAR_H) && per_item > 1) {
         auto same = [&](int a, int b) {

@@ -27,7 +27,9 @@ def test_lookup_per_mode():
 
 
 def test_mode_of_plan_options():
-    assert autotune.mode_of("linear", "f32") == "linear2"
+    assert autotune.mode_of("linear", "f32") == "linear4"
+    assert autotune.mode_of("linear", "f32", slices_per_item=4) == "linear4"
+    assert autotune.mode_of("linear", "f32", slices_per_item=2) == "linear2"
     assert autotune.mode_of("linear", "f32", slices_per_item=1) == "linear"
     assert autotune.mode_of("linear", "f16", lds_half=True) == "h16"
     assert autotune.mode_of("linear", "f16", lds_half=True, slices_per_item=32) == "m32"
@@ -70,6 +72,10 @@ def test_apply_picks_the_kernel_of_each_class_size():
             assert (specs[0].tile, specs[0].chunk) == want, (mode, n)
     specs = autotune.apply(mk(1024.37, 2) + mk(1030.0, 1), [2048] * 3, "linear2", table)
     assert [(s.tile, s.chunk) for s in specs] == [(1, 16), (1, 16), (2, 48)]
+    # FP32 default (four per item): a class of four or more takes the four-slice entry
+    table["linear4:2048:1800"] = {"tile": 0, "chunk": 8}
+    specs = autotune.apply(mk(1024.37, 5) + mk(1030.0, 3) + mk(1040.0, 1), [2048] * 9, "linear4", table)
+    assert [(s.tile, s.chunk) for s in specs] == [(0, 8)] * 5 + [(1, 16)] * 3 + [(2, 48)]
 
 
 def test_apply_one_configuration_per_kernel_in_a_mixed_batch():

@@ -294,13 +294,14 @@ typedef struct {
                           * the reference CPU filter); 0 (default) filters two rows per transform with
                           * the symmetrised filter, the same operator at half the cost */
     int slices_per_item; /* LINEAR modes: slices of identical geometry back-projected together, sharing
-                          * each pixel's coordinate and LDS reads: 0 (default) = two per work item
-                          * with FP32 pairs in LDS (results identical to one per item), sixteen with
-                          * FP16 pairs (lds_half: one FP16 dot product per update and one
-                          * power-of-two scale per work item; eight on the 64x64 tile, which has
-                          * no sixteen-slice kernel); or 1, 2, 4, 8, 16, 32 (4 and up only with
-                          * FP16 pairs; 8 on the 64x64, 32x32 and 32x64 tiles, 16 on the 32x32
-                          * and 32x64 tiles, else fewer; 32, opt-in: the matrix-core kernel, which
+                          * each pixel's coordinate and LDS reads: 0 (default) = four per work item
+                          * with FP32 values in LDS (results identical to one per item; a class's
+                          * leftovers as a pair and a single), sixteen with FP16 pairs (lds_half:
+                          * one FP16 dot product per update and one power-of-two scale per work
+                          * item; eight on the 64x64 tile, which has no sixteen-slice kernel); or
+                          * 1, 2, 4, 8, 16, 32 (at most 4 with FP32 values in LDS; 8 on the 64x64,
+                          * 32x32 and 32x64 tiles, 16 on the 32x32 and 32x64 tiles, else fewer;
+                          * 32, opt-in: the matrix-core kernel, which
                           * runs 32x32 tiles whatever the slice's tile, for every 32 slices of one
                           * geometry, the rest of a class as FP16 pairs).  With FP16 pairs and more
                           * than two per item, the pairs and singles left over by a class run the

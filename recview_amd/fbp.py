@@ -93,10 +93,11 @@ class FbpPlan:
         instead of raw back-projected sums.  exact_filter: LINEAR modes keep CFft's
         operation order (bit-identical filtered rows) instead of two rows per transform.
         slices_per_item: LINEAR modes back-project slices of identical geometry
-        together (0: two per work item with FP32 LDS pairs -- the same results as
-        one per item -- and with FP16 pairs (lds_half) sixteen on the 32x64 and
-        32x32 tiles, eight on 64x64; or 1, 2, 4, 8, 16, 32, where 4 and up need
-        FP16 pairs, 16 the 32x32 or 32x64 tile, and 32 selects the matrix-core
+        together (0: four per work item with FP32 values in LDS -- the same results
+        as one per item; a class's leftovers as a pair and a single -- and with FP16
+        pairs (lds_half) sixteen on the 32x64 and 32x32 tiles, eight on 64x64; or 1,
+        2, 4, 8, 16, 32, where 8 and up need FP16 pairs, 16 the 32x32 or 32x64
+        tile, and 32 selects the matrix-core
         kernel for every 32 slices of one geometry; with FP16 pairs and more than
         two per item, the pairs and singles a class leaves over run the FP32-pair
         kernels on the 16-bit rows)."""

@@ -1184,10 +1184,12 @@ bool bp_has_tgeom(int mode)
 // the L2 hit rate of the FP32 pair kernel from 36% to 61% and of the
 // sixteen-slice FP16 kernel from 55% to 75% (fetched bytes -40% and -48%), but
 // only the wide FP16 items gain time from it (sixteen per item 23.5 -> 23.15 ms,
-// 32 on the matrix cores 21.9 -> 21.7 ms); the FP32 pairs lose 1.3-4% with
-// runs of 32 to 1024 tiles (LDS-bound, they gain nothing from the L2 and lose
-// to the concentrated reads), and the mixed eight/four/two-slice FP16 batch
-// loses 0.7%, so those keep one queue (a single run of the whole list).
+// 32 on the matrix cores 21.9 -> 21.7 ms).  The FP32 pairs lost 1.3-4% with runs
+// of 32 to 1024 tiles, but those builds also carried the dealing's three VGPRs,
+// which cost the pair kernel a wave per SIMD; the four-slice kernel, at four
+// waves either way, measured neutral (64-slice stack +0.4%, uneven batch -0.3%);
+// the mixed eight/four/two-slice FP16 batch lost 0.7%.  Those keep one queue (a
+// single run of the whole list).
 bool bp_deals_runs(int mode)
 {
     return mode == BP_LINEAR_H16 || mode == BP_LINEAR_M32;      // bp_tiled_kernel and bp_m32_kernel agree

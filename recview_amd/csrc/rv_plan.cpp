@@ -470,20 +470,19 @@ int rvPlanCreate(rv_plan_t* out, int device, const RvSliceSpec* specs, int nslic
     }
 
     // LINEAR modes: slices whose back-projection geometry is identical (angle
-    // table, sizes, tile) share work items -- two at a time, or up to four with
-    // FP16 pairs in LDS (classes of equal geometry, in slice order).
+    // table, sizes, tile) share work items -- up to four with FP32 values in LDS,
+    // up to 32 with FP16 pairs (classes of equal geometry, in slice order).
     std::vector<std::vector<int>> classes;
     if (opt->slices_per_item != 0 && opt->slices_per_item != 1 && opt->slices_per_item != 2 &&
         opt->slices_per_item != 4 && opt->slices_per_item != 8 && opt->slices_per_item != 16 &&
         opt->slices_per_item != 32)
         return fail(p, "slices_per_item must be 0, 1, 2, 4, 8, 16 or 32");
-    // default: two FP32 slices or sixteen FP16-pair slices per item, all on the
+    // default: four FP32 slices or sixteen FP16-pair slices per item, all on the
     // vector ALU (packed rows; sixteen form on the 32x64 and 32x32 tiles, eight on
-    // 64x64; measured per 2048^2 x 1800 slice: FP32 pairs 0.630 ms, FP16 pairs
-    // 0.367 / 0.381 / 0.492 ms with sixteen (32x64) / eight (64x64) / four per
-    // item).  32 per item (opt-in) runs the matrix-core kernel: 0.343 ms.
-    // FP32 rows: four slices per item by default (2048^2 x 1800, 64 slices: 36.3 vs 37.2
-    // ms for pairs; a class's leftovers run as a pair and a single); FP16 pairs: sixteen
+    // 64x64).  Measured per 2048^2 x 1800 slice (64-slice stack): FP32 0.57 ms with
+    // four per item, 0.58 ms with two; FP16 pairs 0.367 / 0.381 / 0.492 ms with
+    // sixteen (32x64) / eight (64x64) / four per item.  A class's leftovers run as
+    // a pair and a single.  32 per item (opt-in) runs the matrix-core kernel: 0.343 ms.
     const int max_per_item = (p->mode == rv::BP_LINEAR_H) ? 32 : 4;
     const int per_item = std::min(opt->slices_per_item == 0 ? (p->mode == rv::BP_LINEAR_H ? 16 : 4)
                                                             : opt->slices_per_item, max_per_item);
